@@ -1,0 +1,215 @@
+/* wtc.h -- C ABI of the MI355X-native wedge-parallel triangle counter.
+ *
+ * The reference (jeffreyspaan/wedge-parallel-triangle-counting) has no library
+ * interface: its seam is the internal call tc_GPU() in tc.cu, fed by the C++
+ * reader and preprocessing in the same file.  This header exports those seams
+ * as plain C functions (plain pointers and sizes, status codes instead of
+ * exit()), plus the device-resident, batched and served paths that the
+ * reference does not have.
+ *
+ * Conventions
+ *   - Every function returns WTC_OK (0) or a negative WTC_ERR_* code and never
+ *     exits the process; wtc_strerror() turns a code into text.
+ *   - Host graphs are owned by the caller and only read during a call.
+ *   - Device memory is owned by the library (contexts, device graphs, batches).
+ *   - One wtc_ctx per host thread; a context is not safe to share without a
+ *     lock.  The GPU is device `device` of the HIP runtime.
+ */
+#ifndef WTC_H
+#define WTC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status codes */
+#define WTC_OK 0
+#define WTC_ERR_ARG (-1)       /* bad argument (e.g. spread 0, a % 32 != 0)   */
+#define WTC_ERR_IO (-2)        /* cannot open or read the graph file          */
+#define WTC_ERR_FORMAT (-3)    /* no edge line / vertex id range overflow     */
+#define WTC_ERR_ZERO_ID (-4)   /* id 0 in a 1-indexed file (tc.cu:599-602)    */
+#define WTC_ERR_NOMEM (-5)     /* host allocation failed                      */
+#define WTC_ERR_HIP (-6)       /* HIP runtime error (device alloc, launch...) */
+#define WTC_ERR_RANGE (-7)     /* graph too large for the 32-bit path         */
+#define WTC_ERR_NODEV (-8)     /* no HIP device / kernels not loadable        */
+#define WTC_ERR_ADMIT (-9)     /* serving: request exceeds the HBM budget     */
+#define WTC_ERR_CSR (-10)      /* CSR fails validation (unsorted, id >= n...) */
+#define WTC_ERR_BUSY (-11)     /* serving: queue full / server stopped        */
+
+const char *wtc_strerror(int status);
+/* Text of the last HIP failure on this thread (empty if none). */
+const char *wtc_last_error_detail(void);
+
+/* ------------------------------------------------------------------- types */
+
+/* GRAPH_TYPE, tc.cu:58-63: oriented CSR with 32-bit offsets. */
+typedef struct {
+    uint32_t numVertices;
+    uint32_t numEdges;
+    uint32_t *rowPtr; /* numVertices + 1 */
+    uint32_t *colInd; /* numEdges        */
+} wtc_graph;
+
+/* BIG_GRAPH_TYPE, experiments/tc_big_graphs.cu:58-63: 64-bit offsets. */
+typedef struct {
+    uint32_t numVertices;
+    uint64_t numEdges;
+    uint64_t *rowPtr; /* numVertices + 1 */
+    uint32_t *colInd; /* numEdges        */
+} wtc_big_graph;
+
+/* GPU_time, tc.cu:76-79: milliseconds, accumulated with += by each call. */
+typedef struct {
+    double copy;
+    double exec;
+} wtc_gpu_time;
+
+/* preprocess_t, tc.cu:56: accepted for interface parity; every style yields
+ * the same CSR (the CPU implementation is used for all three). */
+#define WTC_PREPROCESS_CPU 0
+#define WTC_PREPROCESS_GPU 1
+#define WTC_PREPROCESS_GPU_CONSTRAINED 2
+
+/* ------------------------------------------------------- host: graph input */
+
+/* read_graph, tc.cu:561-685 (64-bit: experiments/tc_big_graphs.cu:567-691).
+ * Edge list (-e) or Matrix Market (-m) text; drops self-loops, symmetrises,
+ * sorts, deduplicates; n = largest id + 1.  *out is freed with
+ * wtc_free_graph. */
+int wtc_read_graph(const char *filename, int matrix_market, int zero_indexed, int preprocess_style,
+                   wtc_big_graph **out);
+
+/* The same construction from npairs (src,dst) pairs in memory (interleaved),
+ * exactly as if they were the lines of an edge-list file read with -z. */
+int wtc_graph_from_pairs(const uint32_t *pairs, uint64_t npairs, wtc_big_graph **out);
+
+/* preprocess, tc.cu:687-787: stable ascending degree order, relabel, keep
+ * v->w iff rank(w) > rank(v), rows ascending.  numEdges halves. */
+int wtc_preprocess(const wtc_big_graph *original, int preprocess_style, wtc_big_graph **out);
+
+/* free_graph, tc.cu:789-793. */
+void wtc_free_graph(wtc_big_graph *g);
+
+/* 32-bit view for the tc.cu path; fails with WTC_ERR_RANGE if numEdges does
+ * not fit 32 bits.  Free with wtc_free_graph32. */
+int wtc_narrow_graph(const wtc_big_graph *g, wtc_graph **out);
+void wtc_free_graph32(wtc_graph *g);
+
+/* Checks an oriented CSR: rowPtr monotone, rows strictly ascending, every
+ * id < n and greater than its row (the orientation invariant; compare
+ * experiments/edge_get.cu:311-316). */
+int wtc_validate_oriented(const wtc_big_graph *g);
+
+/* W = sum_v C(d+(v),2), the wedgeSum_total of tc.cu:361. */
+uint64_t wtc_wedge_total(const wtc_big_graph *g);
+
+/* Algorithmic bytes of one count over the oriented CSR with offset_bytes-wide
+ * row offsets: b_alg = o(n+1) + 4m + 8(n+1) + sum over wedges (v;w,u) of
+ * [4 + 4*ceil(log2(d+(w)+1))] (the u read plus a sorted-list probe of
+ * adj+(w), one word per binary-search level); b_min = o(n+1) + 4m + 8(n+1)
+ * + 8W.  Independent of spread, bitmap and tiling. */
+void wtc_alg_bytes(const wtc_big_graph *g, uint32_t offset_bytes, double *b_alg, double *b_min);
+
+/* ------------------------------------------------------ host: generators
+ * Seeded, deterministic for any thread count.  Each returns the symmetric,
+ * deduplicated CSR that read_graph -z would build from the generated edge
+ * list (pass it through wtc_preprocess to orient).  The *_pairs variants
+ * return the raw edge list (free with wtc_free_pairs). */
+
+/* R-MAT: edge_factor * 2^scale pairs, quadrant probabilities a,b,c (d=1-a-b-c). */
+int wtc_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                 wtc_big_graph **out);
+int wtc_gen_rmat_pairs(uint32_t scale, uint32_t edge_factor, double a, double b, double c, uint64_t seed,
+                       uint32_t **pairs, uint64_t *npairs);
+/* Chung-Lu: n vertices, expected degrees ~ power law (exponent, on
+ * [wmin, wmax]) drawn by inverse CDF; endpoints drawn proportional to weight
+ * until target_edges distinct undirected non-loop edges exist. */
+int wtc_gen_chung_lu(uint32_t n, double exponent, double wmin, double wmax, uint64_t target_edges,
+                     uint64_t seed, wtc_big_graph **out);
+/* Erdos-Renyi G(n, p). */
+int wtc_gen_erdos_renyi(uint32_t n, double p, uint64_t seed, wtc_big_graph **out);
+/* Barabasi-Albert: each new vertex attaches to m0 distinct earlier vertices. */
+int wtc_gen_barabasi_albert(uint32_t n, uint32_t m0, uint64_t seed, wtc_big_graph **out);
+void wtc_free_pairs(uint32_t *pairs);
+
+/* ------------------------------------------------ GPU: reference-style call */
+
+/* tc_GPU, tc.cu:315-441: allocates device buffers, copies the oriented CSR,
+ * computes the wedge prefix, search tree and top-a bitmap on the device,
+ * counts, copies the count back and frees everything, every call.
+ * t->copy += alloc+H2D ms, t->exec += device compute ms (both accumulated,
+ * like GPU_time).  adjacency_matrix_len must be a multiple of 32. */
+int wtc_tc_gpu(const wtc_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len, wtc_gpu_time *t,
+               uint64_t *triangles);
+/* tc_GPU of experiments/tc_big_graphs.cu:315-447 (64-bit offsets). */
+int wtc_tc_gpu_big(const wtc_big_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len,
+                   wtc_gpu_time *t, uint64_t *triangles);
+
+/* --------------------------------------------- GPU: device-resident graphs */
+
+typedef struct wtc_ctx wtc_ctx;       /* HIP stream + reusable workspace */
+typedef struct wtc_dgraph wtc_dgraph; /* oriented CSR resident in HBM    */
+
+typedef struct {
+    double prefix_ms;  /* wedge prefix scan + setup                          */
+    double bitmap_ms;  /* bitmap clear + build                               */
+    double count_ms;   /* counting kernel                                    */
+    double total_ms;   /* first launch to count back on the host             */
+    uint64_t wedges;   /* W counted                                          */
+    uint32_t blocks;   /* counting-kernel grid                               */
+    uint32_t launches; /* counting-kernel launches                           */
+} wtc_count_stats;
+
+int wtc_device_count(int *count);
+int wtc_ctx_create(int device, wtc_ctx **out);
+void wtc_ctx_destroy(wtc_ctx *ctx);
+
+/* Copies an oriented CSR to the device.  offset_bytes 4 or 8 selects the
+ * device rowPtr width (0: 4 if numEdges < 2^32 else 8). */
+int wtc_dgraph_upload(wtc_ctx *ctx, const wtc_big_graph *g, uint32_t offset_bytes, wtc_dgraph **out);
+void wtc_dgraph_free(wtc_dgraph *dg);
+uint64_t wtc_dgraph_bytes(const wtc_dgraph *dg);
+
+/* Counts the wedges [w_begin, w_end) (clamped to [0, W); pass 0, UINT64_MAX
+ * for all) of a resident graph: the device part of tc_GPU, from the resident
+ * CSR to the count on the host.  `st` may be NULL. */
+int wtc_dgraph_count(wtc_ctx *ctx, wtc_dgraph *dg, uint32_t spread, uint32_t adjacency_matrix_len,
+                     uint64_t w_begin, uint64_t w_end, uint64_t *triangles, wtc_count_stats *st);
+
+/* Diagnostics for parity tests: the device wedge prefix (n+1 entries, the
+ * h_wedgeSum of tc.cu:348-361), the 4095-entry search tree (the
+ * c_binary_search_cache of tc.cu:364-368, filled for single-graph counts) and
+ * the bitmap words (tc.cu:329-346) left by the last count on `ctx`. */
+int wtc_ctx_debug_prefix(wtc_ctx *ctx, uint64_t n, uint64_t *wedge_sum, uint64_t *tree);
+int wtc_ctx_debug_bitmap(wtc_ctx *ctx, uint64_t words, uint32_t *out);
+/* Runs the device first-wedge decode (tc.cu:209-212) on k host-given
+ * (d, i_v) pairs, d >= 2 and i_v < C(d,2), writing (w_i, u_i). */
+int wtc_debug_decode(wtc_ctx *ctx, const uint32_t *d, const uint64_t *iv, uint64_t k, uint32_t *w_out,
+                     uint32_t *u_out);
+
+/* -------------------------------------------------------- GPU: batching
+ * Many graphs counted by one persistent launch.  The graphs are laid out in
+ * HBM as one disjoint union (ids shifted); every tile of the wedge space
+ * belongs to one graph, and each graph has its own spread and bitmap
+ * length.  spread/a arrays of length k; a spread of 0 asks the autotuner. */
+
+typedef struct wtc_batch wtc_batch;
+
+int wtc_batch_upload(wtc_ctx *ctx, const wtc_big_graph *const *graphs, size_t k, wtc_batch **out);
+void wtc_batch_free(wtc_batch *b);
+int wtc_batch_count(wtc_ctx *ctx, wtc_batch *b, const uint32_t *spread, const uint32_t *adjacency_matrix_len,
+                    uint64_t *triangles, wtc_count_stats *st);
+/* Host CSRs in, counts out: upload + count + free in one call (the batched
+ * counterpart of calling wtc_tc_gpu once per graph). */
+int wtc_batch_count_host(wtc_ctx *ctx, const wtc_big_graph *const *graphs, size_t k, const uint32_t *spread,
+                         const uint32_t *adjacency_matrix_len, uint64_t *triangles, wtc_count_stats *st);
+/* The autotuner's choice for a graph of n vertices, m edges and W wedges. */
+void wtc_autotune(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *adjacency_matrix_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,134 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle and the
+golden fixtures.  Integer counts must match bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+import wtc
+
+pytestmark = pytest.mark.gpu
+
+SPREADS = (1, 2, 3, 5, 8, 16, 17, 40)
+ALENS = (0, 32, 64, 4096, 8192)
+
+
+def _rmat(scale, ef=16, seed=1):
+    pairs = wtc.gen_rmat_pairs(scale, ef, seed=seed)
+    g = wtc.preprocess(wtc.graph_from_pairs(pairs))
+    return g
+
+
+def test_golden_fixtures(gpu_ctx, golden):
+    for fx in golden:
+        g = wtc.preprocess(wtc.read_graph(fx["path"], fx["format"] == "mm", fx["zero_indexed"]))
+        dg = gpu_ctx.upload(g)
+        for s in (1, 5, 8):
+            for a in (0, 32, 64, 8192):
+                cnt, st = dg.count(s, a, stats=True)
+                assert cnt == fx["triangles"], (fx["file"], s, a)
+                assert st.wedges == fx["W"], fx["file"]
+        dg.free()
+
+
+def test_device_prefix_tree_bitmap_match_oracle(gpu_ctx):
+    for scale, seed in ((8, 1), (11, 2), (13, 3)):
+        g = _rmat(scale, seed=seed)
+        dg = gpu_ctx.upload(g)
+        for a in (32, 64, 1024):
+            dg.count(3, a)
+            ws, tree = gpu_ctx.debug_prefix(g.n)
+            ows = O.wedge_prefix(g)
+            assert np.array_equal(ws, ows)
+            assert np.array_equal(tree, O.search_tree(ows, g.n))
+            bm = gpu_ctx.debug_bitmap(O.bitmap_words(a))
+            assert np.array_equal(bm, O.bitmap_build(g, a)), (scale, a)
+        dg.free()
+
+
+def test_decode_full_domain(gpu_ctx):
+    rng = np.random.default_rng(5)
+    d_list, iv_list = [], []
+    for d in [2, 3, 4, 5, 17, 255, 65536, 65537, 92682, 92683, 1 << 20, (1 << 28) + 3, (1 << 31) - 1,
+              (1 << 31), (1 << 32) - 1]:
+        C = d * (d - 1) // 2
+        ivs = {0, C - 1, C // 2}
+        for w in rng.integers(0, d - 1, size=20).tolist() + [0, d - 2]:
+            start = w * d - w * (w + 1) // 2  # first wedge of row w
+            ivs.update({start, max(start - 1, 0), min(start + 1, C - 1)})
+        for iv in sorted(ivs):
+            d_list.append(d)
+            iv_list.append(iv)
+    w, u = gpu_ctx.debug_decode(d_list, iv_list)
+    for k, (d, iv) in enumerate(zip(d_list, iv_list)):
+        assert (int(w[k]), int(u[k])) == O.decode(d, iv), (d, iv)
+
+
+@pytest.mark.parametrize("ob", [4, 8])
+def test_rmat_sweep(gpu_ctx, ob):
+    g = _rmat(14, seed=9)
+    expect = O.tc_merge(g)
+    dg = gpu_ctx.upload(g, offset_bytes=ob)
+    for s in SPREADS:
+        for a in ALENS:
+            assert dg.count(s, a) == expect, (s, a, ob)
+    dg.free()
+
+
+def test_wedge_ranges(gpu_ctx):
+    g = _rmat(12, seed=4)
+    W = g.wedges()
+    dg = gpu_ctx.upload(g)
+    total = dg.count(5, 64)
+    cuts = sorted({0, W, *np.random.default_rng(1).integers(0, W, size=9).tolist()})
+    parts = [dg.count(5, 64, b, e) for b, e in zip(cuts[:-1], cuts[1:])]
+    assert sum(parts) == total
+    for b, e in list(zip(cuts[:-1], cuts[1:]))[:4]:
+        assert dg.count(3, 0, b, e) == O.tc_wedge(g, 3, 0, b, e)
+    assert dg.count(5, 64, W, W + 10) == 0
+    dg.free()
+
+
+def test_reference_style_entry_points():
+    g = _rmat(12, seed=6)
+    expect = O.tc_merge(g)
+    t = wtc.GpuTime()
+    assert wtc.tc_gpu(g, 5, 8192, t) == expect
+    c1, e1 = t.copy, t.exec
+    assert c1 > 0 and e1 > 0
+    assert wtc.tc_gpu_big(g, 5, 8192, t) == expect
+    assert t.copy > c1 and t.exec > e1  # accumulated with +=, like GPU_time
+
+
+def test_edge_cases(gpu_ctx):
+    cases = [
+        np.zeros((0, 2), np.uint32),                                   # no edges: n = 1
+        np.array([[0, 1], [2, 3], [4, 5]], np.uint32),                 # no wedges
+        np.array([[0, i] for i in range(1, 3000)], np.uint32),        # star: W = 0 after orientation
+        np.array([[i, j] for i in range(300) for j in range(i + 1, 300)], np.uint32),  # K_300, d+ up to 299
+        np.array([[0, 1], [1, 2], [2, 0]], np.uint32),                 # one triangle
+    ]
+    expect = [0, 0, 0, 300 * 299 * 298 // 6, 1]
+    for pairs, ex in zip(cases, expect):
+        g = wtc.preprocess(wtc.graph_from_pairs(pairs))
+        dg = gpu_ctx.upload(g)
+        for s, a in ((1, 0), (5, 8192), (17, 64)):
+            assert dg.count(s, a) == ex
+        dg.free()
+
+
+def test_batch_exact(gpu_ctx):
+    graphs, expect = [], []
+    for i in range(40):
+        if i % 2:
+            g = wtc.preprocess(wtc.gen_erdos_renyi(50 + 37 * i, 8.0 / (50 + 37 * i), seed=i))
+        else:
+            g = wtc.preprocess(wtc.gen_barabasi_albert(60 + 29 * i, 2 + i % 7, seed=i))
+        graphs.append(g)
+        expect.append(O.tc_merge(g))
+    b = wtc.Batch(gpu_ctx, graphs)
+    for spread, a in ((None, None), (1, 0), (5, 64), ([1 + (i % 9) for i in range(40)], [32 * (i % 4) for i in range(40)])):
+        got = b.count(spread, a)
+        assert got.tolist() == expect, (spread, a)
+    b.free()
+    got = wtc.batch_count_host(gpu_ctx, graphs)
+    assert got.tolist() == expect

@@ -1,0 +1,166 @@
+"""Host side of the drop-in (CPU only): the library's reader, preprocessing and
+generators against the oracle; C-ABI exports; the tc CLI's error paths."""
+import hashlib
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+import wtc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "wedge-parallel-triangle-counting_amd")
+
+from test_oracle import READER_CASES, _write  # noqa: E402
+
+
+def same_csr(a, b):
+    return a.n == b.n and a.m == b.m and np.array_equal(np.asarray(a.row_ptr, np.uint64),
+                                                       np.asarray(b.row_ptr, np.uint64)) \
+        and np.array_equal(np.asarray(a.col_ind, np.uint32), np.asarray(b.col_ind, np.uint32))
+
+
+def test_c_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "wtc.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set(re.findall(r"\b(wtc_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) > 30
+    import ctypes
+    lib = ctypes.CDLL(wtc.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(wtc.EXPORTED) <= names
+
+
+def test_golden_reader_and_preprocess_match_oracle(golden):
+    for fx in golden:
+        g = wtc.read_graph(fx["path"], fx["format"] == "mm", fx["zero_indexed"])
+        og = O.read_graph(fx["path"], fx["format"] == "mm", fx["zero_indexed"])
+        assert same_csr(g, og), fx["file"]
+        p = wtc.preprocess(g)
+        h = hashlib.sha256(np.asarray(p.row_ptr).astype("<u8").tobytes() +
+                           np.asarray(p.col_ind).astype("<u4").tobytes()).hexdigest()
+        assert h == fx["csr_sha256"], fx["file"]
+        assert p.wedges() == fx["W"]
+        assert p.validate_oriented()
+
+
+@pytest.mark.parametrize("case", READER_CASES, ids=[c[0] for c in READER_CASES])
+def test_reader_cases_match_oracle(tmp_path, case):
+    name, text, mm, zero, n, expect = case
+    path = _write(tmp_path, name, text)
+    og = O.read_graph(path, mm, zero)
+    if isinstance(og, int):
+        with pytest.raises(wtc.WtcError) as e:
+            wtc.read_graph(path, mm, zero)
+        assert e.value.code == og
+        return
+    assert same_csr(wtc.read_graph(path, mm, zero), og)
+
+
+def test_preprocess_styles_identical():
+    g = wtc.gen_rmat(11, 8, seed=3)
+    ref = wtc.preprocess(g, 0)
+    for style in (1, 2):
+        assert same_csr(wtc.preprocess(g, style), ref)
+    with pytest.raises(wtc.WtcError):
+        wtc.preprocess(g, 3)
+
+
+def test_random_graphs_match_oracle():
+    for seed in range(4):
+        pairs = wtc.gen_rmat_pairs(11, 8, seed=seed)
+        g, og = wtc.graph_from_pairs(pairs), O.graph_from_pairs(pairs)
+        assert same_csr(g, og)
+        p, op = wtc.preprocess(g), O.preprocess(og)
+        assert same_csr(p, op)
+        assert p.wedges() == int(O.wedge_prefix(op)[-1])
+        assert p.alg_bytes(4) == O.alg_bytes(op, 4)
+        assert p.alg_bytes(8) == O.alg_bytes(op, 8)
+
+
+def test_rmat_paths_agree(tmp_path):
+    pairs = wtc.gen_rmat_pairs(10, 4, seed=5)
+    path = tmp_path / "rmat.txt"
+    np.savetxt(path, pairs, fmt="%d", header="rmat", comments="# ")
+    g_file = wtc.read_graph(str(path), zero_indexed=True)
+    assert same_csr(g_file, wtc.gen_rmat(10, 4, seed=5))
+    assert same_csr(g_file, wtc.graph_from_pairs(pairs))
+
+
+def test_generators_deterministic_and_sized():
+    a = wtc.gen_rmat(9, 8, seed=1)
+    assert same_csr(a, wtc.gen_rmat(9, 8, seed=1))
+    assert not same_csr(a, wtc.gen_rmat(9, 8, seed=2))
+    cl = wtc.gen_chung_lu(5000, 2.7, 1, 60, 8000, seed=4)
+    assert cl.m == 2 * 8000
+    assert same_csr(cl, wtc.gen_chung_lu(5000, 2.7, 1, 60, 8000, seed=4))
+    ba = wtc.gen_barabasi_albert(1000, 5, seed=2)
+    assert ba.m == 2 * (1000 - 5) * 5
+    er = wtc.gen_erdos_renyi(2000, 0.01, seed=3)
+    mean = er.m / 2 / (2000 * 1999 / 2)
+    assert 0.008 < mean < 0.012
+
+
+def test_generators_thread_count_independent():
+    code = ("import sys; sys.path.insert(0, %r); import wtc, hashlib, numpy as np; "
+            "g = wtc.preprocess(wtc.gen_rmat(12, 8, seed=9)); "
+            "print(hashlib.sha256(np.asarray(g.row_ptr).tobytes() + np.asarray(g.col_ind).tobytes()).hexdigest())"
+            % PKG)
+    outs = set()
+    for t in ("1", "3", "8"):
+        env = dict(os.environ, WTC_HOST_THREADS=t)
+        outs.add(subprocess.run(["python3", "-c", code], env=env, capture_output=True, text=True,
+                                check=True).stdout.strip())
+    assert len(outs) == 1
+
+
+def test_validate_rejects_bad_csr():
+    good = wtc.preprocess(wtc.gen_rmat(8, 8, seed=1))
+    assert good.validate_oriented()
+    bad_cases = []
+    ci = np.array(good.col_ind).copy()
+    r = int(np.argmax(np.diff(good.row_ptr.astype(np.int64)) >= 2))
+    s = int(good.row_ptr[r])
+    ci[s], ci[s + 1] = ci[s + 1], ci[s]  # unsorted row
+    bad_cases.append(wtc.CSR(good.n, good.m, good.row_ptr.copy(), ci))
+    ci2 = np.array(good.col_ind).copy()
+    ci2[-1] = good.n  # id out of range
+    bad_cases.append(wtc.CSR(good.n, good.m, good.row_ptr.copy(), ci2))
+    rp = np.array(good.row_ptr).copy()
+    rp[-1] -= 1  # rowPtr[n] != m
+    bad_cases.append(wtc.CSR(good.n, good.m, rp, np.array(good.col_ind).copy()))
+    sym = wtc.gen_rmat(8, 8, seed=1)  # symmetric: has ids below their row
+    bad_cases.append(sym)
+    for b in bad_cases:
+        assert not b.validate_oriented()
+
+
+def test_autotune_table():
+    for n in (64, 1000, 5000, 65536, 10 ** 6):
+        s, a = wtc.autotune(n, 0, 0)
+        assert s >= 1 and a % 32 == 0
+
+
+# ------------------------------------------------------------------- CLI
+
+def _tc(*args, binary="tc"):
+    return subprocess.run([os.path.join(PKG, binary), *args], capture_output=True, text=True, timeout=60)
+
+
+@pytest.mark.parametrize("binary", ["tc", "tc_big_graphs"])
+def test_cli_usage_errors(tmp_path, binary):
+    good = _write(tmp_path, "g.txt", "1 2\n2 3\n3 1\n")
+    zero = _write(tmp_path, "z.txt", "0 1\n1 2\n")
+    for args in ([], ["-e", good], ["-e", good, "-s", "0"], ["-e", good, "-s", "5", "-a", "33"],
+                 ["-e", good, "-s", "5", "-p", "3"], ["-q"], ["-e"]):
+        r = _tc(*args, binary=binary)
+        assert r.returncode == 1, args
+        assert "Usage:" in r.stdout and "-s <num>" in r.stdout
+    r = _tc("-e", "/nonexistent.txt", "-s", "5", "-a", "0", binary=binary)
+    assert r.returncode == 1 and "unable to open graph file" in r.stderr
+    r = _tc("-e", zero, "-s", "5", "-a", "0", binary=binary)
+    assert r.returncode == 1 and "zero vertex id detected but -z was not set" in r.stderr
