@@ -29,10 +29,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def load_traffic(workload):
+def traffic_name(workload, spread, a):
+    return f"k_count_{workload}_s{spread}_a{a}_traffic.json"
+
+
+def load_traffic(workload, spread, a):
     """HBM bytes per counting-kernel launch from the committed rocprofv3 PMC
-    summary, if one exists for this workload."""
-    path = os.path.join(ROOT, "profiles", f"k_count_{workload}_traffic.json")
+    summary, if one exists for this workload and configuration."""
+    path = os.path.join(ROOT, "profiles", traffic_name(workload, spread, a))
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -77,6 +81,121 @@ def cpu_baseline(g, dg, spread, a, target_s, threads):
     }
 
 
+def batch_graphs(k, base_seed):
+    """The batching workload: k seeded graphs, each Erdos-Renyi or
+    Barabasi-Albert (50/50), n log-uniform in [64, 65536], average degree
+    uniform in [4, 64]; returned oriented (degree order)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(i):
+        r = np.random.default_rng(base_seed + i)
+        er = r.random() < 0.5
+        n = int(round(np.exp(r.uniform(np.log(64), np.log(65536)))))
+        kdeg = r.uniform(4, 64)
+        if er:
+            g = wtc.gen_erdos_renyi(n, min(1.0, kdeg / (n - 1)), seed=base_seed + i)
+        else:
+            g = wtc.gen_barabasi_albert(n, max(1, min(n - 1, int(np.ceil(kdeg / 2)))), seed=base_seed + i)
+        return wtc.preprocess(g)
+
+    with ThreadPoolExecutor(max_workers=min(16, len(os.sched_getaffinity(0)))) as ex:
+        return list(ex.map(one, range(k)))
+
+
+def batch_bench(ctx, k, base_seed, check_sample, threads):
+    """Batched vs plain graphs/s on the same host-resident oriented CSRs.
+    plain: one reference-style tc_GPU call per graph (alloc, copy, count, copy
+    back, free); batched: one upload of all graphs and one persistent counting
+    launch.  Both use the autotuner's spread and bitmap length per graph."""
+    t0 = time.perf_counter()
+    graphs = batch_graphs(k, base_seed)
+    tgen = time.perf_counter() - t0
+    tune = [wtc.autotune(g.n, g.m, g.wedges()) for g in graphs]
+    spread = [s for s, _ in tune]
+    alen = [a for _, a in tune]
+    log(f"batch: {k} graphs, {sum(g.m for g in graphs)} edges, {sum(g.wedges() for g in graphs)} wedges, "
+        f"generated in {tgen:.1f}s")
+    # warm-up both paths on a few graphs
+    for g, s, a in list(zip(graphs, spread, alen))[:4]:
+        wtc.tc_gpu_big(g, s, a)
+    wtc.batch_count_host(ctx, graphs[:8], spread[:8], alen[:8])
+    t0 = time.perf_counter()
+    plain = [wtc.tc_gpu_big(g, s, a) for g, s, a in zip(graphs, spread, alen)]
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batched, st = wtc.batch_count_host(ctx, graphs, spread, alen, stats=True)
+    t_batch = time.perf_counter() - t0
+    b = wtc.Batch(ctx, graphs)
+    b.count(spread, alen)
+    t0 = time.perf_counter()
+    resident, st_res = b.count(spread, alen, stats=True)
+    t_res = time.perf_counter() - t0
+    b.free()
+    exact = bool(np.array_equal(np.asarray(plain, np.uint64), batched)) and bool(np.array_equal(batched, resident))
+    sample_ok = None
+    if check_sample:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        idx = np.random.default_rng(base_seed).choice(k, size=min(check_sample, k), replace=False)
+        sample_ok = all(O.tc_merge(graphs[i], threads) == int(batched[i]) for i in idx)
+    return {"graphs": k, "edges": int(sum(g.m for g in graphs)), "wedges": int(sum(g.wedges() for g in graphs)),
+            "plain_graphs_per_s": k / t_plain, "batched_graphs_per_s": k / t_batch,
+            "resident_batched_graphs_per_s": k / t_res, "batched_over_plain": t_plain / t_batch,
+            "plain_s": t_plain, "batched_s": t_batch, "resident_batched_s": t_res,
+            "batched_count_kernel_ms": st.count_ms, "resident_count_kernel_ms": st_res.count_ms,
+            "per_graph_counts_equal": exact, "oracle_sample_equal": sample_ok,
+            "sample_size": int(min(check_sample, k)) if check_sample else 0}
+
+
+def _rocprof_rows(directory, pattern):
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(directory, "**", pattern), recursive=True):
+        with open(path, newline="") as f:
+            rows.extend(csv.DictReader(f))
+    return rows
+
+
+def _per_dispatch(directory, kernel_sub, counter):
+    """Mean per-dispatch value of `counter` for kernels whose name contains
+    kernel_sub, from rocprofv3 counter_collection.csv files."""
+    vals = {}
+    for r in _rocprof_rows(directory, "*counter_collection.csv"):
+        if kernel_sub in r.get("Kernel_Name", "") and r.get("Counter_Name") == counter:
+            key = r.get("Dispatch_Id") or r.get("Correlation_Id")
+            vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
+    if not vals:
+        raise SystemExit(f"no {counter} rows for {kernel_sub} under {directory}")
+    return sum(vals.values()) / len(vals), len(vals)
+
+
+def pmc_summary(args):
+    """Turns four rocprofv3 --pmc runs (FETCH_SIZE and WRITE_SIZE, each on the
+    counting kernel and on the calibration copy) into HBM bytes per counting
+    launch, each counter scaled by what it reads for the copy of known size."""
+    kdir_f, kdir_w, cdir_f, cdir_w = args.pmc_summary
+    copy_bytes = args.copy_calibration_mb * 2 ** 20
+    kf, nk = _per_dispatch(kdir_f, "k_count", "FETCH_SIZE")
+    kw, _ = _per_dispatch(kdir_w, "k_count", "WRITE_SIZE")
+    cf, _ = _per_dispatch(cdir_f, "k_copy", "FETCH_SIZE")
+    cw, _ = _per_dispatch(cdir_w, "k_copy", "WRITE_SIZE")
+    f_scale = copy_bytes / (cf * 1024.0)  # true bytes per counted byte
+    w_scale = copy_bytes / (cw * 1024.0)
+    fetch = kf * 1024.0 * f_scale
+    write = kw * 1024.0 * w_scale
+    out = {"workload": args.pmc_workload, "kernel": "k_count", "dispatches": nk,
+           "fetch_kib_per_dispatch": kf, "write_kib_per_dispatch": kw,
+           "copy_bytes": copy_bytes, "copy_fetch_kib": cf, "copy_write_kib": cw,
+           "fetch_scale": f_scale, "write_scale": w_scale,
+           "fetch_bytes": fetch, "write_bytes": write, "bytes_per_launch": fetch + write}
+    out["config"] = {"spread": args.spread, "a": args.a}
+    path = os.path.join(ROOT, "profiles", traffic_name(args.pmc_workload, args.spread, args.a))
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,19 +204,46 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--spread", type=int, default=5)
-    ap.add_argument("--a", type=int, default=8192)
+    ap.add_argument("--spread", type=int, default=8)
+    ap.add_argument("--a", type=int, default=16384)
+    ap.add_argument("--ways", type=int, default=4, help="probes advanced together per lane")
+    ap.add_argument("--chunk", type=int, default=16, help="wedges per lane staged in LDS per phase")
     ap.add_argument("--offset-bytes", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run the calibration copy of this many MiB (for profiler counters)")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("KFETCH", "KWRITE", "CFETCH", "CWRITE"),
+                    help="summarise rocprofv3 --pmc output dirs into profiles/ (no GPU use)")
+    ap.add_argument("--pmc-workload", default="rmat24")
+    ap.add_argument("--batch-graphs", type=int, default=0, help="also run batched vs plain on this many graphs")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched vs plain comparison")
     args = ap.parse_args()
 
+    if args.pmc_summary:
+        pmc_summary(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
         sys.exit(2)
     if wtc.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration_mb:
+        ctx = wtc.Context(0)
+        nbytes = args.copy_calibration_mb * 2 ** 20
+        ms = ctx.debug_copy(nbytes, reps=1)
+        print(json.dumps({"copy_bytes": nbytes, "ms": ms, "GB_per_s": 2 * nbytes / ms / 1e6}), flush=True)
+        ctx.close()
+        return
+
+    if args.batch_only:
+        ctx = wtc.Context(0)
+        ctx.set_tuning(args.ways, args.chunk)
+        res = batch_bench(ctx, args.batch_graphs or 4096, 1000, 32, min(len(os.sched_getaffinity(0)), 16))
+        ctx.close()
+        print(json.dumps(res), flush=True)
+        return
 
     workload = f"rmat{args.scale}"
     t0 = time.perf_counter()
@@ -112,6 +258,7 @@ def main():
     b_alg, b_min = g.alg_bytes(ob)
 
     ctx = wtc.Context(0)
+    ctx.set_tuning(args.ways, args.chunk)
     dg = ctx.upload(g, offset_bytes=ob)
     log(f"uploaded {dg.nbytes / 1e9:.2f} GB")
     for _ in range(args.warmup):
@@ -131,7 +278,7 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     count_ms = statistics.median(s["count_ms"] for s in stats)
     achieved = b_alg / (count_ms / 1e3)
-    traffic = load_traffic(workload)
+    traffic = load_traffic(workload, args.spread, args.a)
     out = {
         "metric": "wedge checks/s & edges/s at RMAT-24 (1 GPU, exact); batched vs plain graphs/s",
         "value": W / (ms_per_step / 1e3),
@@ -165,6 +312,10 @@ def main():
         out["cpu_baseline"] = cpu_baseline(g, dg, args.spread, args.a, args.cpu_seconds, threads)
         log(f"cpu baseline: {out['cpu_baseline']['sample']}")
     dg.free()
+    if args.batch_graphs:
+        out["batched_vs_plain"] = batch_bench(ctx, args.batch_graphs, 1000, 32,
+                                              min(len(os.sched_getaffinity(0)), 16))
+        log(f"batched/plain = {out['batched_vs_plain']['batched_over_plain']:.2f}x")
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -166,6 +166,10 @@ typedef struct {
 int wtc_device_count(int *count);
 int wtc_ctx_create(int device, wtc_ctx **out);
 void wtc_ctx_destroy(wtc_ctx *ctx);
+/* Tuning of the counting kernel: probe_ways = independent binary searches
+ * each lane advances together (1, 2, 4 or 8; default 4 or $WTC_PROBE_WAYS);
+ * lds_chunk = wedges per lane staged in LDS per phase (1..16, default 16). */
+int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk);
 
 /* Copies an oriented CSR to the device.  offset_bytes 4 or 8 selects the
  * device rowPtr width (0: 4 if numEdges < 2^32 else 8). */
@@ -185,6 +189,11 @@ int wtc_dgraph_count(wtc_ctx *ctx, wtc_dgraph *dg, uint32_t spread, uint32_t adj
  * the bitmap words (tc.cu:329-346) left by the last count on `ctx`. */
 int wtc_ctx_debug_prefix(wtc_ctx *ctx, uint64_t n, uint64_t *wedge_sum, uint64_t *tree);
 int wtc_ctx_debug_bitmap(wtc_ctx *ctx, uint64_t words, uint32_t *out);
+/* Copies `bytes` (rounded down to 16) between two fresh device buffers `reps`
+ * times with a plain streaming kernel; *ms = average per copy.  The byte
+ * count is known exactly, so profiler HBM counters read on it calibrate the
+ * counters read on the counting kernel. */
+int wtc_debug_copy(wtc_ctx *ctx, uint64_t bytes, int reps, double *ms);
 /* Runs the device first-wedge decode (tc.cu:209-212) on k host-given
  * (d, i_v) pairs, d >= 2 and i_v < C(d,2), writing (w_i, u_i). */
 int wtc_debug_decode(wtc_ctx *ctx, const uint32_t *d, const uint64_t *iv, uint64_t k, uint32_t *w_out,

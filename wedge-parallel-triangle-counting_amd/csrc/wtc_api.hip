@@ -108,7 +108,9 @@ struct wtc_ctx {
     hipEvent_t ev[6] = {};
     Workspace work;
     PinBuf h_desc, h_out, h_stage;
-    uint32_t grid_cache[2][kChunkMax + 2] = {};
+    int ways = 4;         // independent probes per lane in the counting kernel
+    uint32_t chunk = 16;  // wedges per lane staged in LDS per phase (1..kChunkMax)
+    uint32_t grid_cache[2][4][kChunkMax + 2] = {};
 };
 
 struct wtc_dgraph {
@@ -119,12 +121,14 @@ struct wtc_dgraph {
     DevBuf rowPtr, colInd;
 };
 
+// Many oriented CSRs in one device allocation: every graph keeps its own
+// rowPtr (offsets relative to its own colInd) and colInd (local ids).
 struct wtc_batch {
     wtc_ctx *ctx;
     uint32_t G;
-    uint64_t N, M;
     int ob;
-    std::vector<uint32_t> vbase, n;
+    std::vector<uint32_t> n;
+    std::vector<uint64_t> m, row_off, col_off;  // element offsets into rowPtr / colInd
     DevBuf rowPtr, colInd;
 };
 
@@ -149,24 +153,33 @@ int validate_oriented_serial(const wtc_big_graph *g) {
     return validate_oriented_rows(g->numVertices, g->numEdges, g->rowPtr, g->colInd, 0, g->numVertices);
 }
 
-// Host-filled descriptor of one graph of a set.
-GraphDesc make_desc(uint32_t vbase, uint32_t n, uint32_t spread, uint32_t a, uint64_t w_begin, uint64_t w_end,
-                    uint64_t &bm_words, uint64_t &rows) {
+// Host-filled descriptor of one graph of a set (device pointers to its CSR;
+// workspace slices are assigned by run_set).
+GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint32_t spread, uint32_t a,
+                    uint64_t w_begin, uint64_t w_end) {
     GraphDesc d{};
-    d.vbase = vbase;
+    d.rowPtr = rowPtr;
+    d.colInd = colInd;
     d.n = n;
     d.spread = spread;
     d.a = a;
     d.bm_rows = a ? std::min(n, a) : 0u;
-    d.thr = a ? vbase + (std::max(n, a) - a) : vbase + n;
+    d.thr = a ? std::max(n, a) - a : n;
     d.bm_size = binom2(a);
-    d.bm_word_off = bm_words;
-    d.row_base = rows;
     d.w_begin = w_begin;
     d.w_end = w_end;
-    bm_words += a ? bm_words_for(a) : 0;
-    rows += d.bm_rows;
     return d;
+}
+
+bool valid_ways(int w) { return w == 1 || w == 2 || w == 4 || w == 8; }
+int ways_index(int w) { return w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3; }
+
+// Default probes per lane for new contexts: WTC_PROBE_WAYS (1, 2, 4 or 8),
+// else 4.
+int default_probe_ways() {
+    const char *e = std::getenv("WTC_PROBE_WAYS");
+    const int v = e ? std::atoi(e) : 4;
+    return valid_ways(v) ? v : 4;
 }
 
 struct RunTimes {
@@ -174,46 +187,67 @@ struct RunTimes {
 };
 
 // Runs the device pipeline over a resident graph set and returns the
-// per-graph counts on the host.  `work` provides the device workspace.
-int run_set(wtc_ctx *c, Workspace &work, const void *rowPtr, int ob, const uint32_t *colInd, uint64_t N,
-            std::vector<GraphDesc> &hd, uint64_t bm_words, uint64_t rows, uint64_t *counts, uint64_t *wedges,
+// per-graph counts on the host.  `work` provides the device workspace: the
+// wedge prefixes, scan partials and bitmaps of all graphs, laid out back to
+// back.
+int run_set(wtc_ctx *c, Workspace &work, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, uint64_t *wedges,
             uint32_t *grid_out, RunTimes *rt) {
     const uint32_t G = (uint32_t)hd.size();
     if (G == 0) return WTC_OK;
-    if (N == 0 || N > 0xffffffffULL) return WTC_ERR_ARG;
     uint32_t max_spread = 1;
-    for (const auto &d : hd) max_spread = std::max(max_spread, d.spread);
+    uint64_t ws_n = 0, bm_words = 0, rows = 0, blocks = 0;
+    for (const auto &d : hd) {
+        if (d.n == 0 || d.spread == 0) return WTC_ERR_ARG;
+        max_spread = std::max(max_spread, d.spread);
+        ws_n += (uint64_t)d.n + 1;
+        bm_words += d.a ? bm_words_for(d.a) : 0;
+        rows += d.bm_rows;
+        blocks += prefix_blocks(d.n);
+    }
+    if (blocks > 0x7fffffffULL) return WTC_ERR_RANGE;
     const bool single = G == 1;
-    HIPC(work.ws.ensure((N + 1) * sizeof(uint64_t)));
-    HIPC(work.partials.ensure((size_t)prefix_blocks((uint32_t)N) * sizeof(uint64_t)));
+    HIPC(work.ws.ensure(ws_n * sizeof(uint64_t)));
+    HIPC(work.partials.ensure(blocks * sizeof(uint64_t)));
     HIPC(work.tree.ensure(kTreeSize * sizeof(uint64_t)));
     HIPC(work.desc.ensure((size_t)G * sizeof(GraphDesc)));
     HIPC(work.out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
     if (bm_words) HIPC(work.bm.ensure(bm_words * sizeof(uint32_t)));
     HIPC(c->h_desc.ensure((size_t)G * sizeof(GraphDesc)));
     HIPC(c->h_out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
+    uint64_t ws_off = 0, bm_off = 0, row_off = 0, blk_off = 0;
+    for (auto &d : hd) {
+        d.ws = work.ws.as<uint64_t>() + ws_off;
+        d.bitmap = d.a ? work.bm.as<uint32_t>() + bm_off : nullptr;
+        d.row_base = row_off;
+        d.blk_base = blk_off;
+        ws_off += (uint64_t)d.n + 1;
+        bm_off += d.a ? bm_words_for(d.a) : 0;
+        row_off += d.bm_rows;
+        blk_off += prefix_blocks(d.n);
+    }
     std::memcpy(c->h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
 
     uint64_t *meta = work.out.as<uint64_t>();
     unsigned long long *dcounts = reinterpret_cast<unsigned long long *>(meta + kMetaSlots);
     GraphDesc *ddesc = work.desc.as<GraphDesc>();
+    uint64_t *tree = single ? work.tree.as<uint64_t>() : nullptr;
     hipStream_t st = c->stream;
 
     const int oi = ob == 8 ? 1 : 0;
-    const uint32_t sci = std::min<uint32_t>(max_spread, kChunkMax);
-    if (!c->grid_cache[oi][sci]) HIPC(count_grid(ob, max_spread, c->num_cus, &c->grid_cache[oi][sci]));
-    const uint32_t grid = c->grid_cache[oi][sci];
+    const uint32_t sci = std::min<uint32_t>(max_spread, c->chunk);
+    const int ways = c->ways;
+    uint32_t &gc = c->grid_cache[oi][ways_index(ways)][sci];
+    if (!gc) HIPC(count_grid(ob, sci, ways, c->num_cus, &gc));
+    const uint32_t grid = gc;
 
     HIPC(hipMemcpyAsync(ddesc, c->h_desc.p, (size_t)G * sizeof(GraphDesc), hipMemcpyHostToDevice, st));
     HIPC(hipEventRecord(c->ev[0], st));
-    HIPC(launch_prefix(rowPtr, ob, (uint32_t)N, work.partials.as<uint64_t>(), work.ws.as<uint64_t>(), st));
-    HIPC(launch_setup(work.ws.as<uint64_t>(), ddesc, G, single ? work.tree.as<uint64_t>() : nullptr, meta, dcounts,
-                      st));
+    HIPC(launch_prefix(ddesc, G, ob, (uint32_t)blocks, work.partials.as<uint64_t>(), st));
+    HIPC(launch_setup(ddesc, G, tree, meta, dcounts, st));
     HIPC(hipEventRecord(c->ev[1], st));
-    HIPC(launch_bitmap(rowPtr, ob, colInd, ddesc, G, rows, work.bm.as<uint32_t>(), bm_words, st));
+    HIPC(launch_bitmap(ddesc, G, ob, rows, work.bm.as<uint32_t>(), bm_words, st));
     HIPC(hipEventRecord(c->ev[2], st));
-    HIPC(launch_count(rowPtr, ob, colInd, work.ws.as<uint64_t>(), single ? work.tree.as<uint64_t>() : nullptr, ddesc,
-                      G, meta, work.bm.as<uint32_t>(), dcounts, max_spread, grid, st));
+    HIPC(launch_count(ddesc, G, ob, tree, meta, dcounts, max_spread, c->chunk, ways, grid, st));
     HIPC(hipEventRecord(c->ev[3], st));
     HIPC(hipMemcpyAsync(c->h_out.p, meta, (size_t)(kMetaSlots + G) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
@@ -287,12 +321,10 @@ int tc_gpu_impl(uint32_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colI
     }
     float copy_ms = 0;
     (void)hipEventElapsedTime(&copy_ms, c->ev[4], c->ev[5]);
-    uint64_t bm_words = 0, rows = 0;
-    std::vector<GraphDesc> hd{make_desc(0, n, spread, a, 0, UINT64_MAX, bm_words, rows)};
+    std::vector<GraphDesc> hd{make_desc(drow.p, dcol.as<uint32_t>(), n, spread, a, 0, UINT64_MAX)};
     uint64_t cnt = 0;
     RunTimes rt;
-    rc = run_set(c, work, drow.p, (int)sizeof(OffT), dcol.as<uint32_t>(), n, hd, bm_words, rows, &cnt, nullptr,
-                 nullptr, &rt);
+    rc = run_set(c, work, (int)sizeof(OffT), hd, &cnt, nullptr, nullptr, &rt);
     cleanup();
     if (rc) return rc;
     if (t) {
@@ -332,6 +364,7 @@ int wtc_ctx_create(int device, wtc_ctx **out) {
         return hip_fail("hipGetDeviceProperties", e);
     }
     c->num_cus = prop.multiProcessorCount;
+    c->ways = default_probe_ways();
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
         delete c;
         return hip_fail("hipStreamCreate", e);
@@ -342,6 +375,13 @@ int wtc_ctx_create(int device, wtc_ctx **out) {
             return hip_fail("hipEventCreate", e);
         }
     *out = c;
+    return WTC_OK;
+}
+
+int wtc_ctx_set_tuning(wtc_ctx *c, int probe_ways, uint32_t lds_chunk) {
+    if (!c || !valid_ways(probe_ways) || lds_chunk < 1 || lds_chunk > kChunkMax) return WTC_ERR_ARG;
+    c->ways = probe_ways;
+    c->chunk = lds_chunk;
     return WTC_OK;
 }
 
@@ -426,14 +466,12 @@ int wtc_dgraph_count(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t a, ui
     int rc = check_args(spread, a);
     if (rc) return rc;
     HIPC(hipSetDevice(c->device));
-    uint64_t bm_words = 0, rows = 0;
-    std::vector<GraphDesc> hd{make_desc(0, dg->n, spread, a, w_begin, w_end, bm_words, rows)};
+    std::vector<GraphDesc> hd{make_desc(dg->rowPtr.p, dg->colInd.as<uint32_t>(), dg->n, spread, a, w_begin, w_end)};
     RunTimes rt;
     uint64_t W = 0;
     uint32_t grid = 0;
     auto t0 = std::chrono::steady_clock::now();
-    rc = run_set(c, c->work, dg->rowPtr.p, dg->ob, dg->colInd.as<uint32_t>(), dg->n, hd, bm_words, rows, triangles,
-                 &W, &grid, &rt);
+    rc = run_set(c, c->work, dg->ob, hd, triangles, &W, &grid, &rt);
     auto t1 = std::chrono::steady_clock::now();
     if (rc) return rc;
     if (st) {
@@ -484,6 +522,27 @@ int wtc_debug_decode(wtc_ctx *c, const uint32_t *d, const uint64_t *iv, uint64_t
     return e == hipSuccess ? WTC_OK : hip_fail("debug_decode", e);
 }
 
+int wtc_debug_copy(wtc_ctx *c, uint64_t bytes, int reps, double *ms) {
+    if (!c || bytes < 16 || reps < 1) return WTC_ERR_ARG;
+    bytes &= ~(uint64_t)15;
+    HIPC(hipSetDevice(c->device));
+    DevBuf src, dst;
+    hipError_t e = src.ensure(bytes);
+    if (e == hipSuccess) e = dst.ensure(bytes);
+    if (e == hipSuccess) e = launch_fill(src.as<uint32_t>(), bytes / 4, 0x5a5a5a5au, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[4], c->stream);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy(src.p, dst.p, bytes, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[5], c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float t = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev[4], c->ev[5]);
+    src.release();
+    dst.release();
+    if (e != hipSuccess) return hip_fail("debug_copy", e);
+    if (ms) *ms = t / reps;
+    return WTC_OK;
+}
+
 int wtc_ctx_debug_bitmap(wtc_ctx *c, uint64_t words, uint32_t *out) {
     if (!c || !out) return WTC_ERR_ARG;
     if (c->work.bm.cap < words * sizeof(uint32_t)) return WTC_ERR_ARG;
@@ -510,28 +569,27 @@ void wtc_autotune(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uin
 int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, wtc_batch **out) {
     if (!c || !out || (k && !graphs) || k > 0xffffffffULL) return WTC_ERR_ARG;
     *out = nullptr;
-    uint64_t N = 0, M = 0;
-    std::vector<uint64_t> ebase(k);
     wtc_batch *b = new (std::nothrow) wtc_batch();
     if (!b) return WTC_ERR_NOMEM;
     b->ctx = c;
     b->G = (uint32_t)k;
-    b->vbase.resize(k);
     b->n.resize(k);
+    b->m.resize(k);
+    b->row_off.resize(k);
+    b->col_off.resize(k);
+    uint64_t R = 0, M = 0, max_m = 0;
     for (size_t g = 0; g < k; g++) {
         if (!graphs[g] || graphs[g]->numVertices == 0) {
             delete b;
             return WTC_ERR_ARG;
         }
-        b->vbase[g] = (uint32_t)N;
         b->n[g] = graphs[g]->numVertices;
-        ebase[g] = M;
-        N += graphs[g]->numVertices;
+        b->m[g] = graphs[g]->numEdges;
+        b->row_off[g] = R;
+        b->col_off[g] = M;
+        R += (uint64_t)graphs[g]->numVertices + 1;
         M += graphs[g]->numEdges;
-        if (N > 0xfffffffeULL) {
-            delete b;
-            return WTC_ERR_RANGE;
-        }
+        max_m = std::max<uint64_t>(max_m, graphs[g]->numEdges);
     }
     std::atomic<int> bad{0};
     par_chunks(k, 1, [&](uint64_t g0, uint64_t g1, int) {
@@ -542,11 +600,10 @@ int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, w
         delete b;
         return WTC_ERR_CSR;
     }
-    b->N = N;
-    b->M = M;
-    b->ob = M < 0xffffffffULL ? 4 : 8;
+    b->ob = max_m < 0xffffffffULL ? 4 : 8;
+    const int ob = b->ob;
     hipError_t e = hipSetDevice(c->device);
-    const size_t rbytes = (N + 1) * (size_t)b->ob, cbytes = std::max<uint64_t>(M, 1) * 4;
+    const size_t rbytes = R * (size_t)ob, cbytes = std::max<uint64_t>(M, 1) * 4;
     if (e == hipSuccess) e = b->rowPtr.ensure(rbytes);
     if (e == hipSuccess) e = b->colInd.ensure(cbytes);
     if (e == hipSuccess) e = c->h_stage.ensure(rbytes + cbytes);
@@ -554,25 +611,23 @@ int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, w
         wtc_batch_free(b);
         return hip_fail("batch alloc", e);
     }
-    // pack the disjoint union into pinned staging: ids and offsets shifted
+    // pack every graph into pinned staging (row offsets narrowed when they fit)
     char *stage = c->h_stage.as<char>();
     uint32_t *sc = reinterpret_cast<uint32_t *>(stage + rbytes);
-    const int ob = b->ob;
     par_chunks(k, 1, [&](uint64_t g0, uint64_t g1, int) {
         for (uint64_t g = g0; g < g1; g++) {
             const wtc_big_graph *gr = graphs[g];
-            const uint32_t vb = b->vbase[g];
-            const uint64_t eb = ebase[g];
-            for (uint64_t v = 0; v < gr->numVertices; v++) {
-                const uint64_t off = eb + gr->rowPtr[v];
-                if (ob == 8) reinterpret_cast<uint64_t *>(stage)[vb + v] = off;
-                else reinterpret_cast<uint32_t *>(stage)[vb + v] = (uint32_t)off;
+            const uint64_t ro = b->row_off[g];
+            if (ob == 8) {
+                std::memcpy(reinterpret_cast<uint64_t *>(stage) + ro, gr->rowPtr,
+                            ((uint64_t)gr->numVertices + 1) * 8);
+            } else {
+                uint32_t *dst = reinterpret_cast<uint32_t *>(stage) + ro;
+                for (uint64_t v = 0; v <= gr->numVertices; v++) dst[v] = (uint32_t)gr->rowPtr[v];
             }
-            for (uint64_t j = 0; j < gr->numEdges; j++) sc[eb + j] = gr->colInd[j] + vb;
+            if (gr->numEdges) std::memcpy(sc + b->col_off[g], gr->colInd, gr->numEdges * 4);
         }
     });
-    if (ob == 8) reinterpret_cast<uint64_t *>(stage)[N] = M;
-    else reinterpret_cast<uint32_t *>(stage)[N] = (uint32_t)M;
     e = hipMemcpyAsync(b->rowPtr.p, stage, rbytes, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess && M) e = hipMemcpyAsync(b->colInd.p, sc, M * 4, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -598,19 +653,18 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
     if (b->G == 0) return WTC_OK;
     HIPC(hipSetDevice(c->device));
     std::vector<GraphDesc> hd(b->G);
-    uint64_t bm_words = 0, rows = 0;
     for (uint32_t g = 0; g < b->G; g++) {
         uint32_t s = spread ? spread[g] : 0, aa = a ? a[g] : 0;
-        if (s == 0) wtc_autotune(b->n[g], 0, 0, &s, a ? nullptr : &aa);
+        if (s == 0) wtc_autotune(b->n[g], b->m[g], 0, &s, a ? nullptr : &aa);
         if (s == 0 || aa % 32 != 0) return WTC_ERR_ARG;
-        hd[g] = make_desc(b->vbase[g], b->n[g], s, aa, 0, UINT64_MAX, bm_words, rows);
+        const void *rp = b->rowPtr.as<char>() + b->row_off[g] * b->ob;
+        hd[g] = make_desc(rp, b->colInd.as<uint32_t>() + b->col_off[g], b->n[g], s, aa, 0, UINT64_MAX);
     }
     RunTimes rt;
     uint64_t W = 0;
     uint32_t grid = 0;
     auto t0 = std::chrono::steady_clock::now();
-    int rc = run_set(c, c->work, b->rowPtr.p, b->ob, b->colInd.as<uint32_t>(), b->N, hd, bm_words, rows, triangles,
-                     &W, &grid, &rt);
+    int rc = run_set(c, c->work, b->ob, hd, triangles, &W, &grid, &rt);
     auto t1 = std::chrono::steady_clock::now();
     if (rc) return rc;
     if (st) {

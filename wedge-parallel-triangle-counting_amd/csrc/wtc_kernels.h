@@ -16,29 +16,34 @@ constexpr int kScanItems = 16;          // vertices per thread in the prefix sca
 constexpr uint32_t kScanChunk = kBlock * kScanItems;  // vertices per scan block
 constexpr uint32_t kSentinel = 0xffffffffu;           // empty LDS wedge slot
 
-// One graph of a graph set.  A graph set is a disjoint union of oriented
-// CSRs laid out back to back in HBM (vertex ids shifted by vbase, edge
-// offsets by the rows before); a single graph is a set of one.  The host
-// fills the first block of fields, the setup kernel the rest.
+// One graph of a graph set.  A graph set is any number of oriented CSRs
+// resident in HBM (ids local to each graph, row offsets relative to the
+// graph's own colInd); one count launch covers all of them.  A single graph
+// is a set of one.  The host fills the first block of fields, the setup
+// kernel the last three.
 struct GraphDesc {
     // host-filled
-    uint64_t bm_size;      // C(a,2): bits of the graph's top-a bitmap
-    uint64_t bm_word_off;  // first 32-bit word of that bitmap in the pool
-    uint64_t row_base;     // first bitmap row of this graph in the set's row list
-    uint64_t w_begin;      // wedge range to count, relative to the graph
+    const void *rowPtr;      // n+1 offsets of the set's offset width
+    const uint32_t *colInd;  // ids in [0, n), every row strictly ascending
+    uint64_t *ws;            // n+1 wedge prefix (workspace slice)
+    uint32_t *bitmap;        // ceil(C(a,2)/32) words (workspace slice), or null
+    uint64_t bm_size;        // C(a,2): bits of the graph's top-a bitmap
+    uint64_t row_base;       // first bitmap row of this graph in the set's row list
+    uint64_t blk_base;       // first prefix-scan block of this graph
+    uint64_t w_begin;        // wedge range to count, relative to the graph
     uint64_t w_end;
-    uint32_t vbase;        // first vertex id of the graph in the union
-    uint32_t n;            // vertices
-    uint32_t spread;       // wedges per lane (s)
-    uint32_t a;            // bitmap length (top-a ranks), multiple of 32
-    uint32_t thr;          // union id threshold for the bitmap: vbase + max(n,a) - a
-    uint32_t bm_rows;      // min(n, a)
+    uint32_t n;              // vertices
+    uint32_t spread;         // wedges per lane (s)
+    uint32_t a;              // bitmap length (top-a ranks), multiple of 32
+    uint32_t thr;            // bitmap threshold max(n,a) - a (a = 0: n)
+    uint32_t bm_rows;        // min(n, a)
+    uint32_t pad_;
     // device-filled by the setup kernel
-    uint64_t wbase;        // union wedge index of the first counted wedge
-    uint64_t wcount;       // wedges counted
-    uint64_t tile_base;    // first tile of this graph
+    uint64_t wbase;          // first counted wedge (graph-local index)
+    uint64_t wcount;         // wedges counted
+    uint64_t tile_base;      // first tile of this graph
 };
-static_assert(sizeof(GraphDesc) == 88, "GraphDesc layout");
+static_assert(sizeof(GraphDesc) == 120, "GraphDesc layout");
 
 // meta[] slots written by the setup kernel
 enum : int { kMetaTiles = 0, kMetaWedges = 1, kMetaSlots = 4 };

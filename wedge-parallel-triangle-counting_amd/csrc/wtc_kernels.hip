@@ -1,11 +1,10 @@
 // wtc_kernels.hip -- HIP kernels of the wedge-parallel triangle counter for
 // CDNA4 (gfx950, wave64).
 //
-// Device pipeline for a graph set (one graph or a batch laid out as a
-// disjoint union, see GraphDesc):
+// Device pipeline for a graph set (GraphDesc array; one graph or many):
 //   k_wedge_partials / k_scan_partials / k_wedge_scan
-//       wedgeSum[v] = sum_{x<v} C(d+(x),2) over the union's vertices
-//       (the host prefix of tc.cu:348-361, as a reduce-then-scan in HBM)
+//       per graph, wedgeSum[v] = sum_{x<v} C(d+(x),2) (the host prefix of
+//       tc.cu:348-361, as one reduce-then-scan over all graphs' vertices)
 //   k_setup       per-graph wedge ranges, tile bases, the 12-level search
 //                 tree over wedgeSum (tc.cu:306-313) and zeroed counts
 //   k_fill_u32 / k_bitmap_build
@@ -60,18 +59,36 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, uint64_t *s_tot,
     return off + inc - x;
 }
 
+// Rightmost graph g with desc[g].*field <= key (wave-uniform, scalar loads).
+__device__ __forceinline__ uint32_t graph_of(const GraphDesc *__restrict__ desc, uint32_t G, uint64_t key,
+                                            uint64_t GraphDesc::*field) {
+    uint32_t lo = 0, hi = G - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (desc[mid].*field <= key) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // ------------------------------------------------------------ wedge prefix
 
+// Block b scans vertices [lb*kScanChunk, (lb+1)*kScanChunk) of graph g, where
+// b = desc[g].blk_base + lb.
 template <typename OffT>
-__global__ __launch_bounds__(kBlock) void k_wedge_partials(const OffT *__restrict__ rowPtr, uint32_t N,
+__global__ __launch_bounds__(kBlock) void k_wedge_partials(const GraphDesc *__restrict__ desc, uint32_t G,
                                                            uint64_t *__restrict__ partials) {
     __shared__ uint64_t s_tot[kWavesPerBlock];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    const uint32_t g = G > 1 ? graph_of(desc, G, blockIdx.x, &GraphDesc::blk_base) : 0u;
+    const GraphDesc &D = desc[g];
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
+    const uint32_t n = D.n;
+    const uint64_t base = (blockIdx.x - D.blk_base) * (uint64_t)kScanChunk;
     uint64_t acc = 0;
 #pragma unroll 4
     for (int k = 0; k < kScanItems; k++) {
         const uint64_t v = base + (uint64_t)k * kBlock + threadIdx.x;
-        if (v < N) acc += binom2_d((uint64_t)(rowPtr[v + 1] - rowPtr[v]));
+        if (v < n) acc += binom2_d((uint64_t)(rowPtr[v + 1] - rowPtr[v]));
     }
     acc = wave_sum(acc);
     if ((threadIdx.x & (kWave - 1)) == 0) s_tot[threadIdx.x >> 6] = acc;
@@ -93,21 +110,26 @@ __global__ __launch_bounds__(kBlock) void k_scan_partials(uint64_t *__restrict__
 }
 
 template <typename OffT>
-__global__ __launch_bounds__(kBlock) void k_wedge_scan(const OffT *__restrict__ rowPtr, uint32_t N,
-                                                       const uint64_t *__restrict__ partials,
-                                                       uint64_t *__restrict__ ws) {
+__global__ __launch_bounds__(kBlock) void k_wedge_scan(const GraphDesc *__restrict__ desc, uint32_t G,
+                                                       const uint64_t *__restrict__ partials) {
     __shared__ uint64_t s_tot[kWavesPerBlock];
-    uint64_t carry = partials[blockIdx.x];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    const uint32_t g = G > 1 ? graph_of(desc, G, blockIdx.x, &GraphDesc::blk_base) : 0u;
+    const GraphDesc &D = desc[g];
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
+    uint64_t *__restrict__ ws = D.ws;
+    const uint32_t n = D.n;
+    const uint64_t lb = blockIdx.x - D.blk_base;
+    uint64_t carry = partials[blockIdx.x] - partials[D.blk_base];  // graph-local prefix
+    const uint64_t base = lb * kScanChunk;
     for (int k = 0; k < kScanItems; k++) {
         const uint64_t v = base + (uint64_t)k * kBlock + threadIdx.x;
-        const uint64_t x = v < N ? binom2_d((uint64_t)(rowPtr[v + 1] - rowPtr[v])) : 0;
+        const uint64_t x = v < n ? binom2_d((uint64_t)(rowPtr[v + 1] - rowPtr[v])) : 0;
         uint64_t tot;
         const uint64_t ex = block_excl_scan(x, s_tot, tot);
-        if (v < N) ws[v] = carry + ex;
+        if (v < n) ws[v] = carry + ex;
         carry += tot;
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) ws[N] = carry;
+    if (lb == (n - 1) / kScanChunk && threadIdx.x == 0) ws[n] = carry;  // the graph's last block
 }
 
 // ------------------------------------------------------------------- setup
@@ -115,9 +137,8 @@ __global__ __launch_bounds__(kBlock) void k_wedge_scan(const OffT *__restrict__ 
 // One block.  Per graph: wedge range, tile count (64*s wedges per tile) and
 // exclusive tile base; meta totals; counts zeroed; search tree for a single
 // graph (node i = wedgeSum at the midpoint of its range, tc.cu:306-313).
-__global__ __launch_bounds__(kBlock) void k_setup(const uint64_t *__restrict__ ws, GraphDesc *__restrict__ desc,
-                                                  uint32_t G, uint64_t *__restrict__ tree,
-                                                  uint64_t *__restrict__ meta,
+__global__ __launch_bounds__(kBlock) void k_setup(GraphDesc *__restrict__ desc, uint32_t G,
+                                                  uint64_t *__restrict__ tree, uint64_t *__restrict__ meta,
                                                   unsigned long long *__restrict__ counts) {
     __shared__ uint64_t s_tot[kWavesPerBlock];
     uint64_t carry = 0, wsum = 0;
@@ -126,11 +147,11 @@ __global__ __launch_bounds__(kBlock) void k_setup(const uint64_t *__restrict__ w
         uint64_t tiles = 0, wc = 0;
         if (g < G) {
             GraphDesc &D = desc[g];
-            const uint64_t w0 = ws[D.vbase], W = ws[(uint64_t)D.vbase + D.n] - w0;
+            const uint64_t W = D.ws[D.n];
             const uint64_t b = D.w_begin < W ? D.w_begin : W;
             const uint64_t e = D.w_end < W ? D.w_end : W;
             wc = e > b ? e - b : 0;
-            D.wbase = w0 + b;
+            D.wbase = b;
             D.wcount = wc;
             const uint64_t per = (uint64_t)kWave * D.spread;
             tiles = (wc + per - 1) / per;
@@ -147,8 +168,9 @@ __global__ __launch_bounds__(kBlock) void k_setup(const uint64_t *__restrict__ w
         meta[kMetaTiles] = carry;
         meta[kMetaWedges] = wsum;
     }
-    if (tree != nullptr) {  // only for G == 1, where vbase == 0
+    if (tree != nullptr) {  // single-graph sets only
         const uint32_t n = desc[0].n;
+        const uint64_t *ws = desc[0].ws;
         for (uint32_t i = threadIdx.x; i < kTreeSize; i += kBlock) {
             const uint32_t h = i + 1;
             const int L = 31 - __clz(h);
@@ -170,37 +192,25 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *__restrict__ p, u
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-template <typename T>
-__device__ __forceinline__ uint32_t rightmost_le_desc(const GraphDesc *__restrict__ desc, uint32_t G, T key,
-                                                      uint64_t GraphDesc::*field) {
-    uint32_t lo = 0, hi = G - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) >> 1;
-        if (desc[mid].*field <= key) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
 // One wave per bitmap row (rows v in [n - min(n,a), n) of every graph).
 template <typename OffT>
-__global__ __launch_bounds__(kBlock) void k_bitmap_build(const OffT *__restrict__ rowPtr,
-                                                         const uint32_t *__restrict__ colInd,
-                                                         const GraphDesc *__restrict__ desc, uint32_t G,
-                                                         uint64_t rows, uint32_t *__restrict__ bm) {
+__global__ __launch_bounds__(kBlock) void k_bitmap_build(const GraphDesc *__restrict__ desc, uint32_t G,
+                                                         uint64_t rows) {
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t gw = (uint64_t)blockIdx.x * kWavesPerBlock + uni32(threadIdx.x >> 6);
     const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
     for (uint64_t r = gw; r < rows; r += nw) {
-        const uint32_t g = G > 1 ? rightmost_le_desc(desc, G, r, &GraphDesc::row_base) : 0u;
+        const uint32_t g = G > 1 ? graph_of(desc, G, r, &GraphDesc::row_base) : 0u;
         const GraphDesc &D = desc[g];
-        const uint32_t vl = D.n - D.bm_rows + (uint32_t)(r - D.row_base);
-        const uint64_t v = (uint64_t)D.vbase + vl;
-        const uint64_t rowbits = D.bm_size - binom2_d((uint64_t)D.n - vl) - vl - 1;
+        const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
+        const uint32_t *__restrict__ colInd = D.colInd;
+        uint32_t *__restrict__ bm = D.bitmap;
+        const uint32_t v = D.n - D.bm_rows + (uint32_t)(r - D.row_base);
+        const uint64_t rowbits = D.bm_size - binom2_d((uint64_t)D.n - v) - v - 1;
         const OffT b = rowPtr[v], e = rowPtr[v + 1];
         for (OffT j = b + lane; j < e; j += kWave) {
-            const uint64_t idx = rowbits + (colInd[j] - D.vbase);
-            atomicOr(&bm[D.bm_word_off + (idx >> 5)], 1u << (idx & 31u));
+            const uint64_t idx = rowbits + colInd[j];
+            atomicOr(&bm[idx >> 5], 1u << (idx & 31u));
         }
     }
 }
@@ -286,15 +296,12 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
     u_out[i] = u;
 }
 
-template <typename OffT>
-__global__ __launch_bounds__(kBlock) void k_count(const OffT *__restrict__ rowPtr,
-                                                  const uint32_t *__restrict__ colInd,
-                                                  const uint64_t *__restrict__ ws,
+template <typename OffT, int kWays>
+__global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ desc, uint32_t G,
                                                   const uint64_t *__restrict__ tree,
-                                                  const GraphDesc *__restrict__ desc, uint32_t G,
                                                   const uint64_t *__restrict__ meta,
-                                                  const uint32_t *__restrict__ bitmap,
-                                                  unsigned long long *__restrict__ counts, uint32_t pad_max) {
+                                                  unsigned long long *__restrict__ counts, uint32_t pad_max,
+                                                  uint32_t chunk) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_g[kWavesPerBlock];
     __shared__ uint64_t s_c[kWavesPerBlock];
@@ -316,21 +323,24 @@ __global__ __launch_bounds__(kBlock) void k_count(const OffT *__restrict__ rowPt
     uint64_t cur_cnt = 0;
 
     for (uint64_t t = t_lo + (uint64_t)jx * kWavesPerBlock + wid; t < t_hi; t += wpx) {
-        const uint32_t g = G > 1 ? rightmost_le_desc(desc, G, t, &GraphDesc::tile_base) : 0u;
+        const uint32_t g = G > 1 ? graph_of(desc, G, t, &GraphDesc::tile_base) : 0u;
         const GraphDesc &D = desc[g];
+        const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
+        const uint32_t *__restrict__ colInd = D.colInd;
+        const uint64_t *__restrict__ ws = D.ws;
+        const uint32_t *__restrict__ bitmap = D.bitmap;
         const uint32_t s = D.spread;
-        const uint32_t sc = s < kChunkMax ? s : kChunkMax;
+        const uint32_t sc = s < chunk ? s : chunk;  // wedges per lane staged at once
         const uint32_t pad = sc | 1u;
         const uint64_t wend = D.wbase + D.wcount;
         const uint64_t tile_start = D.wbase + (t - D.tile_base) * ((uint64_t)kWave * s);
         const uint64_t tile_end = tile_start + (uint64_t)kWave * s < wend ? tile_start + (uint64_t)kWave * s : wend;
-        const uint32_t vmax = D.vbase + D.n - 1;
-        const uint32_t thr = D.thr, vbase = D.vbase, gn = D.n;
-        const uint64_t bm_size = D.bm_size, bm_off = D.bm_word_off;
+        const uint32_t gn = D.n, vmax = gn - 1, thr = D.thr;
+        const uint64_t bm_size = D.bm_size;
 
         // wave-uniform vertex range [v_lo, v_hi] of the tile
         const uint32_t v_lo = (tree != nullptr) ? tree_search(tree, ws, gn, tile_start)
-                                                : rightmost_le(ws, D.vbase, vmax, tile_start);
+                                                : rightmost_le(ws, 0, vmax, tile_start);
         const uint32_t v_hi = gallop_le(ws, v_lo, vmax, tile_end - 1);
 
         // lane state: wedges [first, lend)
@@ -377,39 +387,68 @@ __global__ __launch_bounds__(kBlock) void k_count(const OffT *__restrict__ rowPt
                 lu[lane * pad + k] = u_out;
             }
             __syncwarp();
-            // phase 2: consecutive lanes probe consecutive wedges
+            // phase 2: consecutive lanes probe consecutive wedges; each lane
+            // advances kWays independent searches in lockstep so that their
+            // dependent loads overlap (memory-level parallelism per lane).
             const uint32_t magic = (65536u + kn - 1) / kn;
-            for (uint32_t j = 0; j < kn; j++) {
-                const uint32_t p = j * kWave + lane;
-                const uint32_t q = (p * magic) >> 16;
-                const uint32_t r = p - q * kn;
-                const uint32_t pw = lw[q * pad + r], pu = lu[q * pad + r];
-                bool hit = false;
-                if (pw != kSentinel) {
-                    if (pw >= thr) {
-                        const uint32_t wl = pw - vbase, ul = pu - vbase;
-                        const uint64_t idx = bm_size - binom2_d((uint64_t)gn - wl) + (ul - wl - 1);
-                        hit = (bitmap[bm_off + (idx >> 5)] >> (idx & 31u)) & 1u;
-                    } else {
-                        OffT b = rowPtr[pw];
-                        OffT len = rowPtr[pw + 1] - b;
-                        while (len > 0) {
-                            const OffT half = len >> 1;
-                            const uint32_t x = colInd[b + half];
-                            if (x == pu) {
-                                hit = true;
-                                break;
-                            }
-                            if (x < pu) {
-                                b += half + 1;
-                                len -= half + 1;
+            for (uint32_t j0 = 0; j0 < kn; j0 += kWays) {
+                uint32_t pw[kWays], pu[kWays];
+                OffT b[kWays], len[kWays];
+                bool hit[kWays];
+#pragma unroll
+                for (int k = 0; k < kWays; k++) {
+                    const uint32_t j = j0 + k;
+                    pw[k] = kSentinel;
+                    pu[k] = 0;
+                    if (j < kn) {
+                        const uint32_t p = j * kWave + lane;
+                        const uint32_t q = (p * magic) >> 16;
+                        const uint32_t r = p - q * kn;
+                        pw[k] = lw[q * pad + r];
+                        pu[k] = lu[q * pad + r];
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kWays; k++) {
+                    hit[k] = false;
+                    b[k] = 0;
+                    len[k] = 0;
+                    if (pw[k] != kSentinel) {
+                        if (pw[k] >= thr) {  // top-a bitmap (tc.cu:262-272)
+                            const uint64_t idx = bm_size - binom2_d((uint64_t)gn - pw[k]) + (pu[k] - pw[k] - 1);
+                            hit[k] = (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
+                        } else {  // sorted-list probe of adj+(w) (tc.cu:273-286)
+                            b[k] = rowPtr[pw[k]];
+                            len[k] = rowPtr[pw[k] + 1] - b[k];
+                        }
+                    }
+                }
+                for (;;) {
+                    bool more = false;
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) more |= len[k] > 0;
+                    if (!more) break;
+                    uint32_t x[kWays];
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) x[k] = len[k] > 0 ? colInd[b[k] + (len[k] >> 1)] : 0u;
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) {
+                        if (len[k] > 0) {
+                            const OffT half = len[k] >> 1;
+                            if (x[k] == pu[k]) {
+                                hit[k] = true;
+                                len[k] = 0;
+                            } else if (x[k] < pu[k]) {
+                                b[k] += half + 1;
+                                len[k] -= half + 1;
                             } else {
-                                len = half;
+                                len[k] = half;
                             }
                         }
                     }
                 }
-                tile_cnt += (uint64_t)__popcll(__ballot(hit));
+#pragma unroll
+                for (int k = 0; k < kWays; k++) tile_cnt += (uint64_t)__popcll(__ballot(hit[k]));
             }
             __syncwarp();
         }
@@ -442,17 +481,39 @@ __global__ __launch_bounds__(kBlock) void k_count(const OffT *__restrict__ rowPt
     }
 }
 
+// Plain streaming copy (16-byte vectors), the reference kernel for scaling
+// the HBM byte counters of a profiler run.
+__global__ __launch_bounds__(kBlock) void k_copy_u32x4(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                       uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- launches
 
 // Launches KERNEL<uint64_t> or KERNEL<uint32_t> by row-offset width.
-#define WTC_LAUNCH_OFF(ob, KERNEL, GRID, BLOCK, SHMEM, STREAM, ROWPTR, ...)                                      \
-    [&]() -> hipError_t {                                                                                        \
-        if ((ob) == 8)                                                                                           \
-            KERNEL<uint64_t><<<GRID, BLOCK, SHMEM, STREAM>>>(static_cast<const uint64_t *>(ROWPTR), __VA_ARGS__); \
-        else                                                                                                     \
-            KERNEL<uint32_t><<<GRID, BLOCK, SHMEM, STREAM>>>(static_cast<const uint32_t *>(ROWPTR), __VA_ARGS__); \
-        return hipGetLastError();                                                                                \
+#define WTC_LAUNCH_OFF(ob, KERNEL, GRID, BLOCK, SHMEM, STREAM, ...)               \
+    [&]() -> hipError_t {                                                         \
+        if ((ob) == 8)                                                            \
+            KERNEL<uint64_t><<<GRID, BLOCK, SHMEM, STREAM>>>(__VA_ARGS__);        \
+        else                                                                      \
+            KERNEL<uint32_t><<<GRID, BLOCK, SHMEM, STREAM>>>(__VA_ARGS__);        \
+        return hipGetLastError();                                                 \
     }()
+
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
+    const uint64_t n = bytes / 16;
+    if (n == 0) return hipSuccess;
+    k_copy_u32x4<<<dim3(4096), dim3(kBlock), 0, st>>>(static_cast<const uint4 *>(src), static_cast<uint4 *>(dst), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(uint32_t *p, uint64_t words, uint32_t v, hipStream_t st) {
+    if (words == 0) return hipSuccess;
+    const uint64_t fb = (words + kBlock - 1) / kBlock;
+    k_fill_u32<<<dim3((uint32_t)(fb < 4096 ? fb : 4096)), dim3(kBlock), 0, st>>>(p, words, v);
+    return hipGetLastError();
+}
 
 hipError_t launch_debug_decode(const uint32_t *d, const uint64_t *iv, uint64_t k, uint32_t *w, uint32_t *u,
                                hipStream_t st) {
@@ -461,36 +522,33 @@ hipError_t launch_debug_decode(const uint32_t *d, const uint64_t *iv, uint64_t k
     return hipGetLastError();
 }
 
-uint32_t prefix_blocks(uint32_t N) { return (uint32_t)(((uint64_t)N + kScanChunk - 1) / kScanChunk); }
+uint32_t prefix_blocks(uint32_t n) { return (uint32_t)(((uint64_t)n + kScanChunk - 1) / kScanChunk); }
 
-hipError_t launch_prefix(const void *rowPtr, int ob, uint32_t N, uint64_t *partials, uint64_t *ws,
+hipError_t launch_prefix(const GraphDesc *desc, uint32_t G, int ob, uint32_t nblocks, uint64_t *partials,
                          hipStream_t st) {
-    const uint32_t nb = prefix_blocks(N);
-    if (nb == 0) return hipSuccess;  // callers never pass an empty vertex set
-    hipError_t e = WTC_LAUNCH_OFF(ob, k_wedge_partials, dim3(nb), dim3(kBlock), 0, st, rowPtr, N, partials);
+    if (nblocks == 0) return hipSuccess;
+    hipError_t e = WTC_LAUNCH_OFF(ob, k_wedge_partials, dim3(nblocks), dim3(kBlock), 0, st, desc, G, partials);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kBlock), 0, st, partials, nb);
+    k_scan_partials<<<dim3(1), dim3(kBlock), 0, st>>>(partials, nblocks);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    return WTC_LAUNCH_OFF(ob, k_wedge_scan, dim3(nb), dim3(kBlock), 0, st, rowPtr, N,
-                          (const uint64_t *)partials, ws);
+    return WTC_LAUNCH_OFF(ob, k_wedge_scan, dim3(nblocks), dim3(kBlock), 0, st, desc, G,
+                          (const uint64_t *)partials);
 }
 
-hipError_t launch_setup(const uint64_t *ws, GraphDesc *desc, uint32_t G, uint64_t *tree, uint64_t *meta,
-                        unsigned long long *counts, hipStream_t st) {
-    hipLaunchKernelGGL(k_setup, dim3(1), dim3(kBlock), 0, st, ws, desc, G, tree, meta, counts);
+hipError_t launch_setup(GraphDesc *desc, uint32_t G, uint64_t *tree, uint64_t *meta, unsigned long long *counts,
+                        hipStream_t st) {
+    k_setup<<<dim3(1), dim3(kBlock), 0, st>>>(desc, G, tree, meta, counts);
     return hipGetLastError();
 }
 
-hipError_t launch_bitmap(const void *rowPtr, int ob, const uint32_t *colInd, const GraphDesc *desc, uint32_t G,
-                         uint64_t rows, uint32_t *bm, uint64_t words, hipStream_t st) {
+hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t rows, uint32_t *bm, uint64_t words,
+                         hipStream_t st) {
     if (words == 0 || rows == 0) return hipSuccess;
-    uint64_t fb = (words + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)(fb < 4096 ? fb : 4096)), dim3(kBlock), 0, st, bm, words, 0u);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_fill(bm, words, 0u, st);
     if (e != hipSuccess) return e;
-    uint64_t bb = (rows + kWavesPerBlock - 1) / kWavesPerBlock;
-    return WTC_LAUNCH_OFF(ob, k_bitmap_build, dim3((uint32_t)(bb < 8192 ? bb : 8192)), dim3(kBlock), 0, st, rowPtr,
-                          colInd, desc, G, rows, bm);
+    const uint64_t bb = (rows + kWavesPerBlock - 1) / kWavesPerBlock;
+    return WTC_LAUNCH_OFF(ob, k_bitmap_build, dim3((uint32_t)(bb < 8192 ? bb : 8192)), dim3(kBlock), 0, st, desc, G,
+                          rows);
 }
 
 size_t count_lds_bytes(uint32_t max_spread) {
@@ -499,16 +557,27 @@ size_t count_lds_bytes(uint32_t max_spread) {
     return (size_t)kWavesPerBlock * 2u * kWave * pad * sizeof(uint32_t);
 }
 
-uint32_t count_pad(uint32_t max_spread) {
+static uint32_t count_pad(uint32_t max_spread) {
     const uint32_t sc = max_spread < kChunkMax ? max_spread : kChunkMax;
     return sc | 1u;
 }
 
-hipError_t count_grid(int ob, uint32_t max_spread, int num_cus, uint32_t *grid) {
+template <int W>
+static hipError_t occupancy_w(int ob, size_t shm, int *per_cu) {
+    return ob == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint64_t, W>, kBlock, shm)
+                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint32_t, W>, kBlock, shm);
+}
+
+hipError_t count_grid(int ob, uint32_t max_spread, int ways, int num_cus, uint32_t *grid) {
     int per_cu = 0;
     const size_t shm = count_lds_bytes(max_spread);
-    hipError_t e = ob == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count<uint64_t>, kBlock, shm)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_count<uint32_t>, kBlock, shm);
+    hipError_t e;
+    switch (ways) {
+        case 1: e = occupancy_w<1>(ob, shm, &per_cu); break;
+        case 2: e = occupancy_w<2>(ob, shm, &per_cu); break;
+        case 8: e = occupancy_w<8>(ob, shm, &per_cu); break;
+        default: e = occupancy_w<4>(ob, shm, &per_cu); break;
+    }
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     uint64_t g = (uint64_t)per_cu * (uint64_t)(num_cus > 0 ? num_cus : 1);
@@ -518,13 +587,30 @@ hipError_t count_grid(int ob, uint32_t max_spread, int num_cus, uint32_t *grid) 
     return hipSuccess;
 }
 
-hipError_t launch_count(const void *rowPtr, int ob, const uint32_t *colInd, const uint64_t *ws, const uint64_t *tree,
-                        const GraphDesc *desc, uint32_t G, const uint64_t *meta, const uint32_t *bm,
-                        unsigned long long *counts, uint32_t max_spread, uint32_t grid, hipStream_t st) {
-    const size_t shm = count_lds_bytes(max_spread);
-    const uint32_t pad = count_pad(max_spread);
-    return WTC_LAUNCH_OFF(ob, k_count, dim3(grid), dim3(kBlock), shm, st, rowPtr, colInd, ws, tree, desc, G, meta, bm,
-                          counts, pad);
+template <int W>
+static hipError_t launch_count_w(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree,
+                                 const uint64_t *meta, unsigned long long *counts, uint32_t pad, uint32_t chunk,
+                                 size_t shm, uint32_t grid, hipStream_t st) {
+    if (ob == 8)
+        k_count<uint64_t, W><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
+    else
+        k_count<uint32_t, W><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
+    return hipGetLastError();
+}
+
+hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, const uint64_t *meta,
+                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, uint32_t grid,
+                        hipStream_t st) {
+    if (chunk < 1 || chunk > kChunkMax) return hipErrorInvalidValue;
+    const uint32_t eff = max_spread < chunk ? max_spread : chunk;
+    const size_t shm = count_lds_bytes(eff);
+    const uint32_t pad = count_pad(eff);
+    switch (ways) {
+        case 1: return launch_count_w<1>(desc, G, ob, tree, meta, counts, pad, chunk, shm, grid, st);
+        case 2: return launch_count_w<2>(desc, G, ob, tree, meta, counts, pad, chunk, shm, grid, st);
+        case 8: return launch_count_w<8>(desc, G, ob, tree, meta, counts, pad, chunk, shm, grid, st);
+        default: return launch_count_w<4>(desc, G, ob, tree, meta, counts, pad, chunk, shm, grid, st);
+    }
 }
 
 }  // namespace wtc

@@ -1,5 +1,5 @@
 // wtc_launch.h -- host-callable launchers for the kernels in wtc_kernels.hip.
-// `ob` is the byte width of the row offsets (4 or 8).
+// `ob` is the byte width of the row offsets of the graph set (4 or 8).
 #pragma once
 #include "wtc_kernels.h"
 
@@ -7,16 +7,28 @@ namespace wtc {
 
 hipError_t launch_debug_decode(const uint32_t *d, const uint64_t *iv, uint64_t k, uint32_t *w, uint32_t *u,
                                hipStream_t st);
-uint32_t prefix_blocks(uint32_t N);
-hipError_t launch_prefix(const void *rowPtr, int ob, uint32_t N, uint64_t *partials, uint64_t *ws, hipStream_t st);
-hipError_t launch_setup(const uint64_t *ws, GraphDesc *desc, uint32_t G, uint64_t *tree, uint64_t *meta,
-                        unsigned long long *counts, hipStream_t st);
-hipError_t launch_bitmap(const void *rowPtr, int ob, const uint32_t *colInd, const GraphDesc *desc, uint32_t G,
-                         uint64_t rows, uint32_t *bm, uint64_t words, hipStream_t st);
-size_t count_lds_bytes(uint32_t max_spread);
-hipError_t count_grid(int ob, uint32_t max_spread, int num_cus, uint32_t *grid);
-hipError_t launch_count(const void *rowPtr, int ob, const uint32_t *colInd, const uint64_t *ws, const uint64_t *tree,
-                        const GraphDesc *desc, uint32_t G, const uint64_t *meta, const uint32_t *bm,
-                        unsigned long long *counts, uint32_t max_spread, uint32_t grid, hipStream_t st);
+hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
+hipError_t launch_fill(uint32_t *p, uint64_t words, uint32_t v, hipStream_t st);
+
+// Prefix-scan blocks one graph of n vertices needs.
+uint32_t prefix_blocks(uint32_t n);
+// Wedge prefix of every graph of the set: nblocks = sum of prefix_blocks(n),
+// partials holds nblocks entries.
+hipError_t launch_prefix(const GraphDesc *desc, uint32_t G, int ob, uint32_t nblocks, uint64_t *partials,
+                         hipStream_t st);
+hipError_t launch_setup(GraphDesc *desc, uint32_t G, uint64_t *tree, uint64_t *meta, unsigned long long *counts,
+                        hipStream_t st);
+// Clears the set's bitmap region (words) and sets the bits of every graph.
+hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t rows, uint32_t *bm, uint64_t words,
+                         hipStream_t st);
+// LDS bytes of a counting block that stages `staged` wedges per lane.
+size_t count_lds_bytes(uint32_t staged);
+// `ways`: independent probes each lane advances together (1, 2, 4 or 8);
+// `staged` = min(max spread, chunk).
+hipError_t count_grid(int ob, uint32_t staged, int ways, int num_cus, uint32_t *grid);
+// `chunk` (1..kChunkMax): wedges per lane staged in LDS per phase.
+hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, const uint64_t *meta,
+                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, uint32_t grid,
+                        hipStream_t st);
 
 }  // namespace wtc

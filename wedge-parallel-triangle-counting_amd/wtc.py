@@ -105,6 +105,8 @@ _sig = {
     "wtc_ctx_debug_prefix": (C.c_int, [_P, C.c_uint64, _P, _P]),
     "wtc_ctx_debug_bitmap": (C.c_int, [_P, C.c_uint64, _P]),
     "wtc_debug_decode": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _P]),
+    "wtc_debug_copy": (C.c_int, [_P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
+    "wtc_ctx_set_tuning": (C.c_int, [_P, C.c_int, C.c_uint32]),
     "wtc_alg_bytes": (None, [_BGP, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "wtc_batch_upload": (C.c_int, [_P, C.POINTER(_BGP), C.c_size_t, C.POINTER(_P)]),
     "wtc_batch_free": (None, [_P]),
@@ -318,6 +320,16 @@ class Context:
         tree = np.zeros(4095, dtype=np.uint64)
         _check(_lib.wtc_ctx_debug_prefix(self._p, n, ws.ctypes.data, tree.ctypes.data), "debug_prefix")
         return ws, tree
+
+    def set_tuning(self, probe_ways: int = 4, lds_chunk: int = 16):
+        """Counting-kernel knobs: probes advanced together per lane, wedges
+        per lane staged in LDS per phase."""
+        _check(_lib.wtc_ctx_set_tuning(self._p, probe_ways, lds_chunk), "set_tuning")
+
+    def debug_copy(self, nbytes: int, reps: int = 1) -> float:
+        ms = C.c_double()
+        _check(_lib.wtc_debug_copy(self._p, nbytes, reps, C.byref(ms)), "debug_copy")
+        return float(ms.value)
 
     def debug_decode(self, d, iv):
         d = np.ascontiguousarray(d, dtype=np.uint32)
