@@ -215,6 +215,52 @@ int wtc_batch_count(wtc_ctx *ctx, wtc_batch *b, const uint32_t *spread, const ui
  * counterpart of calling wtc_tc_gpu once per graph). */
 int wtc_batch_count_host(wtc_ctx *ctx, const wtc_big_graph *const *graphs, size_t k, const uint32_t *spread,
                          const uint32_t *adjacency_matrix_len, uint64_t *triangles, wtc_count_stats *st);
+/* ---------------------------------------------------------- GPU: serving
+ * A request loop with continuous batching over device-resident CSRs (no
+ * counterpart in the reference).  Graphs are registered once; requests name
+ * a graph.  A worker thread admits requests by HBM footprint (resident CSR +
+ * per-request wedge prefix and bitmap, carved from a fixed device pool;
+ * least-recently-used idle graphs are evicted), uploads missing graphs on
+ * their own stream, and starts a batch (one graph set, one launch sequence)
+ * whenever one of `slots` execution slots is free, so later requests run
+ * while earlier batches are still on the device.  Thread-safe. */
+typedef struct wtc_server wtc_server;
+typedef struct {
+    uint64_t hbm_budget_bytes; /* device pool (default 8 GiB)                     */
+    uint32_t max_batch;        /* graphs per batch (default 256)                  */
+    uint32_t slots;            /* batches in flight (default 2, at most 8)        */
+    uint32_t queue_capacity;   /* queued requests before WTC_ERR_BUSY (65536)     */
+    int32_t probe_ways;        /* counting-kernel tuning, 0 = default             */
+    uint32_t lds_chunk;        /* counting-kernel tuning, 0 = default             */
+} wtc_server_config;
+typedef struct {
+    double queue_ms;      /* submit -> batch start                                */
+    double service_ms;    /* batch start -> completion seen                       */
+    double latency_ms;    /* submit -> completion seen                            */
+    uint32_t batch_graphs;
+    uint64_t batch_id;
+} wtc_request_stats;
+typedef struct {
+    uint64_t graphs, submitted, completed, rejected, batches, max_batch_graphs;
+    uint64_t uploads, evictions, resident_bytes, resident_graphs, queued;
+} wtc_server_stats;
+
+int wtc_server_create(int device, const wtc_server_config *cfg, wtc_server **out);
+/* Validates and copies an oriented CSR (numEdges < 2^32) to pinned host
+ * memory; the caller's graph may be freed afterwards. */
+int wtc_server_register(wtc_server *srv, const wtc_big_graph *oriented, uint64_t *graph_id);
+/* WTC_ERR_ADMIT if the request can never fit the pool, WTC_ERR_BUSY if the
+ * queue is full. */
+int wtc_server_submit(wtc_server *srv, uint64_t graph_id, uint32_t spread, uint32_t adjacency_matrix_len,
+                      uint64_t *request_id);
+/* Waits (timeout_us < 0: forever; 0: poll) for a request; WTC_ERR_BUSY if not
+ * done in time.  A finished request is forgotten after a successful wait. */
+int wtc_server_wait(wtc_server *srv, uint64_t request_id, int64_t timeout_us, uint64_t *triangles,
+                    wtc_request_stats *st);
+int wtc_server_get_stats(wtc_server *srv, wtc_server_stats *st);
+/* Finishes queued and running requests, then frees everything. */
+void wtc_server_destroy(wtc_server *srv);
+
 /* The autotuner's choice for a graph of n vertices, m edges and W wedges. */
 void wtc_autotune(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *adjacency_matrix_len);
 

@@ -1,0 +1,36 @@
+"""The tc / tc_big_graphs command lines end to end on the GPU: same flags and
+results table as the reference's main (tc.cu:816-918)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG = os.path.join(ROOT, "wedge-parallel-triangle-counting_amd")
+HEADER = ["graph", "n", "m", "s", "a", "triangles", "prepro (s)", "GPU copy (s)", "GPU exec (s)",
+          "GPU total (s)", "CPU+GPU (s)"]
+
+
+@pytest.mark.parametrize("binary", ["tc", "tc_big_graphs"])
+def test_cli_table(golden, binary):
+    for fx in golden[:6]:
+        flag = "-m" if fx["format"] == "mm" else "-e"
+        args = [os.path.join(PKG, binary), flag, fx["path"], "-s", "5", "-a", "64", "-l", "3"]
+        if fx["zero_indexed"]:
+            args.append("-z")
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        lines = r.stdout.strip().splitlines()
+        assert len(lines) == 1 + 3  # header + loop count rows (warm-up row dropped)
+        assert re.split(r"\s{2,}", lines[0].strip()) == HEADER
+        for row in lines[1:]:
+            cols = row.split()
+            assert cols[0] == fx["path"]
+            n, m, s, a, tri = (int(x) for x in cols[1:6])
+            assert (n, m, s, a, tri) == (fx["n"], fx["m"], 5, 64, fx["triangles"])
+            copy, exe, total = (float(x) for x in cols[7:10])
+            assert abs(copy + exe - total) < 2e-6
+            assert float(cols[10]) >= total - 1e-6

@@ -1,0 +1,83 @@
+"""Serving with continuous batching: every result equals the count of the
+same graph alone (and the oracle), whatever batch it ran in."""
+import random
+import threading
+
+import numpy as np
+import pytest
+
+import oracle as O
+import wtc
+
+pytestmark = pytest.mark.gpu
+
+
+def _pool():
+    graphs = []
+    for i, scale in enumerate((6, 8, 9, 10, 11, 12)):
+        graphs.append(wtc.preprocess(wtc.gen_rmat(scale, 16, seed=100 + i)))
+    graphs.append(wtc.preprocess(wtc.gen_erdos_renyi(3000, 0.004, seed=7)))
+    graphs.append(wtc.preprocess(wtc.graph_from_pairs(np.array([[0, 1], [1, 2], [2, 0]], np.uint32))))
+    return graphs
+
+
+def test_alone_equals_mix():
+    graphs = _pool()
+    expect = [O.tc_merge(g) for g in graphs]
+    srv = wtc.Server(0, hbm_budget_bytes=256 << 20, max_batch=16, slots=2)
+    ids = [srv.register(g) for g in graphs]
+    # alone: one request at a time
+    alone = []
+    for gid in ids:
+        cnt, st = srv.wait(srv.submit(gid, 5, 64))
+        assert st.batch_graphs == 1
+        alone.append(cnt)
+    assert alone == expect
+    # mix: many concurrent requests with varied s and a, submitted from two threads
+    rng = random.Random(3)
+    plan = [(rng.randrange(len(ids)), rng.choice([1, 3, 5, 8, 17]), rng.choice([0, 32, 64, 1024]))
+            for _ in range(300)]
+    rids = [None] * len(plan)
+
+    def submitter(part):
+        for k in part:
+            gi, s, a = plan[k]
+            rids[k] = srv.submit(ids[gi], s, a)
+
+    th = [threading.Thread(target=submitter, args=(range(j, len(plan), 2),)) for j in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    batch_sizes = set()
+    for k, rid in enumerate(rids):
+        cnt, st = srv.wait(rid)
+        assert cnt == expect[plan[k][0]], plan[k]
+        batch_sizes.add(st.batch_graphs)
+    stats = srv.stats()
+    srv.close()
+    assert stats["completed"] == len(ids) + len(plan)
+    assert max(batch_sizes) > 1  # requests were batched together
+
+
+def test_eviction_under_small_budget():
+    graphs = _pool()
+    expect = [O.tc_merge(g) for g in graphs]
+    srv = wtc.Server(0, hbm_budget_bytes=6 << 20, max_batch=4, slots=2)
+    ids = [srv.register(g) for g in graphs]
+    rids = [(k % len(ids), srv.submit(ids[k % len(ids)], 4, 32)) for k in range(60)]
+    for gi, rid in rids:
+        assert srv.wait(rid)[0] == expect[gi]
+    stats = srv.stats()
+    srv.close()
+    assert stats["evictions"] > 0
+
+
+def test_admission_rejects_oversized():
+    g = wtc.preprocess(wtc.gen_rmat(12, 16, seed=1))
+    srv = wtc.Server(0, hbm_budget_bytes=64 << 10, max_batch=4)
+    gid = srv.register(g)
+    with pytest.raises(wtc.WtcError) as e:
+        srv.submit(gid, 5, 64)
+    assert e.value.code == -9
+    srv.close()

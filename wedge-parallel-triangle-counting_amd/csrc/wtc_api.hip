@@ -1,5 +1,6 @@
-// wtc_api.hip -- device side of the C ABI: contexts, resident graphs, the
-// reference-style tc_GPU drop-ins and the batched path.
+// wtc_api.hip -- device side of the C ABI: contexts, the asynchronous
+// graph-set pipeline, resident graphs, the reference-style tc_GPU drop-ins
+// and the batched path.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -9,108 +10,15 @@
 #include <string>
 #include <vector>
 
-#include "host_util.h"
-#include "wtc.h"
-#include "wtc_kernels.h"
-#include "wtc_launch.h"
+#include "wtc_internal.h"
 
 using namespace wtc;
 
-namespace {
-
-thread_local std::string g_last_error;
-
-int hip_fail(const char *what, hipError_t e) {
-    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
-    return WTC_ERR_HIP;
-}
-
-#define HIPC(call)                                             \
-    do {                                                       \
-        hipError_t _e = (call);                                \
-        if (_e != hipSuccess) return hip_fail(#call, _e);      \
-    } while (0)
-
-// Grow-only device buffer.
-struct DevBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap && p) return hipSuccess;
-        if (p) {
-            hipError_t e = hipFree(p);
-            p = nullptr;
-            cap = 0;
-            if (e != hipSuccess) return e;
-        }
-        size_t want = std::max<size_t>(bytes, 256);
-        hipError_t e = hipMalloc(&p, want);
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T>
-    T *as() const {
-        return static_cast<T *>(p);
-    }
-};
-
-// Grow-only pinned host buffer.
-struct PinBuf {
-    void *p = nullptr;
-    size_t cap = 0;
-    hipError_t ensure(size_t bytes) {
-        if (bytes <= cap && p) return hipSuccess;
-        if (p) {
-            (void)hipHostFree(p);
-            p = nullptr;
-            cap = 0;
-        }
-        size_t want = std::max<size_t>(bytes, 4096);
-        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
-        if (e == hipSuccess) cap = want;
-        return e;
-    }
-    void release() {
-        if (p) (void)hipHostFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-    template <class T>
-    T *as() const {
-        return static_cast<T *>(p);
-    }
-};
-
-// Per-call device workspace of a graph set.
-struct Workspace {
-    DevBuf ws, partials, tree, bm, desc, out;
-    void release() {
-        ws.release();
-        partials.release();
-        tree.release();
-        bm.release();
-        desc.release();
-        out.release();
-    }
-};
-
-}  // namespace
-
 struct wtc_ctx {
     int device = 0;
-    int num_cus = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
-    Workspace work;
-    PinBuf h_desc, h_out, h_stage;
-    int ways = 4;         // independent probes per lane in the counting kernel
-    uint32_t chunk = 16;  // wedges per lane staged in LDS per phase (1..kChunkMax)
-    uint32_t grid_cache[2][4][kChunkMax + 2] = {};
+    Slot slot;
+    Tuning tune;
+    PinBuf h_stage;
 };
 
 struct wtc_dgraph {
@@ -132,18 +40,85 @@ struct wtc_batch {
     DevBuf rowPtr, colInd;
 };
 
-namespace {
+namespace wtc {
 
-int set_device(int dev) {
-    int cnt = 0;
-    hipError_t e = hipGetDeviceCount(&cnt);
-    if (e != hipSuccess || cnt <= 0) {
-        g_last_error = "no HIP device";
-        return WTC_ERR_NODEV;
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int hip_fail(const char *what, hipError_t e) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return WTC_ERR_HIP;
+}
+
+hipError_t DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+        hipError_t e = hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return e;
     }
-    if (dev < 0 || dev >= cnt) return WTC_ERR_ARG;
-    HIPC(hipSetDevice(dev));
+    const size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e == hipSuccess) cap = want;
+    else p = nullptr;
+    return e;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+hipError_t PinBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+        (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    const size_t want = std::max<size_t>(bytes, 4096);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e == hipSuccess) cap = want;
+    else p = nullptr;
+    return e;
+}
+
+void PinBuf::release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+int Slot::create() {
+    WTC_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto &e : ev) WTC_HIPC(hipEventCreate(&e));
     return WTC_OK;
+}
+
+void Slot::release_device() {
+    ws.release();
+    partials.release();
+    tree.release();
+    bm.release();
+    desc.release();
+    out.release();
+}
+
+void Slot::destroy() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    release_device();
+    h_desc.release();
+    h_out.release();
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    for (auto &e : ev) e = nullptr;
 }
 
 uint64_t bm_words_for(uint32_t a) { return (binom2(a) + 31u) / 32u; }
@@ -153,8 +128,12 @@ int validate_oriented_serial(const wtc_big_graph *g) {
     return validate_oriented_rows(g->numVertices, g->numEdges, g->rowPtr, g->colInd, 0, g->numVertices);
 }
 
-// Host-filled descriptor of one graph of a set (device pointers to its CSR;
-// workspace slices are assigned by run_set).
+int check_args(uint32_t spread, uint32_t a) {
+    if (spread == 0) return WTC_ERR_ARG;
+    if (a % 32 != 0) return WTC_ERR_ARG;
+    return WTC_OK;
+}
+
 GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint32_t spread, uint32_t a,
                     uint64_t w_begin, uint64_t w_end) {
     GraphDesc d{};
@@ -171,8 +150,96 @@ GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint
     return d;
 }
 
+static int ways_index(int w) { return w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3; }
+
+int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd) {
+    const uint32_t G = (uint32_t)hd.size();
+    if (G == 0) return WTC_ERR_ARG;
+    uint32_t max_spread = 1;
+    uint64_t ws_n = 0, bm_words = 0, rows = 0, blocks = 0;
+    for (const auto &d : hd) {
+        if (d.n == 0 || d.spread == 0) return WTC_ERR_ARG;
+        max_spread = std::max(max_spread, d.spread);
+        if (!d.ws) ws_n += (uint64_t)d.n + 1;
+        if (d.a && !d.bitmap) bm_words += bm_words_for(d.a);
+        rows += d.bm_rows;
+        blocks += prefix_blocks(d.n);
+    }
+    if (blocks > 0x7fffffffULL) return WTC_ERR_RANGE;
+    const bool single = G == 1;
+    if (ws_n) WTC_HIPC(s.ws.ensure(ws_n * sizeof(uint64_t)));
+    WTC_HIPC(s.partials.ensure(blocks * sizeof(uint64_t)));
+    if (single) WTC_HIPC(s.tree.ensure(kTreeSize * sizeof(uint64_t)));
+    WTC_HIPC(s.desc.ensure((size_t)G * sizeof(GraphDesc)));
+    WTC_HIPC(s.out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
+    if (bm_words) WTC_HIPC(s.bm.ensure(bm_words * sizeof(uint32_t)));
+    WTC_HIPC(s.h_desc.ensure((size_t)G * sizeof(GraphDesc)));
+    WTC_HIPC(s.h_out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
+    uint64_t ws_off = 0, bm_off = 0, row_off = 0, blk_off = 0, clear_off = 0;
+    for (auto &d : hd) {
+        d.bm_clear_base = clear_off;
+        clear_off += d.a ? bm_words_for(d.a) : 0;
+        if (!d.ws) {
+            d.ws = s.ws.as<uint64_t>() + ws_off;
+            ws_off += (uint64_t)d.n + 1;
+        }
+        if (d.a && !d.bitmap) {
+            d.bitmap = s.bm.as<uint32_t>() + bm_off;
+            bm_off += bm_words_for(d.a);
+        }
+        d.row_base = row_off;
+        d.blk_base = blk_off;
+        row_off += d.bm_rows;
+        blk_off += prefix_blocks(d.n);
+    }
+    std::memcpy(s.h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
+
+    uint64_t *meta = s.out.as<uint64_t>();
+    unsigned long long *dcounts = reinterpret_cast<unsigned long long *>(meta + kMetaSlots);
+    GraphDesc *ddesc = s.desc.as<GraphDesc>();
+    uint64_t *tree = single ? s.tree.as<uint64_t>() : nullptr;
+
+    const int oi = ob == 8 ? 1 : 0;
+    const uint32_t staged = std::min<uint32_t>(max_spread, t.chunk);
+    uint32_t &gc = t.grid_cache[oi][ways_index(t.ways)][staged];
+    if (!gc) WTC_HIPC(count_grid(ob, staged, t.ways, t.num_cus, &gc));
+    s.G = G;
+    s.grid = gc;
+
+    WTC_HIPC(hipMemcpyAsync(ddesc, s.h_desc.p, (size_t)G * sizeof(GraphDesc), hipMemcpyHostToDevice, s.stream));
+    WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
+    WTC_HIPC(launch_prefix(ddesc, G, ob, (uint32_t)blocks, s.partials.as<uint64_t>(), s.stream));
+    WTC_HIPC(launch_setup(ddesc, G, tree, meta, dcounts, s.stream));
+    WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
+    WTC_HIPC(launch_bitmap(ddesc, G, ob, rows, clear_off, s.stream));
+    WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
+    WTC_HIPC(launch_count(ddesc, G, ob, tree, meta, dcounts, max_spread, t.chunk, t.ways, s.grid, s.stream));
+    WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
+    WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + G) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                            s.stream));
+    return WTC_OK;
+}
+
+bool slot_done(Slot &s) { return hipStreamQuery(s.stream) == hipSuccess; }
+
+int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt) {
+    WTC_HIPC(hipStreamSynchronize(s.stream));
+    const uint64_t *h = s.h_out.as<uint64_t>();
+    for (uint32_t g = 0; g < s.G; g++) counts[g] = h[kMetaSlots + g];
+    if (wedges) *wedges = h[kMetaWedges];
+    if (rt) {
+        WTC_HIPC(hipEventElapsedTime(&rt->prefix_ms, s.ev[0], s.ev[1]));
+        WTC_HIPC(hipEventElapsedTime(&rt->bitmap_ms, s.ev[1], s.ev[2]));
+        WTC_HIPC(hipEventElapsedTime(&rt->count_ms, s.ev[2], s.ev[3]));
+    }
+    return WTC_OK;
+}
+
+}  // namespace wtc
+
+namespace {
+
 bool valid_ways(int w) { return w == 1 || w == 2 || w == 4 || w == 8; }
-int ways_index(int w) { return w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3; }
 
 // Default probes per lane for new contexts: WTC_PROBE_WAYS (1, 2, 4 or 8),
 // else 4.
@@ -182,90 +249,39 @@ int default_probe_ways() {
     return valid_ways(v) ? v : 4;
 }
 
-struct RunTimes {
-    float prefix_ms = 0, bitmap_ms = 0, count_ms = 0;
-};
-
-// Runs the device pipeline over a resident graph set and returns the
-// per-graph counts on the host.  `work` provides the device workspace: the
-// wedge prefixes, scan partials and bitmaps of all graphs, laid out back to
-// back.
-int run_set(wtc_ctx *c, Workspace &work, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, uint64_t *wedges,
-            uint32_t *grid_out, RunTimes *rt) {
-    const uint32_t G = (uint32_t)hd.size();
-    if (G == 0) return WTC_OK;
-    uint32_t max_spread = 1;
-    uint64_t ws_n = 0, bm_words = 0, rows = 0, blocks = 0;
-    for (const auto &d : hd) {
-        if (d.n == 0 || d.spread == 0) return WTC_ERR_ARG;
-        max_spread = std::max(max_spread, d.spread);
-        ws_n += (uint64_t)d.n + 1;
-        bm_words += d.a ? bm_words_for(d.a) : 0;
-        rows += d.bm_rows;
-        blocks += prefix_blocks(d.n);
+int set_device(int dev) {
+    int cnt = 0;
+    hipError_t e = hipGetDeviceCount(&cnt);
+    if (e != hipSuccess || cnt <= 0) {
+        set_error("no HIP device");
+        return WTC_ERR_NODEV;
     }
-    if (blocks > 0x7fffffffULL) return WTC_ERR_RANGE;
-    const bool single = G == 1;
-    HIPC(work.ws.ensure(ws_n * sizeof(uint64_t)));
-    HIPC(work.partials.ensure(blocks * sizeof(uint64_t)));
-    HIPC(work.tree.ensure(kTreeSize * sizeof(uint64_t)));
-    HIPC(work.desc.ensure((size_t)G * sizeof(GraphDesc)));
-    HIPC(work.out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
-    if (bm_words) HIPC(work.bm.ensure(bm_words * sizeof(uint32_t)));
-    HIPC(c->h_desc.ensure((size_t)G * sizeof(GraphDesc)));
-    HIPC(c->h_out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
-    uint64_t ws_off = 0, bm_off = 0, row_off = 0, blk_off = 0;
-    for (auto &d : hd) {
-        d.ws = work.ws.as<uint64_t>() + ws_off;
-        d.bitmap = d.a ? work.bm.as<uint32_t>() + bm_off : nullptr;
-        d.row_base = row_off;
-        d.blk_base = blk_off;
-        ws_off += (uint64_t)d.n + 1;
-        bm_off += d.a ? bm_words_for(d.a) : 0;
-        row_off += d.bm_rows;
-        blk_off += prefix_blocks(d.n);
-    }
-    std::memcpy(c->h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
-
-    uint64_t *meta = work.out.as<uint64_t>();
-    unsigned long long *dcounts = reinterpret_cast<unsigned long long *>(meta + kMetaSlots);
-    GraphDesc *ddesc = work.desc.as<GraphDesc>();
-    uint64_t *tree = single ? work.tree.as<uint64_t>() : nullptr;
-    hipStream_t st = c->stream;
-
-    const int oi = ob == 8 ? 1 : 0;
-    const uint32_t sci = std::min<uint32_t>(max_spread, c->chunk);
-    const int ways = c->ways;
-    uint32_t &gc = c->grid_cache[oi][ways_index(ways)][sci];
-    if (!gc) HIPC(count_grid(ob, sci, ways, c->num_cus, &gc));
-    const uint32_t grid = gc;
-
-    HIPC(hipMemcpyAsync(ddesc, c->h_desc.p, (size_t)G * sizeof(GraphDesc), hipMemcpyHostToDevice, st));
-    HIPC(hipEventRecord(c->ev[0], st));
-    HIPC(launch_prefix(ddesc, G, ob, (uint32_t)blocks, work.partials.as<uint64_t>(), st));
-    HIPC(launch_setup(ddesc, G, tree, meta, dcounts, st));
-    HIPC(hipEventRecord(c->ev[1], st));
-    HIPC(launch_bitmap(ddesc, G, ob, rows, work.bm.as<uint32_t>(), bm_words, st));
-    HIPC(hipEventRecord(c->ev[2], st));
-    HIPC(launch_count(ddesc, G, ob, tree, meta, dcounts, max_spread, c->chunk, ways, grid, st));
-    HIPC(hipEventRecord(c->ev[3], st));
-    HIPC(hipMemcpyAsync(c->h_out.p, meta, (size_t)(kMetaSlots + G) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    HIPC(hipStreamSynchronize(st));
-    const uint64_t *h = c->h_out.as<uint64_t>();
-    for (uint32_t g = 0; g < G; g++) counts[g] = h[kMetaSlots + g];
-    if (wedges) *wedges = h[kMetaWedges];
-    if (grid_out) *grid_out = grid;
-    if (rt) {
-        HIPC(hipEventElapsedTime(&rt->prefix_ms, c->ev[0], c->ev[1]));
-        HIPC(hipEventElapsedTime(&rt->bitmap_ms, c->ev[1], c->ev[2]));
-        HIPC(hipEventElapsedTime(&rt->count_ms, c->ev[2], c->ev[3]));
-    }
+    if (dev < 0 || dev >= cnt) return WTC_ERR_ARG;
+    WTC_HIPC(hipSetDevice(dev));
     return WTC_OK;
 }
 
-int check_args(uint32_t spread, uint32_t a) {
-    if (spread == 0) return WTC_ERR_ARG;
-    if (a % 32 != 0) return WTC_ERR_ARG;
+// Synchronous count of a graph set on the context's slot.
+int run_set(wtc_ctx *c, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, wtc_count_stats *st,
+            RunTimes *rt_out = nullptr) {
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = enqueue_set(c->slot, c->tune, ob, hd);
+    if (rc) return rc;
+    RunTimes rt;
+    uint64_t W = 0;
+    rc = finish_set(c->slot, counts, &W, &rt);
+    auto t1 = std::chrono::steady_clock::now();
+    if (rc) return rc;
+    if (rt_out) *rt_out = rt;
+    if (st) {
+        st->prefix_ms = rt.prefix_ms;
+        st->bitmap_ms = rt.bitmap_ms;
+        st->count_ms = rt.count_ms;
+        st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->wedges = W;
+        st->blocks = c->slot.grid;
+        st->launches = 1;
+    }
     return WTC_OK;
 }
 
@@ -295,16 +311,15 @@ int tc_gpu_impl(uint32_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colI
     std::lock_guard<std::mutex> lk(g_default_mu);
     wtc_ctx *c;
     if ((rc = default_ctx(&c))) return rc;
-    HIPC(hipSetDevice(c->device));
-    hipStream_t st = c->stream;
-    Workspace work;  // freed at the end of every call, like the reference
+    WTC_HIPC(hipSetDevice(c->device));
+    hipStream_t st = c->slot.stream;
     DevBuf drow, dcol;
-    auto cleanup = [&] {
-        work.release();
+    auto cleanup = [&] {  // every device buffer is freed at the end of the call, like the reference
+        c->slot.release_device();
         drow.release();
         dcol.release();
     };
-    HIPC(hipEventRecord(c->ev[4], st));
+    WTC_HIPC(hipEventRecord(c->slot.ev[0], st));
     hipError_t e;
     if ((e = drow.ensure(((size_t)n + 1) * sizeof(OffT))) != hipSuccess ||
         (e = dcol.ensure((size_t)std::max<uint64_t>(m, 1) * sizeof(uint32_t))) != hipSuccess ||
@@ -315,16 +330,17 @@ int tc_gpu_impl(uint32_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colI
         cleanup();
         return hip_fail("tc_gpu copy", e);
     }
-    if ((e = hipEventRecord(c->ev[5], st)) != hipSuccess || (e = hipEventSynchronize(c->ev[5])) != hipSuccess) {
+    if ((e = hipEventRecord(c->slot.ev[1], st)) != hipSuccess ||
+        (e = hipEventSynchronize(c->slot.ev[1])) != hipSuccess) {
         cleanup();
         return hip_fail("tc_gpu copy sync", e);
     }
     float copy_ms = 0;
-    (void)hipEventElapsedTime(&copy_ms, c->ev[4], c->ev[5]);
+    (void)hipEventElapsedTime(&copy_ms, c->slot.ev[0], c->slot.ev[1]);
     std::vector<GraphDesc> hd{make_desc(drow.p, dcol.as<uint32_t>(), n, spread, a, 0, UINT64_MAX)};
     uint64_t cnt = 0;
     RunTimes rt;
-    rc = run_set(c, work, (int)sizeof(OffT), hd, &cnt, nullptr, nullptr, &rt);
+    rc = run_set(c, (int)sizeof(OffT), hd, &cnt, nullptr, &rt);
     cleanup();
     if (rc) return rc;
     if (t) {
@@ -363,39 +379,28 @@ int wtc_ctx_create(int device, wtc_ctx **out) {
         delete c;
         return hip_fail("hipGetDeviceProperties", e);
     }
-    c->num_cus = prop.multiProcessorCount;
-    c->ways = default_probe_ways();
-    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
-        delete c;
-        return hip_fail("hipStreamCreate", e);
+    c->tune.num_cus = prop.multiProcessorCount;
+    c->tune.ways = default_probe_ways();
+    if ((rc = c->slot.create())) {
+        wtc_ctx_destroy(c);
+        return rc;
     }
-    for (auto &ev : c->ev)
-        if ((e = hipEventCreate(&ev)) != hipSuccess) {
-            wtc_ctx_destroy(c);
-            return hip_fail("hipEventCreate", e);
-        }
     *out = c;
     return WTC_OK;
 }
 
 int wtc_ctx_set_tuning(wtc_ctx *c, int probe_ways, uint32_t lds_chunk) {
     if (!c || !valid_ways(probe_ways) || lds_chunk < 1 || lds_chunk > kChunkMax) return WTC_ERR_ARG;
-    c->ways = probe_ways;
-    c->chunk = lds_chunk;
+    c->tune.ways = probe_ways;
+    c->tune.chunk = lds_chunk;
     return WTC_OK;
 }
 
 void wtc_ctx_destroy(wtc_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    c->work.release();
-    c->h_desc.release();
-    c->h_out.release();
+    c->slot.destroy();
     c->h_stage.release();
-    for (auto &ev : c->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -417,7 +422,7 @@ int wtc_dgraph_upload(wtc_ctx *c, const wtc_big_graph *g, uint32_t offset_bytes,
     if (offset_bytes == 0) offset_bytes = g->numEdges < 0xffffffffULL ? 4 : 8;
     if (offset_bytes != 4 && offset_bytes != 8) return WTC_ERR_ARG;
     if (offset_bytes == 4 && g->numEdges >= 0xffffffffULL) return WTC_ERR_RANGE;
-    HIPC(hipSetDevice(c->device));
+    WTC_HIPC(hipSetDevice(c->device));
     wtc_dgraph *dg = new (std::nothrow) wtc_dgraph();
     if (!dg) return WTC_ERR_NOMEM;
     dg->ctx = c;
@@ -465,37 +470,32 @@ int wtc_dgraph_count(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t a, ui
     if (!c || !dg || !triangles) return WTC_ERR_ARG;
     int rc = check_args(spread, a);
     if (rc) return rc;
-    HIPC(hipSetDevice(c->device));
+    WTC_HIPC(hipSetDevice(c->device));
     std::vector<GraphDesc> hd{make_desc(dg->rowPtr.p, dg->colInd.as<uint32_t>(), dg->n, spread, a, w_begin, w_end)};
-    RunTimes rt;
-    uint64_t W = 0;
-    uint32_t grid = 0;
-    auto t0 = std::chrono::steady_clock::now();
-    rc = run_set(c, c->work, dg->ob, hd, triangles, &W, &grid, &rt);
-    auto t1 = std::chrono::steady_clock::now();
-    if (rc) return rc;
-    if (st) {
-        st->prefix_ms = rt.prefix_ms;
-        st->bitmap_ms = rt.bitmap_ms;
-        st->count_ms = rt.count_ms;
-        st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        st->wedges = W;
-        st->blocks = grid;
-        st->launches = 1;
+    return run_set(c, dg->ob, hd, triangles, st);
+}
+
+// Copies the wedge prefix and search tree left by the last single-graph count
+// on this context (diagnostics for the parity tests).
+int wtc_ctx_debug_prefix(wtc_ctx *c, uint64_t n, uint64_t *wedge_sum, uint64_t *tree) {
+    if (!c) return WTC_ERR_ARG;
+    WTC_HIPC(hipSetDevice(c->device));
+    if (wedge_sum) {
+        if (c->slot.ws.cap < (n + 1) * sizeof(uint64_t)) return WTC_ERR_ARG;
+        WTC_HIPC(hipMemcpy(wedge_sum, c->slot.ws.p, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    }
+    if (tree) {
+        if (!c->slot.tree.p) return WTC_ERR_ARG;
+        WTC_HIPC(hipMemcpy(tree, c->slot.tree.p, kTreeSize * sizeof(uint64_t), hipMemcpyDeviceToHost));
     }
     return WTC_OK;
 }
 
-// Copies the wedge prefix and search tree left by the last count on this
-// context (diagnostics for the parity tests).
-int wtc_ctx_debug_prefix(wtc_ctx *c, uint64_t n, uint64_t *wedge_sum, uint64_t *tree) {
-    if (!c) return WTC_ERR_ARG;
-    HIPC(hipSetDevice(c->device));
-    if (wedge_sum) {
-        if (c->work.ws.cap < (n + 1) * sizeof(uint64_t)) return WTC_ERR_ARG;
-        HIPC(hipMemcpy(wedge_sum, c->work.ws.p, (n + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost));
-    }
-    if (tree) HIPC(hipMemcpy(tree, c->work.tree.p, kTreeSize * sizeof(uint64_t), hipMemcpyDeviceToHost));
+int wtc_ctx_debug_bitmap(wtc_ctx *c, uint64_t words, uint32_t *out) {
+    if (!c || !out) return WTC_ERR_ARG;
+    if (c->slot.bm.cap < words * sizeof(uint32_t)) return WTC_ERR_ARG;
+    WTC_HIPC(hipSetDevice(c->device));
+    WTC_HIPC(hipMemcpy(out, c->slot.bm.p, words * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return WTC_OK;
 }
 
@@ -505,19 +505,20 @@ int wtc_debug_decode(wtc_ctx *c, const uint32_t *d, const uint64_t *iv, uint64_t
     for (uint64_t i = 0; i < k; i++)
         if (d[i] < 2 || iv[i] >= binom2(d[i])) return WTC_ERR_ARG;
     if (k == 0) return WTC_OK;
-    HIPC(hipSetDevice(c->device));
+    WTC_HIPC(hipSetDevice(c->device));
+    hipStream_t st = c->slot.stream;
     DevBuf b;
     hipError_t e = b.ensure(k * (4 + 8 + 4 + 4));
     if (e != hipSuccess) return hip_fail("debug_decode alloc", e);
     char *p = b.as<char>();
     uint64_t *di = reinterpret_cast<uint64_t *>(p);  // 8-byte entries first, at the aligned base
     uint32_t *dd = reinterpret_cast<uint32_t *>(p + 8 * k), *dw = dd + k, *du = dw + k;
-    if ((e = hipMemcpyAsync(dd, d, k * 4, hipMemcpyHostToDevice, c->stream)) == hipSuccess &&
-        (e = hipMemcpyAsync(di, iv, k * 8, hipMemcpyHostToDevice, c->stream)) == hipSuccess &&
-        (e = launch_debug_decode(dd, di, k, dw, du, c->stream)) == hipSuccess &&
-        (e = hipMemcpyAsync(w_out, dw, k * 4, hipMemcpyDeviceToHost, c->stream)) == hipSuccess &&
-        (e = hipMemcpyAsync(u_out, du, k * 4, hipMemcpyDeviceToHost, c->stream)) == hipSuccess)
-        e = hipStreamSynchronize(c->stream);
+    if ((e = hipMemcpyAsync(dd, d, k * 4, hipMemcpyHostToDevice, st)) == hipSuccess &&
+        (e = hipMemcpyAsync(di, iv, k * 8, hipMemcpyHostToDevice, st)) == hipSuccess &&
+        (e = launch_debug_decode(dd, di, k, dw, du, st)) == hipSuccess &&
+        (e = hipMemcpyAsync(w_out, dw, k * 4, hipMemcpyDeviceToHost, st)) == hipSuccess &&
+        (e = hipMemcpyAsync(u_out, du, k * 4, hipMemcpyDeviceToHost, st)) == hipSuccess)
+        e = hipStreamSynchronize(st);
     b.release();
     return e == hipSuccess ? WTC_OK : hip_fail("debug_decode", e);
 }
@@ -525,29 +526,22 @@ int wtc_debug_decode(wtc_ctx *c, const uint32_t *d, const uint64_t *iv, uint64_t
 int wtc_debug_copy(wtc_ctx *c, uint64_t bytes, int reps, double *ms) {
     if (!c || bytes < 16 || reps < 1) return WTC_ERR_ARG;
     bytes &= ~(uint64_t)15;
-    HIPC(hipSetDevice(c->device));
+    WTC_HIPC(hipSetDevice(c->device));
+    hipStream_t st = c->slot.stream;
     DevBuf src, dst;
     hipError_t e = src.ensure(bytes);
     if (e == hipSuccess) e = dst.ensure(bytes);
-    if (e == hipSuccess) e = launch_fill(src.as<uint32_t>(), bytes / 4, 0x5a5a5a5au, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(c->ev[4], c->stream);
-    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy(src.p, dst.p, bytes, c->stream);
-    if (e == hipSuccess) e = hipEventRecord(c->ev[5], c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = launch_fill(src.as<uint32_t>(), bytes / 4, 0x5a5a5a5au, st);
+    if (e == hipSuccess) e = hipEventRecord(c->slot.ev[0], st);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy(src.p, dst.p, bytes, st);
+    if (e == hipSuccess) e = hipEventRecord(c->slot.ev[1], st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     float t = 0;
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, c->ev[4], c->ev[5]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, c->slot.ev[0], c->slot.ev[1]);
     src.release();
     dst.release();
     if (e != hipSuccess) return hip_fail("debug_copy", e);
     if (ms) *ms = t / reps;
-    return WTC_OK;
-}
-
-int wtc_ctx_debug_bitmap(wtc_ctx *c, uint64_t words, uint32_t *out) {
-    if (!c || !out) return WTC_ERR_ARG;
-    if (c->work.bm.cap < words * sizeof(uint32_t)) return WTC_ERR_ARG;
-    HIPC(hipSetDevice(c->device));
-    HIPC(hipMemcpy(out, c->work.bm.p, words * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return WTC_OK;
 }
 
@@ -628,9 +622,10 @@ int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, w
             if (gr->numEdges) std::memcpy(sc + b->col_off[g], gr->colInd, gr->numEdges * 4);
         }
     });
-    e = hipMemcpyAsync(b->rowPtr.p, stage, rbytes, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess && M) e = hipMemcpyAsync(b->colInd.p, sc, M * 4, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipStream_t st = c->slot.stream;
+    e = hipMemcpyAsync(b->rowPtr.p, stage, rbytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && M) e = hipMemcpyAsync(b->colInd.p, sc, M * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
         wtc_batch_free(b);
         return hip_fail("batch upload", e);
@@ -651,7 +646,7 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
                     wtc_count_stats *st) {
     if (!c || !b || !triangles) return WTC_ERR_ARG;
     if (b->G == 0) return WTC_OK;
-    HIPC(hipSetDevice(c->device));
+    WTC_HIPC(hipSetDevice(c->device));
     std::vector<GraphDesc> hd(b->G);
     for (uint32_t g = 0; g < b->G; g++) {
         uint32_t s = spread ? spread[g] : 0, aa = a ? a[g] : 0;
@@ -660,23 +655,7 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
         const void *rp = b->rowPtr.as<char>() + b->row_off[g] * b->ob;
         hd[g] = make_desc(rp, b->colInd.as<uint32_t>() + b->col_off[g], b->n[g], s, aa, 0, UINT64_MAX);
     }
-    RunTimes rt;
-    uint64_t W = 0;
-    uint32_t grid = 0;
-    auto t0 = std::chrono::steady_clock::now();
-    int rc = run_set(c, c->work, b->ob, hd, triangles, &W, &grid, &rt);
-    auto t1 = std::chrono::steady_clock::now();
-    if (rc) return rc;
-    if (st) {
-        st->prefix_ms = rt.prefix_ms;
-        st->bitmap_ms = rt.bitmap_ms;
-        st->count_ms = rt.count_ms;
-        st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        st->wedges = W;
-        st->blocks = grid;
-        st->launches = 1;
-    }
-    return WTC_OK;
+    return run_set(c, b->ob, hd, triangles, st);
 }
 
 int wtc_batch_count_host(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, const uint32_t *spread,

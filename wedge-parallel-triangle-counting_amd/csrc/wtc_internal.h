@@ -1,0 +1,94 @@
+// wtc_internal.h -- library-internal pieces shared by the C-ABI files
+// (wtc_api.hip, wtc_serve.hip): device/pinned buffers, execution slots and
+// the asynchronous graph-set pipeline.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "host_util.h"
+#include "wtc.h"
+#include "wtc_kernels.h"
+#include "wtc_launch.h"
+
+namespace wtc {
+
+// Records the HIP failure text for wtc_last_error_detail and returns WTC_ERR_HIP.
+int hip_fail(const char *what, hipError_t e);
+void set_error(const std::string &msg);
+
+#define WTC_HIPC(call)                                            \
+    do {                                                          \
+        hipError_t _e = (call);                                   \
+        if (_e != hipSuccess) return ::wtc::hip_fail(#call, _e);  \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+// Grow-only pinned host buffer.
+struct PinBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+    template <class T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+// One asynchronous execution slot: a stream, timing events, the device
+// workspace and the pinned staging of one in-flight graph-set count.
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};  // prefix start, bitmap start, count start, count end
+    DevBuf ws, partials, tree, bm, desc, out;
+    PinBuf h_desc, h_out;
+    uint32_t G = 0;       // graphs of the last enqueue
+    uint32_t grid = 0;    // counting grid of the last enqueue
+    int create();
+    void release_device();  // frees the device workspace (keeps stream/events)
+    void destroy();
+};
+
+// Counting-kernel configuration of a context.
+struct Tuning {
+    int num_cus = 0;
+    int ways = 4;         // probes advanced together per lane
+    uint32_t chunk = 16;  // wedges per lane staged in LDS per phase
+    uint32_t grid_cache[2][4][kChunkMax + 2] = {};
+};
+
+struct RunTimes {
+    float prefix_ms = 0, bitmap_ms = 0, count_ms = 0;
+};
+
+uint64_t bm_words_for(uint32_t a);
+int validate_oriented_serial(const wtc_big_graph *g);
+int check_args(uint32_t spread, uint32_t a);
+
+// Host-filled descriptor of one graph (device pointers to its CSR).  ws and
+// bitmap may be preset by the caller; enqueue_set assigns the missing ones
+// from the slot's workspace.
+GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint32_t spread, uint32_t a,
+                    uint64_t w_begin, uint64_t w_end);
+
+// Launches the whole device pipeline for a graph set on the slot's stream
+// and the copy of its counts into the slot's pinned buffer; returns without
+// waiting.  All graphs of a set share the row-offset width ob.
+int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd);
+// Waits for the slot (or, if it is done, does not wait) and reads results.
+int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt);
+// True when the slot's last enqueue has completed.
+bool slot_done(Slot &s);
+
+}  // namespace wtc

@@ -30,6 +30,7 @@ struct GraphDesc {
     uint64_t bm_size;        // C(a,2): bits of the graph's top-a bitmap
     uint64_t row_base;       // first bitmap row of this graph in the set's row list
     uint64_t blk_base;       // first prefix-scan block of this graph
+    uint64_t bm_clear_base;  // first bitmap word of this graph in the set's clear list
     uint64_t w_begin;        // wedge range to count, relative to the graph
     uint64_t w_end;
     uint32_t n;              // vertices
@@ -43,9 +44,10 @@ struct GraphDesc {
     uint64_t wcount;         // wedges counted
     uint64_t tile_base;      // first tile of this graph
 };
-static_assert(sizeof(GraphDesc) == 120, "GraphDesc layout");
+static_assert(sizeof(GraphDesc) == 128, "GraphDesc layout");
 
 // meta[] slots written by the setup kernel
-enum : int { kMetaTiles = 0, kMetaWedges = 1, kMetaSlots = 4 };
+// (kMetaNext..kMetaNext+7: per-XCD work counters of the counting kernel)
+enum : int { kMetaTiles = 0, kMetaWedges = 1, kMetaNext = 8, kMetaSlots = 16 };
 
 }  // namespace wtc

@@ -168,6 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(GraphDesc *__restrict__ desc, 
         meta[kMetaTiles] = carry;
         meta[kMetaWedges] = wsum;
     }
+    if (threadIdx.x < 8) meta[kMetaNext + threadIdx.x] = 0;  // per-XCD tile counters of k_count
     if (tree != nullptr) {  // single-graph sets only
         const uint32_t n = desc[0].n;
         const uint64_t *ws = desc[0].ws;
@@ -190,6 +191,22 @@ __global__ __launch_bounds__(kBlock) void k_setup(GraphDesc *__restrict__ desc, 
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *__restrict__ p, uint64_t n, uint32_t v) {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Zeroes the bitmaps of every graph of the set: word i of the set's clear
+// list belongs to the rightmost graph whose bm_clear_base <= i.
+__global__ __launch_bounds__(kBlock) void k_bitmap_clear(const GraphDesc *__restrict__ desc, uint32_t G,
+                                                         uint64_t total) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < total; i += stride) {
+        uint32_t lo = 0, hi = G - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) >> 1;
+            if (desc[mid].bm_clear_base <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        desc[lo].bitmap[i - desc[lo].bm_clear_base] = 0u;
+    }
 }
 
 // One wave per bitmap row (rows v in [n - min(n,a), n) of every graph).
@@ -296,10 +313,17 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
     u_out[i] = u;
 }
 
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)x);
+}
+
+// Tiles a wave takes per grab from its XCD's work counter.
+constexpr uint64_t kTilesPerGrab = 8;
+
 template <typename OffT, int kWays>
 __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ desc, uint32_t G,
-                                                  const uint64_t *__restrict__ tree,
-                                                  const uint64_t *__restrict__ meta,
+                                                  const uint64_t *__restrict__ tree, uint64_t *__restrict__ meta,
                                                   unsigned long long *__restrict__ counts, uint32_t pad_max,
                                                   uint32_t chunk) {
     extern __shared__ uint32_t lds[];
@@ -313,152 +337,205 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
 
     // XCD partition: blocks go round-robin to the 8 XCDs, so blocks with the
     // same blockIdx % 8 share an L2; each such group sweeps one contiguous
-    // eighth of the tiles.
+    // eighth of the tiles, handed out kTilesPerGrab at a time by a counter
+    // (dynamic balance; consecutive tiles let a wave gallop from the vertex
+    // where its previous tile ended instead of searching from the root).
     const uint64_t ntiles = meta[kMetaTiles];
-    const uint32_t xcd = blockIdx.x & 7u, jx = blockIdx.x >> 3, bpx = gridDim.x >> 3;
+    const uint32_t xcd = blockIdx.x & 7u;
     const uint64_t t_lo = ntiles * xcd / 8, t_hi = ntiles * (xcd + 1) / 8;
-    const uint64_t wpx = (uint64_t)bpx * kWavesPerBlock;
+    unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaNext + xcd);
 
     uint32_t cur_g = kSentinel;
     uint64_t cur_cnt = 0;
 
-    for (uint64_t t = t_lo + (uint64_t)jx * kWavesPerBlock + wid; t < t_hi; t += wpx) {
-        const uint32_t g = G > 1 ? graph_of(desc, G, t, &GraphDesc::tile_base) : 0u;
-        const GraphDesc &D = desc[g];
-        const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
-        const uint32_t *__restrict__ colInd = D.colInd;
-        const uint64_t *__restrict__ ws = D.ws;
-        const uint32_t *__restrict__ bitmap = D.bitmap;
-        const uint32_t s = D.spread;
-        const uint32_t sc = s < chunk ? s : chunk;  // wedges per lane staged at once
-        const uint32_t pad = sc | 1u;
-        const uint64_t wend = D.wbase + D.wcount;
-        const uint64_t tile_start = D.wbase + (t - D.tile_base) * ((uint64_t)kWave * s);
-        const uint64_t tile_end = tile_start + (uint64_t)kWave * s < wend ? tile_start + (uint64_t)kWave * s : wend;
-        const uint32_t gn = D.n, vmax = gn - 1, thr = D.thr;
-        const uint64_t bm_size = D.bm_size;
+    for (;;) {
+        unsigned long long grab = 0;
+        if (lane == 0) grab = atomicAdd(ctr, 1ull);
+        const uint64_t tb = t_lo + uni64(grab) * kTilesPerGrab;
+        if (tb >= t_hi) break;
+        const uint64_t te = tb + kTilesPerGrab < t_hi ? tb + kTilesPerGrab : t_hi;
 
-        // wave-uniform vertex range [v_lo, v_hi] of the tile
-        const uint32_t v_lo = (tree != nullptr) ? tree_search(tree, ws, gn, tile_start)
-                                                : rightmost_le(ws, 0, vmax, tile_start);
-        const uint32_t v_hi = gallop_le(ws, v_lo, vmax, tile_end - 1);
+        uint32_t g = kSentinel, v_prev = 0;
+        uint64_t g_tile_end = 0;
+        bool have_prev = false;
+        // per-graph state, wave-uniform
+        const OffT *__restrict__ rowPtr = nullptr;
+        const uint32_t *__restrict__ colInd = nullptr;
+        const uint64_t *__restrict__ ws = nullptr;
+        const uint32_t *__restrict__ bitmap = nullptr;
+        uint32_t s = 1, sc = 1, pad = 1, gn = 1, thr = 0;
+        uint64_t wbase = 0, wend = 0, tile_base = 0, bm_size = 0;
 
-        // lane state: wedges [first, lend)
-        const uint64_t first = tile_start + (uint64_t)lane * s;
-        const uint64_t lend = first + s < tile_end ? first + s : tile_end;
-        uint64_t i = first;
-        bool started = false;
-        uint32_t v = 0, d = 0, w_i = 0, u_i = 0, w = 0;
-        OffT vb = 0;
-        uint64_t tile_cnt = 0;
+        for (uint64_t t = tb; t < te; t++) {
+            if (g == kSentinel || t >= g_tile_end) {
+                g = G > 1 ? graph_of(desc, G, t, &GraphDesc::tile_base) : 0u;
+                const GraphDesc &D = desc[g];
+                rowPtr = static_cast<const OffT *>(D.rowPtr);
+                colInd = D.colInd;
+                ws = D.ws;
+                bitmap = D.bitmap;
+                s = D.spread;
+                sc = s < chunk ? s : chunk;  // wedges per lane staged at once
+                pad = sc | 1u;
+                gn = D.n;
+                thr = D.thr;
+                bm_size = D.bm_size;
+                wbase = D.wbase;
+                wend = D.wbase + D.wcount;
+                tile_base = D.tile_base;
+                const uint64_t per = (uint64_t)kWave * s;
+                g_tile_end = tile_base + (D.wcount + per - 1) / per;
+                have_prev = false;
+            }
+            const uint32_t vmax = gn - 1;
+            const uint64_t tile_start = wbase + (t - tile_base) * ((uint64_t)kWave * s);
+            const uint64_t tile_end =
+                tile_start + (uint64_t)kWave * s < wend ? tile_start + (uint64_t)kWave * s : wend;
 
-        for (uint64_t c = 0; c < s; c += sc) {  // 64-bit: s may be close to 2^32
-            const uint32_t kn = (s - c) < sc ? (uint32_t)(s - c) : sc;
-            // phase 1: generate this lane's next kn wedges into LDS
-            for (uint32_t k = 0; k < kn; k++, i++) {
-                uint32_t w_out = kSentinel, u_out = 0;
-                if (i < lend) {
-                    if (!started) {
-                        started = true;
-                        v = rightmost_le(ws, v_lo, v_hi, i);
-                        vb = rowPtr[v];
-                        d = (uint32_t)(rowPtr[v + 1] - vb);
-                        decode_wedge(d, i - ws[v], w_i, u_i);
-                        w = colInd[vb + w_i];
-                    } else {
-                        u_i++;
-                        if (u_i >= d) {
-                            w_i++;
-                            u_i = w_i + 1;
-                            if (u_i >= d) {  // vertex exhausted: next vertex with wedges
-                                v = (ws[v + 2] > i) ? v + 1 : rightmost_le(ws, v + 2, v_hi, i);
+            // wave-uniform vertex range [v_lo, v_hi] of the tile
+            const uint32_t v_lo = have_prev ? gallop_le(ws, v_prev, vmax, tile_start)
+                                  : (tree != nullptr) ? tree_search(tree, ws, gn, tile_start)
+                                                      : rightmost_le(ws, 0, vmax, tile_start);
+            const uint32_t v_hi = gallop_le(ws, v_lo, vmax, tile_end - 1);
+            v_prev = v_hi;
+            have_prev = true;
+
+            // lane state: wedges [first, lend)
+            const uint64_t first = tile_start + (uint64_t)lane * s;
+            const uint64_t lend = first + s < tile_end ? first + s : tile_end;
+            uint64_t i = first;
+            bool started = false;
+            uint32_t v = 0, d = 0, w_i = 0, u_i = 0;
+            OffT vb = 0;
+            uint64_t tile_cnt = 0;
+
+            for (uint64_t c = 0; c < s; c += sc) {  // 64-bit: s may be close to 2^32
+                const uint32_t kn = (s - c) < sc ? (uint32_t)(s - c) : sc;
+                // phase 1: this lane's next kn wedges into LDS, positions first
+                // and then the colInd loads of a group together
+                for (uint32_t k0 = 0; k0 < kn; k0 += 4) {
+                    OffT wpos[4], upos[4];
+                    bool ok[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        ok[j] = (k0 + j < kn) && (i < lend);
+                        wpos[j] = 0;
+                        upos[j] = 0;
+                        if (ok[j]) {
+                            if (!started) {
+                                started = true;
+                                v = rightmost_le(ws, v_lo, v_hi, i);
                                 vb = rowPtr[v];
                                 d = (uint32_t)(rowPtr[v + 1] - vb);
-                                w_i = 0;
-                                u_i = 1;
+                                decode_wedge(d, i - ws[v], w_i, u_i);
+                            } else {
+                                u_i++;
+                                if (u_i >= d) {
+                                    w_i++;
+                                    u_i = w_i + 1;
+                                    if (u_i >= d) {  // vertex exhausted: next vertex with wedges
+                                        v = (ws[v + 2] > i) ? v + 1 : rightmost_le(ws, v + 2, v_hi, i);
+                                        vb = rowPtr[v];
+                                        d = (uint32_t)(rowPtr[v + 1] - vb);
+                                        w_i = 0;
+                                        u_i = 1;
+                                    }
+                                }
                             }
-                            w = colInd[vb + w_i];
+                            wpos[j] = vb + w_i;
+                            upos[j] = vb + u_i;
+                            i++;
                         }
                     }
-                    w_out = w;
-                    u_out = colInd[vb + u_i];
-                }
-                lw[lane * pad + k] = w_out;
-                lu[lane * pad + k] = u_out;
-            }
-            __syncwarp();
-            // phase 2: consecutive lanes probe consecutive wedges; each lane
-            // advances kWays independent searches in lockstep so that their
-            // dependent loads overlap (memory-level parallelism per lane).
-            const uint32_t magic = (65536u + kn - 1) / kn;
-            for (uint32_t j0 = 0; j0 < kn; j0 += kWays) {
-                uint32_t pw[kWays], pu[kWays];
-                OffT b[kWays], len[kWays];
-                bool hit[kWays];
+                    uint32_t wv[4], uv[4];
 #pragma unroll
-                for (int k = 0; k < kWays; k++) {
-                    const uint32_t j = j0 + k;
-                    pw[k] = kSentinel;
-                    pu[k] = 0;
-                    if (j < kn) {
-                        const uint32_t p = j * kWave + lane;
-                        const uint32_t q = (p * magic) >> 16;
-                        const uint32_t r = p - q * kn;
-                        pw[k] = lw[q * pad + r];
-                        pu[k] = lu[q * pad + r];
+                    for (int j = 0; j < 4; j++) {
+                        wv[j] = ok[j] ? colInd[wpos[j]] : kSentinel;
+                        uv[j] = ok[j] ? colInd[upos[j]] : 0u;
                     }
-                }
 #pragma unroll
-                for (int k = 0; k < kWays; k++) {
-                    hit[k] = false;
-                    b[k] = 0;
-                    len[k] = 0;
-                    if (pw[k] != kSentinel) {
-                        if (pw[k] >= thr) {  // top-a bitmap (tc.cu:262-272)
-                            const uint64_t idx = bm_size - binom2_d((uint64_t)gn - pw[k]) + (pu[k] - pw[k] - 1);
-                            hit[k] = (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
-                        } else {  // sorted-list probe of adj+(w) (tc.cu:273-286)
-                            b[k] = rowPtr[pw[k]];
-                            len[k] = rowPtr[pw[k] + 1] - b[k];
+                    for (int j = 0; j < 4; j++) {
+                        if (k0 + j < kn) {
+                            lw[lane * pad + k0 + j] = wv[j];
+                            lu[lane * pad + k0 + j] = uv[j];
                         }
                     }
                 }
-                for (;;) {
-                    bool more = false;
-#pragma unroll
-                    for (int k = 0; k < kWays; k++) more |= len[k] > 0;
-                    if (!more) break;
-                    uint32_t x[kWays];
-#pragma unroll
-                    for (int k = 0; k < kWays; k++) x[k] = len[k] > 0 ? colInd[b[k] + (len[k] >> 1)] : 0u;
+                __syncwarp();
+                // phase 2: consecutive lanes probe consecutive wedges; each lane
+                // advances kWays independent searches in lockstep so that their
+                // dependent loads overlap (memory-level parallelism per lane).
+                const uint32_t magic = (65536u + kn - 1) / kn;
+                for (uint32_t j0 = 0; j0 < kn; j0 += kWays) {
+                    uint32_t pw[kWays], pu[kWays];
+                    OffT b[kWays], len[kWays];
+                    bool hit[kWays];
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
-                        if (len[k] > 0) {
-                            const OffT half = len[k] >> 1;
-                            if (x[k] == pu[k]) {
-                                hit[k] = true;
-                                len[k] = 0;
-                            } else if (x[k] < pu[k]) {
-                                b[k] += half + 1;
-                                len[k] -= half + 1;
-                            } else {
-                                len[k] = half;
+                        const uint32_t j = j0 + k;
+                        pw[k] = kSentinel;
+                        pu[k] = 0;
+                        if (j < kn) {
+                            const uint32_t p = j * kWave + lane;
+                            const uint32_t q = (p * magic) >> 16;
+                            const uint32_t r = p - q * kn;
+                            pw[k] = lw[q * pad + r];
+                            pu[k] = lu[q * pad + r];
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) {
+                        hit[k] = false;
+                        b[k] = 0;
+                        len[k] = 0;
+                        if (pw[k] != kSentinel) {
+                            if (pw[k] >= thr) {  // top-a bitmap (tc.cu:262-272)
+                                const uint64_t idx =
+                                    bm_size - binom2_d((uint64_t)gn - pw[k]) + (pu[k] - pw[k] - 1);
+                                hit[k] = (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
+                            } else {  // sorted-list probe of adj+(w) (tc.cu:273-286)
+                                b[k] = rowPtr[pw[k]];
+                                len[k] = rowPtr[pw[k] + 1] - b[k];
                             }
                         }
                     }
-                }
+                    for (;;) {
+                        bool more = false;
 #pragma unroll
-                for (int k = 0; k < kWays; k++) tile_cnt += (uint64_t)__popcll(__ballot(hit[k]));
+                        for (int k = 0; k < kWays; k++) more |= len[k] > 0;
+                        if (!more) break;
+                        uint32_t x[kWays];
+#pragma unroll
+                        for (int k = 0; k < kWays; k++) x[k] = len[k] > 0 ? colInd[b[k] + (len[k] >> 1)] : 0u;
+#pragma unroll
+                        for (int k = 0; k < kWays; k++) {
+                            if (len[k] > 0) {
+                                const OffT half = len[k] >> 1;
+                                if (x[k] == pu[k]) {
+                                    hit[k] = true;
+                                    len[k] = 0;
+                                } else if (x[k] < pu[k]) {
+                                    b[k] += half + 1;
+                                    len[k] -= half + 1;
+                                } else {
+                                    len[k] = half;
+                                }
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) tile_cnt += (uint64_t)__popcll(__ballot(hit[k]));
+                }
+                __syncwarp();
             }
-            __syncwarp();
-        }
 
-        if (g != cur_g) {
-            if (cur_cnt != 0 && lane == 0) atomicAdd(&counts[cur_g], (unsigned long long)cur_cnt);
-            cur_g = g;
-            cur_cnt = 0;
+            if (g != cur_g) {
+                if (cur_cnt != 0 && lane == 0) atomicAdd(&counts[cur_g], (unsigned long long)cur_cnt);
+                cur_g = g;
+                cur_cnt = 0;
+            }
+            cur_cnt += tile_cnt;
         }
-        cur_cnt += tile_cnt;
     }
 
     // one 64-bit atomic per block and graph
@@ -541,11 +618,15 @@ hipError_t launch_setup(GraphDesc *desc, uint32_t G, uint64_t *tree, uint64_t *m
     return hipGetLastError();
 }
 
-hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t rows, uint32_t *bm, uint64_t words,
+hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t rows, uint64_t clear_words,
                          hipStream_t st) {
-    if (words == 0 || rows == 0) return hipSuccess;
-    hipError_t e = launch_fill(bm, words, 0u, st);
-    if (e != hipSuccess) return e;
+    if (rows == 0) return hipSuccess;
+    if (clear_words) {
+        const uint64_t cb = (clear_words + kBlock - 1) / kBlock;
+        k_bitmap_clear<<<dim3((uint32_t)(cb < 4096 ? cb : 4096)), dim3(kBlock), 0, st>>>(desc, G, clear_words);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     const uint64_t bb = (rows + kWavesPerBlock - 1) / kWavesPerBlock;
     return WTC_LAUNCH_OFF(ob, k_bitmap_build, dim3((uint32_t)(bb < 8192 ? bb : 8192)), dim3(kBlock), 0, st, desc, G,
                           rows);
@@ -589,7 +670,7 @@ hipError_t count_grid(int ob, uint32_t max_spread, int ways, int num_cus, uint32
 
 template <int W>
 static hipError_t launch_count_w(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree,
-                                 const uint64_t *meta, unsigned long long *counts, uint32_t pad, uint32_t chunk,
+                                 uint64_t *meta, unsigned long long *counts, uint32_t pad, uint32_t chunk,
                                  size_t shm, uint32_t grid, hipStream_t st) {
     if (ob == 8)
         k_count<uint64_t, W><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
@@ -598,7 +679,7 @@ static hipError_t launch_count_w(const GraphDesc *desc, uint32_t G, int ob, cons
     return hipGetLastError();
 }
 
-hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, const uint64_t *meta,
+hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
                         unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, uint32_t grid,
                         hipStream_t st) {
     if (chunk < 1 || chunk > kChunkMax) return hipErrorInvalidValue;

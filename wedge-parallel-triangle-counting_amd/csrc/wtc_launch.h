@@ -18,8 +18,9 @@ hipError_t launch_prefix(const GraphDesc *desc, uint32_t G, int ob, uint32_t nbl
                          hipStream_t st);
 hipError_t launch_setup(GraphDesc *desc, uint32_t G, uint64_t *tree, uint64_t *meta, unsigned long long *counts,
                         hipStream_t st);
-// Clears the set's bitmap region (words) and sets the bits of every graph.
-hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t rows, uint32_t *bm, uint64_t words,
+// Clears every graph's bitmap (clear_words = sum of their word counts, laid
+// out by bm_clear_base) and sets its bits; rows = sum of bm_rows.
+hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t rows, uint64_t clear_words,
                          hipStream_t st);
 // LDS bytes of a counting block that stages `staged` wedges per lane.
 size_t count_lds_bytes(uint32_t staged);
@@ -27,7 +28,7 @@ size_t count_lds_bytes(uint32_t staged);
 // `staged` = min(max spread, chunk).
 hipError_t count_grid(int ob, uint32_t staged, int ways, int num_cus, uint32_t *grid);
 // `chunk` (1..kChunkMax): wedges per lane staged in LDS per phase.
-hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, const uint64_t *meta,
+hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
                         unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, uint32_t grid,
                         hipStream_t st);
 

@@ -69,6 +69,25 @@ class CountStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("slots", C.c_uint32),
+                ("queue_capacity", C.c_uint32), ("probe_ways", C.c_int32), ("lds_chunk", C.c_uint32)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("service_ms", C.c_double), ("latency_ms", C.c_double),
+                ("batch_graphs", C.c_uint32), ("batch_id", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("graphs", "submitted", "completed", "rejected", "batches",
+                                          "max_batch_graphs", "uploads", "evictions", "resident_bytes",
+                                          "resident_graphs", "queued")]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 _P = C.c_void_p
 _BGP = C.POINTER(_BigGraph)
 _sig = {
@@ -113,6 +132,12 @@ _sig = {
     "wtc_batch_count": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(CountStats)]),
     "wtc_batch_count_host": (C.c_int, [_P, C.POINTER(_BGP), C.c_size_t, _P, _P, _P, C.POINTER(CountStats)]),
     "wtc_autotune": (None, [C.c_uint32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "wtc_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "wtc_server_register": (C.c_int, [_P, _BGP, C.POINTER(C.c_uint64)]),
+    "wtc_server_submit": (C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "wtc_server_wait": (C.c_int, [_P, C.c_uint64, C.c_int64, C.POINTER(C.c_uint64), C.POINTER(RequestStats)]),
+    "wtc_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
+    "wtc_server_destroy": (None, [_P]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(_lib, _name)
@@ -432,3 +457,50 @@ def batch_count_host(ctx: Context, graphs: Sequence[CSR], spread=None, a=None, s
     st = CountStats()
     _check(_lib.wtc_batch_count_host(ctx._p, arr, k, sp, ap, out.ctypes.data, C.byref(st)), "batch_count_host")
     return (out, st) if stats else out
+
+
+# ------------------------------------------------------------------ serving
+
+class Server:
+    """Request loop with continuous batching over device-resident CSRs."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, max_batch: int = 256, slots: int = 2,
+                 queue_capacity: int = 65536, probe_ways: int = 0, lds_chunk: int = 0):
+        cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, queue_capacity, probe_ways, lds_chunk)
+        self._p = C.c_void_p()
+        _check(_lib.wtc_server_create(device, C.byref(cfg), C.byref(self._p)), "server_create")
+
+    def register(self, g: CSR) -> int:
+        s = g._struct()
+        gid = C.c_uint64()
+        _check(_lib.wtc_server_register(self._p, C.byref(s), C.byref(gid)), "server_register")
+        return int(gid.value)
+
+    def submit(self, graph_id: int, spread: int, a: int) -> int:
+        """Returns a request id; raises WtcError (code -11 when the queue is full)."""
+        rid = C.c_uint64()
+        _check(_lib.wtc_server_submit(self._p, graph_id, spread, a, C.byref(rid)), "server_submit")
+        return int(rid.value)
+
+    def wait(self, request_id: int, timeout_us: int = -1):
+        """(triangles, RequestStats) once the request has finished."""
+        cnt = C.c_uint64()
+        st = RequestStats()
+        _check(_lib.wtc_server_wait(self._p, request_id, timeout_us, C.byref(cnt), C.byref(st)), "server_wait")
+        return int(cnt.value), st
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(_lib.wtc_server_get_stats(self._p, C.byref(st)), "server_stats")
+        return st.as_dict()
+
+    def close(self):
+        if self._p:
+            _lib.wtc_server_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
