@@ -67,8 +67,10 @@ typedef struct {
     double exec;
 } wtc_gpu_time;
 
-/* preprocess_t, tc.cu:56: accepted for interface parity; every style yields
- * the same CSR (the CPU implementation is used for all three). */
+/* preprocess_t, tc.cu:56.  CPU runs the host builders; GPU and
+ * GPU_CONSTRAINED run the sorts on GPU 0 (rocPRIM radix sorts; there is one
+ * GPU implementation for both) and return WTC_ERR_NODEV without a GPU.  All
+ * three yield the same CSR. */
 #define WTC_PREPROCESS_CPU 0
 #define WTC_PREPROCESS_GPU 1
 #define WTC_PREPROCESS_GPU_CONSTRAINED 2
@@ -176,6 +178,18 @@ int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk);
 int wtc_dgraph_upload(wtc_ctx *ctx, const wtc_big_graph *g, uint32_t offset_bytes, wtc_dgraph **out);
 void wtc_dgraph_free(wtc_dgraph *dg);
 uint64_t wtc_dgraph_bytes(const wtc_dgraph *dg);
+
+/* Builds the oriented CSR on the device from (src,dst) host pairs, as
+ * read_graph -z then preprocess would (GPU preprocessing, result identical to
+ * the CPU builders); offset_bytes as in wtc_dgraph_upload. */
+int wtc_dgraph_from_pairs(wtc_ctx *ctx, const uint32_t *pairs, uint64_t npairs, uint32_t offset_bytes,
+                          wtc_dgraph **out);
+/* The R-MAT graph of wtc_gen_rmat (same pairs, drawn on the device), oriented
+ * on the device. */
+int wtc_dgraph_gen_rmat(wtc_ctx *ctx, uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                        uint64_t seed, uint32_t offset_bytes, wtc_dgraph **out);
+/* Copies a resident oriented CSR back to the host (free with wtc_free_graph). */
+int wtc_dgraph_download(const wtc_dgraph *dg, wtc_big_graph **out);
 
 /* Counts the wedges [w_begin, w_end) (clamped to [0, W); pass 0, UINT64_MAX
  * for all) of a resident graph: the device part of tc_GPU, from the resident

@@ -61,13 +61,16 @@ def test_reader_cases_match_oracle(tmp_path, case):
     assert same_csr(wtc.read_graph(path, mm, zero), og)
 
 
-def test_preprocess_styles_identical():
+def test_preprocess_style_checks():
     g = wtc.gen_rmat(11, 8, seed=3)
     ref = wtc.preprocess(g, 0)
-    for style in (1, 2):
-        assert same_csr(wtc.preprocess(g, style), ref)
+    assert same_csr(wtc.preprocess(g), ref)  # the binding's default is the host builder
     with pytest.raises(wtc.WtcError):
         wtc.preprocess(g, 3)
+    if wtc.device_count() == 0:  # -p 1/2 need the GPU: no silent host fallback
+        with pytest.raises(wtc.WtcError) as e:
+            wtc.preprocess(g, 2)
+        assert e.value.code == -8
 
 
 def test_random_graphs_match_oracle():

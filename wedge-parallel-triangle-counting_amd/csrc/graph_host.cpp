@@ -301,6 +301,9 @@ int wtc_read_graph(const char *filename, int matrix_market, int zero_indexed, in
     std::free(buf);
     if (rc) return rc;
     if ((uint64_t)max_vertex + 1 > 0xffffffffULL) return WTC_ERR_FORMAT;
+    // -p 1/2: the sort and deduplication run on the GPU (tc.cu:628-644)
+    if (preprocess_style != WTC_PREPROCESS_CPU)
+        return wtc_gpu_symmetric_default(pairs.data(), pairs.size() / 2, max_vertex + 1, out);
     const uint32_t *pp = pairs.data();
     return build_symmetric((uint64_t)max_vertex + 1, pairs.size() / 2,
                            [pp](uint64_t i, uint32_t &s, uint32_t &d) {
@@ -349,6 +352,8 @@ int wtc_preprocess(const wtc_big_graph *in, int preprocess_style, wtc_big_graph 
     if (preprocess_style < WTC_PREPROCESS_CPU || preprocess_style > WTC_PREPROCESS_GPU_CONSTRAINED)
         return WTC_ERR_ARG;
     *out = nullptr;
+    // -p 1/2: degree sort, orientation and row sort on the GPU (tc.cu:697-716, 763-784)
+    if (preprocess_style != WTC_PREPROCESS_CPU) return wtc_gpu_orient_default(in, out);
     const uint64_t n = in->numVertices;
     const uint64_t *rp = in->rowPtr;
     // Stable ascending order by degree: counting sort, scattered in id order.

@@ -11,6 +11,8 @@
 #include <thread>
 #include <vector>
 
+#include "wtc.h"
+
 namespace wtc {
 
 // Worker threads for host preprocessing: WTC_HOST_THREADS, else
@@ -69,6 +71,16 @@ T *alloc_n(uint64_t n) {
 // it can run inside par_chunks.  Returns WTC_OK or WTC_ERR_CSR.
 int validate_oriented_rows(uint64_t n, uint64_t m, const uint64_t *rowPtr, const uint32_t *colInd, uint64_t v0,
                            uint64_t v1);
+
+}  // namespace wtc
+
+// GPU builders of the -p 1/2 styles (wtc_prep.hip), on the process-wide
+// context: symmetric CSR of loop-free pairs with ids < n, and the degree
+// orientation of a symmetric CSR.  Same results as the CPU builders.
+int wtc_gpu_symmetric_default(const uint32_t *pairs, uint64_t np, uint32_t n, wtc_big_graph **out);
+int wtc_gpu_orient_default(const wtc_big_graph *g, wtc_big_graph **out);
+
+namespace wtc {
 
 static inline uint64_t binom2(uint64_t x) { return (x & 1u) ? x * ((x - 1u) >> 1) : (x >> 1) * (x - 1u); }
 

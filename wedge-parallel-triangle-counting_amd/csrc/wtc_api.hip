@@ -14,21 +14,6 @@
 
 using namespace wtc;
 
-struct wtc_ctx {
-    int device = 0;
-    Slot slot;
-    Tuning tune;
-    PinBuf h_stage;
-};
-
-struct wtc_dgraph {
-    wtc_ctx *ctx;
-    uint32_t n;
-    uint64_t m;
-    int ob;
-    DevBuf rowPtr, colInd;
-};
-
 // Many oriented CSRs in one device allocation: every graph keeps its own
 // rowPtr (offsets relative to its own colInd) and colInd (local ids).
 struct wtc_batch {
@@ -285,12 +270,20 @@ int run_set(wtc_ctx *c, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, wt
     return WTC_OK;
 }
 
-// Process-wide context for the reference-style entry points (tc_GPU has no
+}  // namespace
+
+namespace wtc {
+
+// Process-wide context for the context-free entry points (tc_GPU has no
 // context argument); serialised by a mutex like the reference's globals.
+namespace {
 std::mutex g_default_mu;
 wtc_ctx *g_default = nullptr;
+}  // namespace
 
-int default_ctx(wtc_ctx **out) {
+std::mutex &default_ctx_mutex() { return g_default_mu; }
+
+int default_context(wtc_ctx **out) {
     if (!g_default) {
         int rc = wtc_ctx_create(0, &g_default);
         if (rc) return rc;
@@ -298,6 +291,10 @@ int default_ctx(wtc_ctx **out) {
     *out = g_default;
     return WTC_OK;
 }
+
+}  // namespace wtc
+
+namespace {
 
 // tc_GPU body for either offset width: per-call allocation, copy, device
 // prefix/tree/bitmap, count, copy back, free (tc.cu:315-441).
@@ -308,9 +305,9 @@ int tc_gpu_impl(uint32_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colI
     if (rc) return rc;
     if (!rowPtr || (m && !colInd) || !triangles || n == 0) return WTC_ERR_ARG;
     if ((uint64_t)rowPtr[n] != m || rowPtr[0] != 0) return WTC_ERR_CSR;
-    std::lock_guard<std::mutex> lk(g_default_mu);
+    std::lock_guard<std::mutex> lk(default_ctx_mutex());
     wtc_ctx *c;
-    if ((rc = default_ctx(&c))) return rc;
+    if ((rc = default_context(&c))) return rc;
     WTC_HIPC(hipSetDevice(c->device));
     hipStream_t st = c->slot.stream;
     DevBuf drow, dcol;

@@ -1,0 +1,53 @@
+// wtc_dev.h -- device helpers shared by the HIP translation units: binomial
+// counts, wave-uniform broadcasts and wave64/block scans.
+#pragma once
+#include "wtc_kernels.h"
+
+namespace wtc {
+
+__device__ __forceinline__ uint64_t binom2_d(uint64_t x) {
+    return (x & 1u) ? x * ((x - 1u) >> 1) : (x >> 1) * (x - 1u);
+}
+
+__device__ __forceinline__ uint32_t uni32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ uint64_t uni64(uint64_t x) {
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
+           __builtin_amdgcn_readfirstlane((uint32_t)x);
+}
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint64_t y = __shfl_up(x, (unsigned)o, kWave);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+
+// Exclusive scan over the 256 lanes of a block; s_tot holds 4 wave totals.
+__device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, uint64_t *s_tot, uint64_t &total) {
+    const uint64_t inc = wave_incl_scan(x);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & (kWave - 1)) == kWave - 1) s_tot[w] = inc;
+    __syncthreads();
+    uint64_t off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; i++) {
+        const uint64_t t = s_tot[i];
+        off += (i < w) ? t : 0;
+        tot += t;
+    }
+    __syncthreads();
+    total = tot;
+    return off + inc - x;
+}
+
+}  // namespace wtc

@@ -2,6 +2,7 @@
 // (wtc_api.hip, wtc_serve.hip): device/pinned buffers, execution slots and
 // the asynchronous graph-set pipeline.
 #pragma once
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -91,4 +92,31 @@ int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt);
 // True when the slot's last enqueue has completed.
 bool slot_done(Slot &s);
 
+// The process-wide context of the context-free entry points (tc_GPU,
+// read_graph/preprocess with -p 1/2); callers hold default_ctx_mutex().
+int default_context(wtc_ctx **out);
+std::mutex &default_ctx_mutex();
+
+// GPU builders behind read_graph/preprocess with -p 1/2 (wtc_prep.hip):
+// the symmetric CSR of directed pairs without self-loops and ids < n, and
+// the degree-ordered orientation of a symmetric CSR.  Both return host CSRs
+// identical to the CPU builders'.
+int gpu_symmetric_from_pairs(wtc_ctx *c, const uint32_t *pairs, uint64_t npairs, uint32_t n, wtc_big_graph **out);
+int gpu_orient(wtc_ctx *c, const wtc_big_graph *sym, wtc_big_graph **out);
+
 }  // namespace wtc
+
+struct wtc_ctx {
+    int device = 0;
+    wtc::Slot slot;
+    wtc::Tuning tune;
+    wtc::PinBuf h_stage;
+};
+
+struct wtc_dgraph {
+    wtc_ctx *ctx;
+    uint32_t n;
+    uint64_t m;
+    int ob;
+    wtc::DevBuf rowPtr, colInd;
+};

@@ -14,50 +14,11 @@
 //                 XCD-partitioned grid; every wave owns tiles of 64*s wedges
 //                 and keeps its own LDS staging, so waves never wait on each
 //                 other inside a tile.
+#include "wtc_dev.h"
 #include "wtc_kernels.h"
 #include "wtc_launch.h"
 
 namespace wtc {
-
-__device__ __forceinline__ uint64_t binom2_d(uint64_t x) {
-    return (x & 1u) ? x * ((x - 1u) >> 1) : (x >> 1) * (x - 1u);
-}
-
-__device__ __forceinline__ uint32_t uni32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-
-__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x) {
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        uint64_t y = __shfl_up(x, (unsigned)o, kWave);
-        if (lane >= o) x += y;
-    }
-    return x;
-}
-
-__device__ __forceinline__ uint64_t wave_sum(uint64_t x) {
-#pragma unroll
-    for (int o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
-    return x;
-}
-
-// Exclusive scan over the 256 lanes of a block; s_tot holds 4 wave totals.
-__device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, uint64_t *s_tot, uint64_t &total) {
-    const uint64_t inc = wave_incl_scan(x);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & (kWave - 1)) == kWave - 1) s_tot[w] = inc;
-    __syncthreads();
-    uint64_t off = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < kWavesPerBlock; i++) {
-        const uint64_t t = s_tot[i];
-        off += (i < w) ? t : 0;
-        tot += t;
-    }
-    __syncthreads();
-    total = tot;
-    return off + inc - x;
-}
 
 // Rightmost graph g with desc[g].*field <= key (wave-uniform, scalar loads).
 __device__ __forceinline__ uint32_t graph_of(const GraphDesc *__restrict__ desc, uint32_t G, uint64_t key,
@@ -311,11 +272,6 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
     decode_wedge(d[i], iv[i], w, u);
     w_out[i] = w;
     u_out[i] = u;
-}
-
-__device__ __forceinline__ uint64_t uni64(uint64_t x) {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
-           __builtin_amdgcn_readfirstlane((uint32_t)x);
 }
 
 // Tiles a wave takes per grab from its XCD's work counter.
@@ -596,6 +552,12 @@ hipError_t launch_debug_decode(const uint32_t *d, const uint64_t *iv, uint64_t k
                                hipStream_t st) {
     if (k == 0) return hipSuccess;
     k_debug_decode<<<dim3((uint32_t)((k + kBlock - 1) / kBlock)), dim3(kBlock), 0, st>>>(d, iv, k, w, u);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_partials(uint64_t *partials, uint32_t nb, hipStream_t st) {
+    if (nb == 0) return hipSuccess;
+    k_scan_partials<<<dim3(1), dim3(kBlock), 0, st>>>(partials, nb);
     return hipGetLastError();
 }
 

@@ -10,6 +10,8 @@ hipError_t launch_debug_decode(const uint32_t *d, const uint64_t *iv, uint64_t k
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st);
 hipError_t launch_fill(uint32_t *p, uint64_t words, uint32_t v, hipStream_t st);
 
+// Exclusive scan of partials[0..nb) in place (one block).
+hipError_t launch_scan_partials(uint64_t *partials, uint32_t nb, hipStream_t st);
 // Prefix-scan blocks one graph of n vertices needs.
 uint32_t prefix_blocks(uint32_t n);
 // Wedge prefix of every graph of the set: nblocks = sum of prefix_blocks(n),

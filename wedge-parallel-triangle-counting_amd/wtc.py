@@ -119,6 +119,10 @@ _sig = {
     "wtc_dgraph_upload": (C.c_int, [_P, _BGP, C.c_uint32, C.POINTER(_P)]),
     "wtc_dgraph_free": (None, [_P]),
     "wtc_dgraph_bytes": (C.c_uint64, [_P]),
+    "wtc_dgraph_from_pairs": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.POINTER(_P)]),
+    "wtc_dgraph_gen_rmat": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                     C.c_uint32, C.POINTER(_P)]),
+    "wtc_dgraph_download": (C.c_int, [_P, C.POINTER(_BGP)]),
     "wtc_dgraph_count": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
                                   C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
     "wtc_ctx_debug_prefix": (C.c_int, [_P, C.c_uint64, _P, _P]),
@@ -222,8 +226,9 @@ def _adopt(ptr) -> CSR:
 
 
 def read_graph(path: str, matrix_market: bool = False, zero_indexed: bool = False,
-               preprocess_style: int = 2) -> CSR:
-    """read_graph of tc.cu:561-685."""
+               preprocess_style: int = 0) -> CSR:
+    """read_graph of tc.cu:561-685.  Style 0 builds on the host; 1 and 2 sort
+    on the GPU (same CSR).  The CLI's default is 2, like the reference's."""
     out = _BGP()
     _check(_lib.wtc_read_graph(path.encode(), int(matrix_market), int(zero_indexed), preprocess_style,
                                C.byref(out)), "read_graph")
@@ -239,8 +244,9 @@ def graph_from_pairs(pairs: np.ndarray) -> CSR:
     return _adopt(out)
 
 
-def preprocess(g: CSR, preprocess_style: int = 2) -> CSR:
-    """preprocess of tc.cu:687-787 (degree order, orientation, sorted rows)."""
+def preprocess(g: CSR, preprocess_style: int = 0) -> CSR:
+    """preprocess of tc.cu:687-787 (degree order, orientation, sorted rows).
+    Style 0 on the host; 1 and 2 on the GPU (same CSR)."""
     s = g._struct()
     out = _BGP()
     _check(_lib.wtc_preprocess(C.byref(s), preprocess_style, C.byref(out)), "preprocess")
@@ -374,12 +380,40 @@ class Context:
 class DeviceGraph:
     """An oriented CSR resident in HBM."""
 
-    def __init__(self, ctx: Context, g: CSR, offset_bytes: int = 0):
+    def __init__(self, ctx: Context, g: Optional[CSR], offset_bytes: int = 0, _handle=None):
         self.ctx = ctx
-        self.n, self.m = g.n, g.m
         self._p = C.c_void_p()
+        if _handle is not None:
+            self._p = _handle
+            host = self.download()
+            self.n, self.m = host.n, host.m
+            return
+        self.n, self.m = g.n, g.m
         s = g._struct()
         _check(_lib.wtc_dgraph_upload(ctx._p, C.byref(s), offset_bytes, C.byref(self._p)), "dgraph_upload")
+
+    @classmethod
+    def from_pairs(cls, ctx: Context, pairs: np.ndarray, offset_bytes: int = 0) -> "DeviceGraph":
+        """Oriented CSR built on the GPU from (src,dst) pairs (as read_graph -z + preprocess)."""
+        p = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        h = C.c_void_p()
+        _check(_lib.wtc_dgraph_from_pairs(ctx._p, p.ctypes.data if p.size else None, p.shape[0], offset_bytes,
+                                          C.byref(h)), "dgraph_from_pairs")
+        return cls(ctx, None, _handle=h)
+
+    @classmethod
+    def gen_rmat(cls, ctx: Context, scale: int, edge_factor: int = 16, a=0.57, b=0.19, c=0.19, seed: int = 1,
+                 offset_bytes: int = 0) -> "DeviceGraph":
+        """The graph of gen_rmat + preprocess, generated and oriented on the GPU."""
+        h = C.c_void_p()
+        _check(_lib.wtc_dgraph_gen_rmat(ctx._p, scale, edge_factor, a, b, c, seed, offset_bytes, C.byref(h)),
+               "dgraph_gen_rmat")
+        return cls(ctx, None, _handle=h)
+
+    def download(self) -> CSR:
+        out = _BGP()
+        _check(_lib.wtc_dgraph_download(self._p, C.byref(out)), "dgraph_download")
+        return _adopt(out)
 
     @property
     def nbytes(self) -> int:
