@@ -60,14 +60,41 @@ def test_alone_equals_mix():
     assert max(batch_sizes) > 1  # requests were batched together
 
 
+def _footprint(g, a):
+    up = lambda x: (x + 255) // 256 * 256  # noqa: E731  (the pool's 256-byte granule)
+    bitmap_words = (a * (a - 1) // 2 + 31) // 32
+    return up((g.n + 1) * 4 + g.m * 4) + up((g.n + 1) * 8) + (up(bitmap_words * 4) if a else 0)
+
+
 def test_eviction_under_small_budget():
     graphs = _pool()
     expect = [O.tc_merge(g) for g in graphs]
-    srv = wtc.Server(0, hbm_budget_bytes=6 << 20, max_batch=4, slots=2)
+    # room for the largest request and a little more, far less than all the CSRs
+    budget = _footprint(max(graphs, key=lambda g: g.m), 32) + (64 << 10)
+    assert budget < sum((g.n + 1) * 4 + g.m * 4 for g in graphs)
+    srv = wtc.Server(0, hbm_budget_bytes=budget, max_batch=4, slots=2)
     ids = [srv.register(g) for g in graphs]
     rids = [(k % len(ids), srv.submit(ids[k % len(ids)], 4, 32)) for k in range(60)]
     for gi, rid in rids:
-        assert srv.wait(rid)[0] == expect[gi]
+        assert srv.wait(rid, timeout_us=60_000_000)[0] == expect[gi]
+    stats = srv.stats()
+    srv.close()
+    assert stats["evictions"] > 0
+    assert stats["completed"] == 60
+
+
+def test_two_graphs_that_do_not_fit_together():
+    # Requests alternate between two graphs of which only one fits at a time:
+    # the arrival-order rule for evictions must still finish every request.
+    big = sorted(_pool(), key=lambda g: g.m)[-2:]
+    expect = [O.tc_merge(g) for g in big]
+    budget = max(_footprint(g, 64) for g in big) + 4096
+    assert budget < sum(_footprint(g, 64) for g in big)
+    srv = wtc.Server(0, hbm_budget_bytes=budget, max_batch=8, slots=2)
+    ids = [srv.register(g) for g in big]
+    rids = [(k % 2, srv.submit(ids[k % 2], 3, 64)) for k in range(40)]
+    for gi, rid in rids:
+        assert srv.wait(rid, timeout_us=60_000_000)[0] == expect[gi]
     stats = srv.stats()
     srv.close()
     assert stats["evictions"] > 0

@@ -87,6 +87,7 @@ struct SrvGraph {
     bool uploading = false, resident = false;
     uint32_t pins = 0;  // admitted requests using it
     uint64_t last_use = 0;
+    uint64_t first_rid = UINT64_MAX;  // oldest queued request naming it
 };
 
 enum ReqState { kQueued, kRunning, kDone, kFailed };
@@ -141,14 +142,19 @@ size_t req_ws_bytes(const SrvGraph *g) { return ((size_t)g->n + 1) * sizeof(uint
 size_t req_bm_bytes(uint32_t a) { return a ? bm_words_for(a) * sizeof(uint32_t) : 0; }
 
 // Frees the CSRs of idle resident graphs, least recently used first, until
-// `bytes` could be allocated.  Caller holds the lock.
-size_t alloc_evicting(wtc_server *s, size_t bytes) {
+// `bytes` could be allocated for request `rid`.  Graphs that an older queued
+// request still needs are kept: memory goes to requests in arrival order, so
+// two requests whose graphs do not fit together cannot evict each other's
+// graph back and forth.  Caller holds the lock.
+size_t alloc_evicting(wtc_server *s, size_t bytes, uint64_t rid) {
     size_t off = s->pool.alloc(bytes);
     while (off == SIZE_MAX) {
         SrvGraph *victim = nullptr;
         for (auto &kv : s->graphs) {
             SrvGraph *g = kv.second;
-            if (g->resident && g->pins == 0 && (!victim || g->last_use < victim->last_use)) victim = g;
+            if (g->resident && g->pins == 0 && g->first_rid > rid &&
+                (!victim || g->last_use < victim->last_use))
+                victim = g;
         }
         if (!victim) return SIZE_MAX;
         s->pool.release(victim->off, victim->bytes);
@@ -191,16 +197,21 @@ bool schedule(wtc_server *s) {
     // 2. uploads finished / needed
     for (auto &kv : s->graphs) {
         SrvGraph *g = kv.second;
+        g->first_rid = UINT64_MAX;
         if (g->uploading && hipEventQuery(g->uploaded) == hipSuccess) {
             g->uploading = false;
             g->resident = true;
             progress = true;
         }
     }
+    for (SrvReq *r : s->queue) {  // ids grow along the queue
+        SrvGraph *g = s->graphs[r->gid];
+        if (g->first_rid == UINT64_MAX) g->first_rid = r->id;
+    }
     for (SrvReq *r : s->queue) {
         SrvGraph *g = s->graphs[r->gid];
         if (g->resident || g->uploading) continue;
-        const size_t off = alloc_evicting(s, g->bytes);
+        const size_t off = alloc_evicting(s, g->bytes, r->id);
         if (off == SIZE_MAX) break;  // the pool is full of graphs in use: wait for completions
         g->off = off;
         char *dst = pool_ptr(s, off);
@@ -230,8 +241,8 @@ bool schedule(wtc_server *s) {
             r->ws_bytes = req_ws_bytes(g);
             r->bm_bytes = req_bm_bytes(r->a);
             g->pins++;  // keep the graph while its own workspace is being carved out
-            r->ws_off = s->pool.alloc(r->ws_bytes);
-            r->bm_off = r->bm_bytes ? s->pool.alloc(r->bm_bytes) : SIZE_MAX;
+            r->ws_off = alloc_evicting(s, r->ws_bytes, r->id);
+            r->bm_off = r->bm_bytes && r->ws_off != SIZE_MAX ? alloc_evicting(s, r->bm_bytes, r->id) : SIZE_MAX;
             if (r->ws_off == SIZE_MAX || (r->bm_bytes && r->bm_off == SIZE_MAX)) {
                 if (r->ws_off != SIZE_MAX) s->pool.release(r->ws_off, r->ws_bytes);
                 if (r->bm_off != SIZE_MAX) s->pool.release(r->bm_off, r->bm_bytes);
