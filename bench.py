@@ -286,6 +286,7 @@ def main():
     ap.add_argument("--a", type=int, default=16384)
     ap.add_argument("--ways", type=int, default=4, help="probes advanced together per lane")
     ap.add_argument("--chunk", type=int, default=16, help="wedges per lane staged in LDS per phase")
+    ap.add_argument("--window", type=int, default=0, help="probe window in elements (0, 8 or 16)")
     ap.add_argument("--offset-bytes", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -325,7 +326,7 @@ def main():
         return
     if args.batch_only:
         ctx = wtc.Context(0)
-        ctx.set_tuning(args.ways, args.chunk)
+        ctx.set_tuning(args.ways, args.chunk, args.window)
         res = batch_bench(ctx, args.batch_graphs or 4096, 1000, 32, min(len(os.sched_getaffinity(0)), 16))
         ctx.close()
         print(json.dumps(res), flush=True)
@@ -344,7 +345,7 @@ def main():
     b_alg, b_min = g.alg_bytes(ob)
 
     ctx = wtc.Context(0)
-    ctx.set_tuning(args.ways, args.chunk)
+    ctx.set_tuning(args.ways, args.chunk, args.window)
     dg = ctx.upload(g, offset_bytes=ob)
     log(f"uploaded {dg.nbytes / 1e9:.2f} GB")
     for _ in range(args.warmup):

@@ -170,8 +170,11 @@ int wtc_ctx_create(int device, wtc_ctx **out);
 void wtc_ctx_destroy(wtc_ctx *ctx);
 /* Tuning of the counting kernel: probe_ways = independent binary searches
  * each lane advances together (1, 2, 4 or 8; default 4 or $WTC_PROBE_WAYS);
- * lds_chunk = wedges per lane staged in LDS per phase (1..16, default 16). */
-int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk);
+ * lds_chunk = wedges per lane staged in LDS per phase (1..16, default 16);
+ * probe_window = 0 for a plain binary search over adj+(w), or 8 / 16 to end
+ * the search with one aligned window of that many elements tested by vector
+ * loads (default 0 or $WTC_PROBE_WINDOW).  Counts do not depend on these. */
+int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int probe_window);
 
 /* Copies an oriented CSR to the device.  offset_bytes 4 or 8 selects the
  * device rowPtr width (0: 4 if numEdges < 2^32 else 8). */
@@ -223,10 +226,17 @@ typedef struct wtc_batch wtc_batch;
 
 int wtc_batch_upload(wtc_ctx *ctx, const wtc_big_graph *const *graphs, size_t k, wtc_batch **out);
 void wtc_batch_free(wtc_batch *b);
+/* Counts a resident batch (ctx must be the one it was uploaded with).  The
+ * launch sequence is captured into a HIP graph on the batch's own stream and
+ * workspace the first time, and replayed as one graph launch while spread,
+ * a and the context's tuning stay the same (a change captures anew).  In
+ * *st, count_ms is the device time of the whole replay (prefix, bitmap and
+ * count); prefix_ms and bitmap_ms are 0. */
 int wtc_batch_count(wtc_ctx *ctx, wtc_batch *b, const uint32_t *spread, const uint32_t *adjacency_matrix_len,
                     uint64_t *triangles, wtc_count_stats *st);
 /* Host CSRs in, counts out: upload + count + free in one call (the batched
- * counterpart of calling wtc_tc_gpu once per graph). */
+ * counterpart of calling wtc_tc_gpu once per graph); launched directly on
+ * the context's stream, without a graph. */
 int wtc_batch_count_host(wtc_ctx *ctx, const wtc_big_graph *const *graphs, size_t k, const uint32_t *spread,
                          const uint32_t *adjacency_matrix_len, uint64_t *triangles, wtc_count_stats *st);
 /* ---------------------------------------------------------- GPU: serving
@@ -246,6 +256,7 @@ typedef struct {
     uint32_t queue_capacity;   /* queued requests before WTC_ERR_BUSY (65536)     */
     int32_t probe_ways;        /* counting-kernel tuning, 0 = default             */
     uint32_t lds_chunk;        /* counting-kernel tuning, 0 = default             */
+    int32_t probe_window;      /* 0 = default, -1 = none, else 8 or 16            */
 } wtc_server_config;
 typedef struct {
     double queue_ms;      /* submit -> batch start                                */

@@ -132,3 +132,69 @@ def test_batch_exact(gpu_ctx):
     b.free()
     got = wtc.batch_count_host(gpu_ctx, graphs)
     assert got.tolist() == expect
+
+
+@pytest.mark.parametrize("win", [0, 8, 16])
+@pytest.mark.parametrize("ways", [1, 2, 4, 8])
+def test_probe_variants(gpu_ctx, ways, win):
+    """Every (probe_ways, probe_window) kernel variant gives the same counts,
+    on single graphs (u32 and u64 offsets) and on batches, whose per-graph
+    colInd starts are not 16-byte aligned (the window's skew path)."""
+    g = _rmat(13, seed=21)
+    expect = O.tc_merge(g)
+    k300 = wtc.preprocess(wtc.graph_from_pairs(
+        np.array([[i, j] for i in range(300) for j in range(i + 1, 300)], np.uint32)))
+    graphs = [wtc.preprocess(wtc.gen_barabasi_albert(70 + 13 * i, 1 + i % 6, seed=40 + i)) for i in range(17)]
+    bexp = [O.tc_merge(x) for x in graphs]
+    try:
+        for chunk in (3, 16):
+            gpu_ctx.set_tuning(ways, chunk, win)
+            for ob in (4, 8):
+                dg = gpu_ctx.upload(g, offset_bytes=ob)
+                for s, a in ((1, 0), (4, 0), (7, 64), (16, 4096)):
+                    assert dg.count(s, a) == expect, (ways, win, chunk, ob, s, a)
+                dg.free()
+            dk = gpu_ctx.upload(k300)
+            assert dk.count(5, 0) == 300 * 299 * 298 // 6
+            dk.free()
+            b = wtc.Batch(gpu_ctx, graphs)
+            assert b.count(3, 0).tolist() == bexp
+            assert b.count(6, 32).tolist() == bexp
+            b.free()
+    finally:
+        gpu_ctx.set_tuning()
+
+
+def test_batch_graph_replays(gpu_ctx):
+    """A resident batch replays its captured graph: every replay, with eager
+    work and synchronisations between them, equals the eager count; a new
+    (spread, a) or tuning captures anew and stays exact."""
+    graphs = []
+    for i in range(24):
+        if i % 2:
+            graphs.append(wtc.preprocess(wtc.gen_erdos_renyi(80 + 51 * i, 10.0 / (80 + 51 * i), seed=300 + i)))
+        else:
+            graphs.append(wtc.preprocess(wtc.gen_barabasi_albert(90 + 41 * i, 2 + i % 5, seed=300 + i)))
+    expect = [O.tc_merge(g) for g in graphs]
+    eager = wtc.batch_count_host(gpu_ctx, graphs)
+    assert eager.tolist() == expect
+    side = _rmat(11, seed=33)
+    side_expect = O.tc_merge(side)
+    b = wtc.Batch(gpu_ctx, graphs)
+    try:
+        for rep in range(5):
+            got, st = b.count(stats=True)
+            assert got.tolist() == expect, rep
+            assert st.wedges == sum(g.wedges() for g in graphs)
+            dg = gpu_ctx.upload(side)  # eager work on the context's own stream between replays
+            assert dg.count(3, 64) == side_expect
+            dg.free()
+        for spread, a in ((2, 32), ([1 + i % 7 for i in range(24)], [32 * (i % 3) for i in range(24)])):
+            for rep in range(3):
+                assert b.count(spread, a).tolist() == expect, (spread, a, rep)
+        gpu_ctx.set_tuning(2, 8, 8)
+        for rep in range(3):
+            assert b.count(4, 64).tolist() == expect, rep
+    finally:
+        gpu_ctx.set_tuning()
+        b.free()

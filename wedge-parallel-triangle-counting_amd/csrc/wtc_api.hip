@@ -23,6 +23,14 @@ struct wtc_batch {
     std::vector<uint32_t> n;
     std::vector<uint64_t> m, row_off, col_off;  // element offsets into rowPtr / colInd
     DevBuf rowPtr, colInd;
+    // Replays: the batch's own stream and workspace (a captured graph keeps
+    // their addresses) and the graph of the last (spread, a, tuning) counted.
+    wtc::Slot slot;
+    hipGraphExec_t exec = nullptr;
+    wtc::SetPlan plan;
+    std::vector<uint32_t> key_s, key_a;
+    int key_ways = 0, key_win = 0;
+    uint32_t key_chunk = 0;
 };
 
 namespace wtc {
@@ -46,8 +54,10 @@ hipError_t DevBuf::ensure(size_t bytes) {
         cap = 0;
         if (e != hipSuccess) return e;
     }
+    // kDevTailSlack readable bytes past the end: the counting kernel's probe
+    // window may load up to 15 elements beyond the last colInd entry.
     const size_t want = std::max<size_t>(bytes, 256);
-    hipError_t e = hipMalloc(&p, want);
+    hipError_t e = hipMalloc(&p, want + kDevTailSlack);
     if (e == hipSuccess) cap = want;
     else p = nullptr;
     return e;
@@ -136,8 +146,9 @@ GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint
 }
 
 static int ways_index(int w) { return w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3; }
+static int win_index(int v) { return v == 8 ? 1 : v == 16 ? 2 : 0; }
 
-int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd) {
+int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan) {
     const uint32_t G = (uint32_t)hd.size();
     if (G == 0) return WTC_ERR_ARG;
     uint32_t max_spread = 1;
@@ -179,30 +190,51 @@ int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd) {
     }
     std::memcpy(s.h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
 
+    const int oi = ob == 8 ? 1 : 0;
+    const uint32_t staged = std::min<uint32_t>(max_spread, t.chunk);
+    uint32_t &gc = t.grid_cache[oi][ways_index(t.ways)][win_index(t.win)][staged];
+    if (!gc) WTC_HIPC(count_grid(ob, staged, t.ways, t.win, t.num_cus, &gc));
+    s.G = G;
+    s.grid = gc;
+    plan->G = G;
+    plan->ob = ob;
+    plan->blocks = (uint32_t)blocks;
+    plan->rows = rows;
+    plan->clear_words = clear_off;
+    plan->max_spread = max_spread;
+    plan->single = single;
+    plan->ways = t.ways;
+    plan->win = t.win;
+    plan->chunk = t.chunk;
+    return WTC_OK;
+}
+
+int launch_set(Slot &s, const SetPlan &p, bool timed) {
     uint64_t *meta = s.out.as<uint64_t>();
     unsigned long long *dcounts = reinterpret_cast<unsigned long long *>(meta + kMetaSlots);
     GraphDesc *ddesc = s.desc.as<GraphDesc>();
-    uint64_t *tree = single ? s.tree.as<uint64_t>() : nullptr;
-
-    const int oi = ob == 8 ? 1 : 0;
-    const uint32_t staged = std::min<uint32_t>(max_spread, t.chunk);
-    uint32_t &gc = t.grid_cache[oi][ways_index(t.ways)][staged];
-    if (!gc) WTC_HIPC(count_grid(ob, staged, t.ways, t.num_cus, &gc));
-    s.G = G;
-    s.grid = gc;
-
-    WTC_HIPC(hipMemcpyAsync(ddesc, s.h_desc.p, (size_t)G * sizeof(GraphDesc), hipMemcpyHostToDevice, s.stream));
-    WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
-    WTC_HIPC(launch_prefix(ddesc, G, ob, (uint32_t)blocks, s.partials.as<uint64_t>(), s.stream));
-    WTC_HIPC(launch_setup(ddesc, G, tree, meta, dcounts, s.stream));
-    WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
-    WTC_HIPC(launch_bitmap(ddesc, G, ob, rows, clear_off, s.stream));
-    WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
-    WTC_HIPC(launch_count(ddesc, G, ob, tree, meta, dcounts, max_spread, t.chunk, t.ways, s.grid, s.stream));
-    WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
-    WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + G) * sizeof(uint64_t), hipMemcpyDeviceToHost,
+    uint64_t *tree = p.single ? s.tree.as<uint64_t>() : nullptr;
+    if (timed) WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
+    WTC_HIPC(launch_prefix(ddesc, p.G, p.ob, p.blocks, s.partials.as<uint64_t>(), s.stream));
+    WTC_HIPC(launch_setup(ddesc, p.G, tree, meta, dcounts, s.stream));
+    if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
+    WTC_HIPC(launch_bitmap(ddesc, p.G, p.ob, p.rows, p.clear_words, s.stream));
+    if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
+    WTC_HIPC(launch_count(ddesc, p.G, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways, p.win, s.grid,
+                          s.stream));
+    if (timed) WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
+    WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + p.G) * sizeof(uint64_t), hipMemcpyDeviceToHost,
                             s.stream));
     return WTC_OK;
+}
+
+int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd) {
+    SetPlan p;
+    int rc = stage_set(s, t, ob, hd, &p);
+    if (rc) return rc;
+    WTC_HIPC(hipMemcpyAsync(s.desc.p, s.h_desc.p, (size_t)p.G * sizeof(GraphDesc), hipMemcpyHostToDevice,
+                            s.stream));
+    return launch_set(s, p, true);
 }
 
 bool slot_done(Slot &s) { return hipStreamQuery(s.stream) == hipSuccess; }
@@ -225,6 +257,7 @@ int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt) {
 namespace {
 
 bool valid_ways(int w) { return w == 1 || w == 2 || w == 4 || w == 8; }
+bool valid_window(int v) { return v == 0 || v == 8 || v == 16; }
 
 // Default probes per lane for new contexts: WTC_PROBE_WAYS (1, 2, 4 or 8),
 // else 4.
@@ -232,6 +265,13 @@ int default_probe_ways() {
     const char *e = std::getenv("WTC_PROBE_WAYS");
     const int v = e ? std::atoi(e) : 4;
     return valid_ways(v) ? v : 4;
+}
+
+// Default probe window for new contexts: WTC_PROBE_WINDOW (0, 8 or 16), else 0.
+int default_probe_window() {
+    const char *e = std::getenv("WTC_PROBE_WINDOW");
+    const int v = e ? std::atoi(e) : 0;
+    return valid_window(v) ? v : 0;
 }
 
 int set_device(int dev) {
@@ -378,6 +418,7 @@ int wtc_ctx_create(int device, wtc_ctx **out) {
     }
     c->tune.num_cus = prop.multiProcessorCount;
     c->tune.ways = default_probe_ways();
+    c->tune.win = default_probe_window();
     if ((rc = c->slot.create())) {
         wtc_ctx_destroy(c);
         return rc;
@@ -386,10 +427,12 @@ int wtc_ctx_create(int device, wtc_ctx **out) {
     return WTC_OK;
 }
 
-int wtc_ctx_set_tuning(wtc_ctx *c, int probe_ways, uint32_t lds_chunk) {
-    if (!c || !valid_ways(probe_ways) || lds_chunk < 1 || lds_chunk > kChunkMax) return WTC_ERR_ARG;
+int wtc_ctx_set_tuning(wtc_ctx *c, int probe_ways, uint32_t lds_chunk, int probe_window) {
+    if (!c || !valid_ways(probe_ways) || lds_chunk < 1 || lds_chunk > kChunkMax || !valid_window(probe_window))
+        return WTC_ERR_ARG;
     c->tune.ways = probe_ways;
     c->tune.chunk = lds_chunk;
+    c->tune.win = probe_window;
     return WTC_OK;
 }
 
@@ -634,25 +677,117 @@ int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, w
 void wtc_batch_free(wtc_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->ctx->device);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->slot.destroy();
     b->rowPtr.release();
     b->colInd.release();
     delete b;
 }
 
-int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint32_t *a, uint64_t *triangles,
-                    wtc_count_stats *st) {
-    if (!c || !b || !triangles) return WTC_ERR_ARG;
-    if (b->G == 0) return WTC_OK;
-    WTC_HIPC(hipSetDevice(c->device));
-    std::vector<GraphDesc> hd(b->G);
+namespace {
+
+// Per-graph spread and bitmap length of a batch count (autotuned where 0).
+int batch_params(const wtc_batch *b, const uint32_t *spread, const uint32_t *a, std::vector<uint32_t> &s_out,
+                 std::vector<uint32_t> &a_out) {
+    s_out.resize(b->G);
+    a_out.resize(b->G);
     for (uint32_t g = 0; g < b->G; g++) {
         uint32_t s = spread ? spread[g] : 0, aa = a ? a[g] : 0;
         if (s == 0) wtc_autotune(b->n[g], b->m[g], 0, &s, a ? nullptr : &aa);
         if (s == 0 || aa % 32 != 0) return WTC_ERR_ARG;
-        const void *rp = b->rowPtr.as<char>() + b->row_off[g] * b->ob;
-        hd[g] = make_desc(rp, b->colInd.as<uint32_t>() + b->col_off[g], b->n[g], s, aa, 0, UINT64_MAX);
+        s_out[g] = s;
+        a_out[g] = aa;
     }
-    return run_set(c, b->ob, hd, triangles, st);
+    return WTC_OK;
+}
+
+std::vector<GraphDesc> batch_descs(const wtc_batch *b, const std::vector<uint32_t> &s,
+                                   const std::vector<uint32_t> &a) {
+    std::vector<GraphDesc> hd(b->G);
+    for (uint32_t g = 0; g < b->G; g++) {
+        const void *rp = b->rowPtr.as<char>() + b->row_off[g] * b->ob;
+        hd[g] = make_desc(rp, b->colInd.as<uint32_t>() + b->col_off[g], b->n[g], s[g], a[g], 0, UINT64_MAX);
+    }
+    return hd;
+}
+
+// Captures the batch's launch sequence (kernels + the copy of the counts to
+// the host) into a graph on the batch's stream.  The descriptors go to the
+// device before the capture, so the graph holds no host-to-device copy and
+// no memset: every replay recomputes the prefix, tree, bitmap and counts
+// from the resident CSRs.
+int batch_capture(wtc_ctx *c, wtc_batch *b, const std::vector<uint32_t> &s, const std::vector<uint32_t> &a) {
+    if (b->exec) {
+        WTC_HIPC(hipGraphExecDestroy(b->exec));
+        b->exec = nullptr;
+    }
+    if (!b->slot.stream) {
+        int rc = b->slot.create();
+        if (rc) return rc;
+    }
+    std::vector<GraphDesc> hd = batch_descs(b, s, a);
+    int rc = stage_set(b->slot, c->tune, b->ob, hd, &b->plan);
+    if (rc) return rc;
+    hipStream_t st = b->slot.stream;
+    WTC_HIPC(hipMemcpyAsync(b->slot.desc.p, b->slot.h_desc.p, (size_t)b->G * sizeof(GraphDesc),
+                            hipMemcpyHostToDevice, st));
+    WTC_HIPC(hipStreamSynchronize(st));
+    WTC_HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    rc = launch_set(b->slot, b->plan, false);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (rc == WTC_OK && ec != hipSuccess) rc = hip_fail("batch capture", ec);
+    if (rc == WTC_OK) {
+        const hipError_t ei = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            b->exec = nullptr;
+            rc = hip_fail("batch graph instantiate", ei);
+        }
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    b->key_s = s;
+    b->key_a = a;
+    b->key_ways = c->tune.ways;
+    b->key_win = c->tune.win;
+    b->key_chunk = c->tune.chunk;
+    return WTC_OK;
+}
+
+}  // namespace
+
+int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint32_t *a, uint64_t *triangles,
+                    wtc_count_stats *st) {
+    if (!c || !b || !triangles || b->ctx != c) return WTC_ERR_ARG;
+    if (b->G == 0) return WTC_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    WTC_HIPC(hipSetDevice(c->device));
+    std::vector<uint32_t> s, aa;
+    int rc = batch_params(b, spread, a, s, aa);
+    if (rc) return rc;
+    if (!b->exec || s != b->key_s || aa != b->key_a || c->tune.ways != b->key_ways || c->tune.win != b->key_win ||
+        c->tune.chunk != b->key_chunk) {
+        if ((rc = batch_capture(c, b, s, aa))) return rc;
+    }
+    Slot &sl = b->slot;
+    WTC_HIPC(hipEventRecord(sl.ev[0], sl.stream));
+    WTC_HIPC(hipGraphLaunch(b->exec, sl.stream));
+    WTC_HIPC(hipEventRecord(sl.ev[3], sl.stream));
+    WTC_HIPC(hipStreamSynchronize(sl.stream));
+    const uint64_t *h = sl.h_out.as<uint64_t>();
+    for (uint32_t g = 0; g < b->G; g++) triangles[g] = h[kMetaSlots + g];
+    if (st) {
+        float ms = 0;
+        WTC_HIPC(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[3]));
+        st->prefix_ms = 0;
+        st->bitmap_ms = 0;
+        st->count_ms = ms;
+        st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        st->wedges = h[kMetaWedges];
+        st->blocks = sl.grid;
+        st->launches = 1;
+    }
+    return WTC_OK;
 }
 
 int wtc_batch_count_host(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, const uint32_t *spread,
@@ -661,7 +796,13 @@ int wtc_batch_count_host(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t 
     wtc_batch *b = nullptr;
     int rc = wtc_batch_upload(c, graphs, k, &b);
     if (rc) return rc;
-    rc = wtc_batch_count(c, b, spread, a, triangles, st);
+    // one pass over freshly uploaded graphs: launched directly, no capture
+    std::vector<uint32_t> s, aa;
+    rc = batch_params(b, spread, a, s, aa);
+    if (rc == WTC_OK && b->G) {
+        std::vector<GraphDesc> hd = batch_descs(b, s, aa);
+        rc = run_set(c, b->ob, hd, triangles, st);
+    }
     wtc_batch_free(b);
     auto t1 = std::chrono::steady_clock::now();
     if (st && rc == WTC_OK) st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();

@@ -23,6 +23,9 @@ void set_error(const std::string &msg);
         if (_e != hipSuccess) return ::wtc::hip_fail(#call, _e);  \
     } while (0)
 
+// Readable bytes every device buffer carries past its requested size.
+constexpr size_t kDevTailSlack = 256;
+
 // Grow-only device buffer.
 struct DevBuf {
     void *p = nullptr;
@@ -65,8 +68,9 @@ struct Slot {
 struct Tuning {
     int num_cus = 0;
     int ways = 4;         // probes advanced together per lane
+    int win = 0;          // probe window in elements: 0, 8 or 16
     uint32_t chunk = 16;  // wedges per lane staged in LDS per phase
-    uint32_t grid_cache[2][4][kChunkMax + 2] = {};
+    uint32_t grid_cache[2][4][3][kChunkMax + 2] = {};
 };
 
 struct RunTimes {
@@ -83,9 +87,24 @@ int check_args(uint32_t spread, uint32_t a);
 GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint32_t spread, uint32_t a,
                     uint64_t w_begin, uint64_t w_end);
 
-// Launches the whole device pipeline for a graph set on the slot's stream
-// and the copy of its counts into the slot's pinned buffer; returns without
-// waiting.  All graphs of a set share the row-offset width ob.
+// What launch_set needs to replay the launches of a staged set.
+struct SetPlan {
+    uint32_t G = 0, blocks = 0, max_spread = 1, chunk = 16;
+    int ob = 4, ways = 4, win = 0;
+    uint64_t rows = 0, clear_words = 0;
+    bool single = false;
+};
+// Sizes the slot's workspace for the set, assigns each graph its prefix and
+// bitmap (unless given), fills the pinned host descriptors and the plan.
+int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan);
+// Enqueues the set's kernels and the copy of meta + counts to the host; no
+// host-to-device copies, allocations or memsets, so it can be captured into
+// a graph once the descriptors are on the device.  timed: record ev[0..3].
+int launch_set(Slot &s, const SetPlan &p, bool timed);
+// stage_set + descriptors to the device + launch_set: the whole device
+// pipeline for a graph set on the slot's stream and the copy of its counts
+// into the slot's pinned buffer; returns without waiting.  All graphs of a
+// set share the row-offset width ob.
 int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd);
 // Waits for the slot (or, if it is done, does not wait) and reads results.
 int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt);

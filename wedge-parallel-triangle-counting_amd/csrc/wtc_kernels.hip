@@ -277,7 +277,16 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
 // Tiles a wave takes per grab from its XCD's work counter.
 constexpr uint64_t kTilesPerGrab = 8;
 
-template <typename OffT, int kWays>
+// Elements between the 16-byte aligned address at or below p and p.
+__device__ __forceinline__ uint32_t win_skew(const uint32_t *p) {
+    return (uint32_t)((reinterpret_cast<uintptr_t>(p) >> 2) & 3u);
+}
+
+// kWays: independent probes each lane advances together.  kWin: 0 for a plain
+// binary search over adj+(w); 8 or 16 to stop the search once the remaining
+// range fits an aligned window of that many elements and test the window
+// with vector loads (fewer dependent round trips per probe).
+template <typename OffT, int kWays, int kWin>
 __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ desc, uint32_t G,
                                                   const uint64_t *__restrict__ tree, uint64_t *__restrict__ meta,
                                                   unsigned long long *__restrict__ counts, uint32_t pad_max,
@@ -455,17 +464,22 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                             }
                         }
                     }
+                    // Binary search; with a window, only until the rest of
+                    // the range fits one aligned kWin-element window.
                     for (;;) {
-                        bool more = false;
+                        bool act[kWays], more = false;
 #pragma unroll
-                        for (int k = 0; k < kWays; k++) more |= len[k] > 0;
+                        for (int k = 0; k < kWays; k++) {
+                            act[k] = len[k] > 0 && (kWin == 0 || (uint64_t)len[k] + win_skew(colInd + b[k]) > kWin);
+                            more |= act[k];
+                        }
                         if (!more) break;
                         uint32_t x[kWays];
 #pragma unroll
-                        for (int k = 0; k < kWays; k++) x[k] = len[k] > 0 ? colInd[b[k] + (len[k] >> 1)] : 0u;
+                        for (int k = 0; k < kWays; k++) x[k] = act[k] ? colInd[b[k] + (len[k] >> 1)] : 0u;
 #pragma unroll
                         for (int k = 0; k < kWays; k++) {
-                            if (len[k] > 0) {
+                            if (act[k]) {
                                 const OffT half = len[k] >> 1;
                                 if (x[k] == pu[k]) {
                                     hit[k] = true;
@@ -476,6 +490,39 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                                 } else {
                                     len[k] = half;
                                 }
+                            }
+                        }
+                    }
+                    if constexpr (kWin > 0) {
+                        // [b, b+len) lies inside the 16-byte aligned window that
+                        // starts at or just below colInd + b: one vector load per
+                        // 4 elements, then compares in registers.  The window may
+                        // reach up to kWin-1 elements past the end of colInd;
+                        // every colInd allocation carries that much tail slack.
+                        uint4 wv4[kWays][kWin / 4];
+#pragma unroll
+                        for (int k = 0; k < kWays; k++) {
+                            if (len[k] > 0) {
+                                const uint4 *p = reinterpret_cast<const uint4 *>(
+                                    reinterpret_cast<uintptr_t>(colInd + b[k]) & ~(uintptr_t)15);
+#pragma unroll
+                                for (int q = 0; q < kWin / 4; q++) wv4[k][q] = p[q];
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < kWays; k++) {
+                            if (len[k] > 0) {
+                                const uint32_t lo = win_skew(colInd + b[k]), hi = lo + (uint32_t)len[k];
+                                bool f = false;
+#pragma unroll
+                                for (int q = 0; q < kWin / 4; q++) {
+                                    const uint32_t e0 = 4 * q;
+                                    f |= (wv4[k][q].x == pu[k]) & (e0 + 0 >= lo) & (e0 + 0 < hi);
+                                    f |= (wv4[k][q].y == pu[k]) & (e0 + 1 >= lo) & (e0 + 1 < hi);
+                                    f |= (wv4[k][q].z == pu[k]) & (e0 + 2 >= lo) & (e0 + 2 < hi);
+                                    f |= (wv4[k][q].w == pu[k]) & (e0 + 3 >= lo) & (e0 + 3 < hi);
+                                }
+                                hit[k] = hit[k] || f;
                             }
                         }
                     }
@@ -605,22 +652,49 @@ static uint32_t count_pad(uint32_t max_spread) {
     return sc | 1u;
 }
 
-template <int W>
-static hipError_t occupancy_w(int ob, size_t shm, int *per_cu) {
-    return ob == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint64_t, W>, kBlock, shm)
-                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint32_t, W>, kBlock, shm);
+// Calls f.template run<W, V>() for probe_ways W and probe window V.
+template <class F>
+static hipError_t with_variant(int ways, int win, F &f) {
+    switch (win) {
+        case 8:
+            switch (ways) {
+                case 1: return f.template run<1, 8>();
+                case 2: return f.template run<2, 8>();
+                case 8: return f.template run<8, 8>();
+                default: return f.template run<4, 8>();
+            }
+        case 16:
+            switch (ways) {
+                case 1: return f.template run<1, 16>();
+                case 2: return f.template run<2, 16>();
+                case 8: return f.template run<8, 16>();
+                default: return f.template run<4, 16>();
+            }
+        default:
+            switch (ways) {
+                case 1: return f.template run<1, 0>();
+                case 2: return f.template run<2, 0>();
+                case 8: return f.template run<8, 0>();
+                default: return f.template run<4, 0>();
+            }
+    }
 }
 
-hipError_t count_grid(int ob, uint32_t max_spread, int ways, int num_cus, uint32_t *grid) {
-    int per_cu = 0;
-    const size_t shm = count_lds_bytes(max_spread);
-    hipError_t e;
-    switch (ways) {
-        case 1: e = occupancy_w<1>(ob, shm, &per_cu); break;
-        case 2: e = occupancy_w<2>(ob, shm, &per_cu); break;
-        case 8: e = occupancy_w<8>(ob, shm, &per_cu); break;
-        default: e = occupancy_w<4>(ob, shm, &per_cu); break;
+struct OccupancyQuery {
+    int ob;
+    size_t shm;
+    int *per_cu;
+    template <int W, int V>
+    hipError_t run() {
+        return ob == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint64_t, W, V>, kBlock, shm)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint32_t, W, V>, kBlock, shm);
     }
+};
+
+hipError_t count_grid(int ob, uint32_t max_spread, int ways, int win, int num_cus, uint32_t *grid) {
+    int per_cu = 0;
+    OccupancyQuery q{ob, count_lds_bytes(max_spread), &per_cu};
+    hipError_t e = with_variant(ways, win, q);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     uint64_t g = (uint64_t)per_cu * (uint64_t)(num_cus > 0 ? num_cus : 1);
@@ -630,30 +704,34 @@ hipError_t count_grid(int ob, uint32_t max_spread, int ways, int num_cus, uint32
     return hipSuccess;
 }
 
-template <int W>
-static hipError_t launch_count_w(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree,
-                                 uint64_t *meta, unsigned long long *counts, uint32_t pad, uint32_t chunk,
-                                 size_t shm, uint32_t grid, hipStream_t st) {
-    if (ob == 8)
-        k_count<uint64_t, W><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
-    else
-        k_count<uint32_t, W><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
-    return hipGetLastError();
-}
+struct CountLaunch {
+    const GraphDesc *desc;
+    uint32_t G;
+    int ob;
+    const uint64_t *tree;
+    uint64_t *meta;
+    unsigned long long *counts;
+    uint32_t pad, chunk;
+    size_t shm;
+    uint32_t grid;
+    hipStream_t st;
+    template <int W, int V>
+    hipError_t run() {
+        if (ob == 8)
+            k_count<uint64_t, W, V><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
+        else
+            k_count<uint32_t, W, V><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
+        return hipGetLastError();
+    }
+};
 
 hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
-                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, uint32_t grid,
-                        hipStream_t st) {
+                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, int win,
+                        uint32_t grid, hipStream_t st) {
     if (chunk < 1 || chunk > kChunkMax) return hipErrorInvalidValue;
     const uint32_t eff = max_spread < chunk ? max_spread : chunk;
-    const size_t shm = count_lds_bytes(eff);
-    const uint32_t pad = count_pad(eff);
-    switch (ways) {
-        case 1: return launch_count_w<1>(desc, G, ob, tree, meta, counts, pad, chunk, shm, grid, st);
-        case 2: return launch_count_w<2>(desc, G, ob, tree, meta, counts, pad, chunk, shm, grid, st);
-        case 8: return launch_count_w<8>(desc, G, ob, tree, meta, counts, pad, chunk, shm, grid, st);
-        default: return launch_count_w<4>(desc, G, ob, tree, meta, counts, pad, chunk, shm, grid, st);
-    }
+    CountLaunch l{desc, G, ob, tree, meta, counts, count_pad(eff), chunk, count_lds_bytes(eff), grid, st};
+    return with_variant(ways, win, l);
 }
 
 }  // namespace wtc

@@ -27,11 +27,12 @@ hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t row
 // LDS bytes of a counting block that stages `staged` wedges per lane.
 size_t count_lds_bytes(uint32_t staged);
 // `ways`: independent probes each lane advances together (1, 2, 4 or 8);
+// `win`: probe window in elements (0 = plain binary search, 8 or 16);
 // `staged` = min(max spread, chunk).
-hipError_t count_grid(int ob, uint32_t staged, int ways, int num_cus, uint32_t *grid);
+hipError_t count_grid(int ob, uint32_t staged, int ways, int win, int num_cus, uint32_t *grid);
 // `chunk` (1..kChunkMax): wedges per lane staged in LDS per phase.
 hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
-                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, uint32_t grid,
-                        hipStream_t st);
+                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, int win,
+                        uint32_t grid, hipStream_t st);
 
 }  // namespace wtc

@@ -34,6 +34,11 @@ using Clock = std::chrono::steady_clock;
 namespace {
 
 constexpr size_t kAlign = 256;
+// A batch finishes as a unit, so it only takes requests whose wedge counts
+// lie within a factor kCostSpread of its oldest request's; graphs below
+// kCostFloor wedges count as equal (their time is launch latency).
+constexpr uint64_t kCostSpread = 16;
+constexpr uint64_t kCostFloor = 1ull << 20;
 size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
 // First-fit allocator over offsets of one device allocation, with coalescing.
@@ -88,6 +93,7 @@ struct SrvGraph {
     uint32_t pins = 0;  // admitted requests using it
     uint64_t last_use = 0;
     uint64_t first_rid = UINT64_MAX;  // oldest queued request naming it
+    uint64_t cost = 0;                 // batching cost class: max(W, kCostFloor)
 };
 
 enum ReqState { kQueued, kRunning, kDone, kFailed };
@@ -227,14 +233,21 @@ bool schedule(wtc_server *s) {
         s->stats.resident_bytes += g->bytes;
         progress = true;
     }
-    // 3. a free slot takes every admitted request whose graph is resident
+    // 3. a free slot takes, oldest first, the queued requests whose graph is
+    // resident and whose cost is in the class of the oldest such request
     for (auto &x : s->slots) {
         if (x.busy || s->queue.empty()) continue;
         std::vector<GraphDesc> hd;
+        uint64_t c0 = 0;
         for (auto it = s->queue.begin(); it != s->queue.end() && hd.size() < s->cfg.max_batch;) {
             SrvReq *r = *it;
             SrvGraph *g = s->graphs[r->gid];
             if (!g->resident) {
+                ++it;
+                continue;
+            }
+            if (c0 == 0) c0 = g->cost;
+            if (g->cost > c0 * kCostSpread || g->cost * kCostSpread < c0) {
                 ++it;
                 continue;
             }
@@ -324,6 +337,11 @@ int wtc_server_create(int device, const wtc_server_config *cfg, wtc_server **out
     s->cfg.queue_capacity = cfg && cfg->queue_capacity ? cfg->queue_capacity : 65536;
     s->cfg.probe_ways = cfg && cfg->probe_ways ? cfg->probe_ways : 4;
     s->cfg.lds_chunk = cfg && cfg->lds_chunk ? cfg->lds_chunk : 16;
+    s->cfg.probe_window = cfg && cfg->probe_window ? cfg->probe_window : -1;
+    if (s->cfg.probe_window != -1 && s->cfg.probe_window != 8 && s->cfg.probe_window != 16) {
+        delete s;
+        return WTC_ERR_ARG;
+    }
     hipDeviceProp_t prop;
     hipError_t e = hipGetDeviceProperties(&prop, device);
     if (e != hipSuccess) {
@@ -333,6 +351,7 @@ int wtc_server_create(int device, const wtc_server_config *cfg, wtc_server **out
     s->tune.num_cus = prop.multiProcessorCount;
     s->tune.ways = s->cfg.probe_ways;
     s->tune.chunk = s->cfg.lds_chunk;
+    s->tune.win = s->cfg.probe_window > 0 ? s->cfg.probe_window : 0;
     int rc = WTC_OK;
     if ((e = s->pool_mem.ensure(s->cfg.hbm_budget_bytes)) != hipSuccess)
         rc = hip_fail("server pool", e);
@@ -373,7 +392,12 @@ int wtc_server_register(wtc_server *s, const wtc_big_graph *g, uint64_t *graph_i
         return hip_fail("server register", e);
     }
     uint32_t *rp = sg->host.as<uint32_t>();
-    for (uint64_t v = 0; v <= sg->n; v++) rp[v] = (uint32_t)g->rowPtr[v];
+    uint64_t W = 0;
+    for (uint64_t v = 0; v <= sg->n; v++) {
+        rp[v] = (uint32_t)g->rowPtr[v];
+        if (v < sg->n) W += binom2(g->rowPtr[v + 1] - g->rowPtr[v]);
+    }
+    sg->cost = std::max<uint64_t>(W, kCostFloor);
     if (sg->m) std::memcpy(rp + sg->n + 1, g->colInd, sg->m * 4);
     std::lock_guard<std::mutex> lk(s->mu);
     sg->id = s->next_gid++;

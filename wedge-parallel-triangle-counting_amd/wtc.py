@@ -71,7 +71,8 @@ class CountStats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("slots", C.c_uint32),
-                ("queue_capacity", C.c_uint32), ("probe_ways", C.c_int32), ("lds_chunk", C.c_uint32)]
+                ("queue_capacity", C.c_uint32), ("probe_ways", C.c_int32), ("lds_chunk", C.c_uint32),
+                ("probe_window", C.c_int32)]
 
 
 class RequestStats(C.Structure):
@@ -129,7 +130,7 @@ _sig = {
     "wtc_ctx_debug_bitmap": (C.c_int, [_P, C.c_uint64, _P]),
     "wtc_debug_decode": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _P]),
     "wtc_debug_copy": (C.c_int, [_P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
-    "wtc_ctx_set_tuning": (C.c_int, [_P, C.c_int, C.c_uint32]),
+    "wtc_ctx_set_tuning": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_int]),
     "wtc_alg_bytes": (None, [_BGP, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "wtc_batch_upload": (C.c_int, [_P, C.POINTER(_BGP), C.c_size_t, C.POINTER(_P)]),
     "wtc_batch_free": (None, [_P]),
@@ -352,10 +353,10 @@ class Context:
         _check(_lib.wtc_ctx_debug_prefix(self._p, n, ws.ctypes.data, tree.ctypes.data), "debug_prefix")
         return ws, tree
 
-    def set_tuning(self, probe_ways: int = 4, lds_chunk: int = 16):
+    def set_tuning(self, probe_ways: int = 4, lds_chunk: int = 16, probe_window: int = 0):
         """Counting-kernel knobs: probes advanced together per lane, wedges
-        per lane staged in LDS per phase."""
-        _check(_lib.wtc_ctx_set_tuning(self._p, probe_ways, lds_chunk), "set_tuning")
+        per lane staged in LDS per phase, probe window (0, 8 or 16)."""
+        _check(_lib.wtc_ctx_set_tuning(self._p, probe_ways, lds_chunk, probe_window), "set_tuning")
 
     def debug_copy(self, nbytes: int, reps: int = 1) -> float:
         ms = C.c_double()
@@ -499,8 +500,8 @@ class Server:
     """Request loop with continuous batching over device-resident CSRs."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, max_batch: int = 256, slots: int = 2,
-                 queue_capacity: int = 65536, probe_ways: int = 0, lds_chunk: int = 0):
-        cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, queue_capacity, probe_ways, lds_chunk)
+                 queue_capacity: int = 65536, probe_ways: int = 0, lds_chunk: int = 0, probe_window: int = 0):
+        cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, queue_capacity, probe_ways, lds_chunk, probe_window)
         self._p = C.c_void_p()
         _check(_lib.wtc_server_create(device, C.byref(cfg), C.byref(self._p)), "server_create")
 
