@@ -95,6 +95,16 @@ int wtc_preprocess(const wtc_big_graph *original, int preprocess_style, wtc_big_
 /* free_graph, tc.cu:789-793. */
 void wtc_free_graph(wtc_big_graph *g);
 
+/* Binary CSR files (no reference counterpart: the fast-ingestion path).
+ * wtc_write_csr stores g with a caller-chosen 64-bit tag (for example a key
+ * of the text file and flags it was built from) and a checksum, through a
+ * temporary file renamed into place.  wtc_read_csr maps the file, checks
+ * size, checksum and structure (offsets non-decreasing from 0 to m, ids < n)
+ * and returns WTC_ERR_FORMAT on any mismatch, WTC_ERR_IO if it cannot be
+ * opened.  tag may be NULL. */
+int wtc_write_csr(const char *path, const wtc_big_graph *g, uint64_t tag);
+int wtc_read_csr(const char *path, uint64_t *tag, wtc_big_graph **out);
+
 /* 32-bit view for the tc.cu path; fails with WTC_ERR_RANGE if numEdges does
  * not fit 32 bits.  Free with wtc_free_graph32. */
 int wtc_narrow_graph(const wtc_big_graph *g, wtc_graph **out);

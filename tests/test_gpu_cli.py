@@ -34,3 +34,30 @@ def test_cli_table(golden, binary):
             copy, exe, total = (float(x) for x in cols[7:10])
             assert abs(copy + exe - total) < 2e-6
             assert float(cols[10]) >= total - 1e-6
+
+
+def test_cli_csr_cache(golden, tmp_path):
+    """WTC_CSR_CACHE: the first run writes the oriented CSR file, the second
+    loads it; a changed input (new mtime) rebuilds it.  Same table either way."""
+    fx = next(f for f in golden if f["format"] != "mm")
+    src = tmp_path / "g.txt"
+    src.write_bytes(open(fx["path"], "rb").read())
+    cache = tmp_path / "g.csr"
+    env = dict(os.environ, WTC_CSR_CACHE=str(cache))
+    args = [os.path.join(PKG, "tc"), "-e", str(src), "-s", "3", "-a", "32", "-l", "1"]
+    if fx["zero_indexed"]:
+        args.append("-z")
+
+    def run():
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120, env=env)
+        assert r.returncode == 0, r.stderr
+        return [int(x) for x in r.stdout.strip().splitlines()[1].split()[1:6]]
+
+    first = run()
+    assert cache.exists()
+    stamp = cache.stat().st_mtime_ns
+    assert run() == first == [fx["n"], fx["m"], 3, 32, fx["triangles"]]
+    assert cache.stat().st_mtime_ns == stamp  # loaded, not rewritten
+    os.utime(src, ns=(stamp + 10 ** 9, stamp + 10 ** 9))
+    assert run() == first
+    assert cache.stat().st_mtime_ns != stamp  # key changed: rebuilt and rewritten

@@ -167,3 +167,103 @@ def test_cli_usage_errors(tmp_path, binary):
     assert r.returncode == 1 and "unable to open graph file" in r.stderr
     r = _tc("-e", zero, "-s", "5", "-a", "0", binary=binary)
     assert r.returncode == 1 and "zero vertex id detected but -z was not set" in r.stderr
+
+
+# ------------------------------------------------------------- ingestion
+
+def _edge_text(rng, n_pairs, n_ids, one_indexed=True):
+    """Edge list text with mixed separators (spaces, tabs, CRLF, several
+    pairs per line, blank lines) so that the parallel reader's cuts land in
+    every kind of whitespace run."""
+    ids = rng.integers(1 if one_indexed else 0, n_ids, size=2 * n_pairs)
+    seps = [" ", "\t", "  ", "\n", "\r\n", "\n\n", " \t "]
+    pick = rng.integers(0, len(seps), size=2 * n_pairs)
+    out = ["% header line\n# another\n"]
+    out.extend(f"{x}{seps[k]}" for x, k in zip(ids.tolist(), pick.tolist()))
+    return "".join(out)
+
+
+def test_parallel_reader_matches_oracle_on_large_files(tmp_path):
+    rng = np.random.default_rng(12)
+    text = _edge_text(rng, 600_000, 200_000)  # several MiB: one piece per thread
+    assert len(text) > 4 << 20
+    path = _write(tmp_path, "big.txt", text)
+    assert same_csr(wtc.read_graph(path), O.read_graph(path, False, False))
+    # garbage at several depths, including right after a cut-sized prefix
+    for frac in (0.0001, 0.3, 0.5, 0.77, 0.999):
+        k = int(len(text) * frac)
+        k = text.index(" ", k) + 1  # start of a token
+        for junk in ("x", "1.5", "+ 3", "-"):
+            bad = text[:k] + junk + " " + text[k:]
+            p2 = _write(tmp_path, "junk.txt", bad)
+            og = O.read_graph(p2, False, False)
+            assert same_csr(wtc.read_graph(p2), og), (frac, junk)
+
+
+def test_parallel_reader_zero_ids_and_odd_tails(tmp_path):
+    rng = np.random.default_rng(13)
+    text = _edge_text(rng, 400_000, 50_000)
+    k = text.index("\n", len(text) * 2 // 3) + 1
+    with_zero = text[:k] + "0 5\n" + text[k:]
+    p = _write(tmp_path, "z.txt", with_zero)
+    with pytest.raises(wtc.WtcError) as e:
+        wtc.read_graph(p)
+    assert e.value.code == O.read_graph(p, False, False) == -4
+    # the zero id is never read when garbage comes first
+    p = _write(tmp_path, "z2.txt", text[:k] + "stop\n0 5\n" + text[k:])
+    assert same_csr(wtc.read_graph(p), O.read_graph(p, False, False))
+    # zero-indexed file, odd number of tokens: the last one is dropped
+    p = _write(tmp_path, "odd.txt", _edge_text(rng, 300_000, 90_000, one_indexed=False) + "17")
+    assert same_csr(wtc.read_graph(p, zero_indexed=True), O.read_graph(p, False, True))
+
+
+def test_reader_thread_count_independent(tmp_path):
+    rng = np.random.default_rng(14)
+    path = _write(tmp_path, "t.txt", _edge_text(rng, 500_000, 100_000))
+    code = ("import sys; sys.path.insert(0, %r); import wtc, hashlib, numpy as np; "
+            "g = wtc.read_graph(%r); "
+            "print(hashlib.sha256(np.asarray(g.row_ptr).tobytes() + np.asarray(g.col_ind).tobytes()).hexdigest())"
+            % (PKG, path))
+    outs = set()
+    for t in ("1", "3", "8"):
+        env = dict(os.environ, WTC_HOST_THREADS=t)
+        outs.add(subprocess.run(["python3", "-c", code], env=env, capture_output=True, text=True,
+                                check=True).stdout.strip())
+    assert len(outs) == 1
+
+
+def test_csr_file_round_trip_and_corruption(tmp_path):
+    g = wtc.preprocess(wtc.gen_rmat(12, 8, seed=2))
+    path = str(tmp_path / "g.csr")
+    wtc.write_csr(path, g, tag=0xDEADBEEF12345678)
+    h, tag = wtc.read_csr(path)
+    assert same_csr(h, g) and tag == 0xDEADBEEF12345678
+    assert not os.path.exists(path + ".tmp")
+    raw = bytearray(open(path, "rb").read())
+
+    def expect_format(data, name):
+        p = str(tmp_path / name)
+        open(p, "wb").write(bytes(data))
+        with pytest.raises(wtc.WtcError) as e:
+            wtc.read_csr(p)
+        assert e.value.code == -3, name
+
+    flip = bytearray(raw)
+    flip[64 + 8 * (g.n + 1) + 4 * 100] ^= 1  # one bit of colInd
+    expect_format(flip, "flip.csr")
+    expect_format(raw[:-4], "short.csr")
+    expect_format(raw + b"\0\0\0\0", "long.csr")
+    magic = bytearray(raw)
+    magic[0] ^= 0xFF
+    expect_format(magic, "magic.csr")
+    expect_format(raw[:40], "tiny.csr")
+    # a checksum-valid file whose ids run past n is still refused
+    bad = wtc.CSR(g.n, g.m, g.row_ptr.copy(), np.array(g.col_ind).copy())
+    bad.col_ind[-1] = g.n + 5
+    wtc.write_csr(str(tmp_path / "ids.csr"), bad)
+    with pytest.raises(wtc.WtcError) as e:
+        wtc.read_csr(str(tmp_path / "ids.csr"))
+    assert e.value.code == -3
+    with pytest.raises(wtc.WtcError) as e:
+        wtc.read_csr(str(tmp_path / "missing.csr"))
+    assert e.value.code == -2

@@ -157,60 +157,13 @@ static int build_symmetric(uint64_t n_cap, uint64_t np, Get get, wtc_big_graph *
     return WTC_OK;
 }
 
-// ------------------------------------------------------------------ reader
-// The text protocol of read_graph: header lines are skipped with fgets into a
-// 256-byte buffer until one starts with a digit (tc.cu:571-574), Matrix
-// Market reads one more line (the size line, tc.cu:577-579), the first pair
-// is scanned from that buffer and the rest with fscanf("%d %d\n") until a
-// conversion fails (tc.cu:590-620).
-
-namespace {
-struct TextCursor {
-    const char *p;
-    const char *end;
-
-    // fgets(buf, 256, f)
-    bool get_line(char *buf, size_t cap) {
-        if (p >= end) return false;
-        size_t k = 0;
-        while (k + 1 < cap && p < end) {
-            char c = *p++;
-            buf[k++] = c;
-            if (c == '\n') break;
-        }
-        buf[k] = 0;
-        return true;
-    }
-};
-
-inline bool space(char c) { return c == ' ' || (c >= '\t' && c <= '\r'); }
-
-// One %d conversion into a UINT_t (value through long, then narrowed).
-inline bool scan_d(const char *&p, const char *end, uint32_t &out) {
-    while (p < end && space(*p)) p++;
-    bool neg = false;
-    if (p < end && (*p == '+' || *p == '-')) neg = (*p++ == '-');
-    if (p >= end || (unsigned)(*p - '0') > 9u) return false;
-    uint64_t v = 0;
-    bool big = false;
-    for (; p < end && (unsigned)(*p - '0') <= 9u; p++) {
-        if (!big) {
-            v = v * 10u + (uint64_t)(*p - '0');
-            big = v > (uint64_t)INT64_MAX;
-        }
-    }
-    int64_t sv = big ? (neg ? INT64_MIN : INT64_MAX) : (neg ? -(int64_t)v : (int64_t)v);
-    out = (uint32_t)(uint64_t)sv;
-    return true;
+int build_symmetric_tokens(uint64_t n_cap, const uint32_t *tok, uint64_t np, uint32_t shift, wtc_big_graph **out) {
+    return build_symmetric(n_cap, np, [tok, shift](uint64_t i, uint32_t &s, uint32_t &d) {
+        s = tok[2 * i] - shift;
+        d = tok[2 * i + 1] - shift;
+        return s != d;
+    }, out);
 }
-
-// "%d %d\n"
-inline bool scan_pair(const char *&p, const char *end, uint32_t &v, uint32_t &w) {
-    if (!scan_d(p, end, v) || !scan_d(p, end, w)) return false;
-    while (p < end && space(*p)) p++;
-    return true;
-}
-}  // namespace
 
 }  // namespace wtc
 
@@ -234,84 +187,6 @@ const char *wtc_strerror(int s) {
         case WTC_ERR_BUSY: return "server queue full or stopped";
         default: return "unknown error";
     }
-}
-
-int wtc_read_graph(const char *filename, int matrix_market, int zero_indexed, int preprocess_style,
-                   wtc_big_graph **out) {
-    if (!filename || !out) return WTC_ERR_ARG;
-    if (preprocess_style < WTC_PREPROCESS_CPU || preprocess_style > WTC_PREPROCESS_GPU_CONSTRAINED)
-        return WTC_ERR_ARG;
-    *out = nullptr;
-    FILE *f = std::fopen(filename, "rb");
-    if (!f) return WTC_ERR_IO;
-    std::fseek(f, 0, SEEK_END);
-    long sz = std::ftell(f);
-    std::fseek(f, 0, SEEK_SET);
-    if (sz < 0) {
-        std::fclose(f);
-        return WTC_ERR_IO;
-    }
-    char *buf = static_cast<char *>(std::malloc((size_t)sz + 1));
-    if (!buf) {
-        std::fclose(f);
-        return WTC_ERR_NOMEM;
-    }
-    size_t got = std::fread(buf, 1, (size_t)sz, f);
-    std::fclose(f);
-    TextCursor tc{buf, buf + got};
-    char line[256];
-    do {
-        if (!tc.get_line(line, sizeof line)) {
-            std::free(buf);
-            return WTC_ERR_FORMAT;
-        }
-    } while (line[0] < '0' || line[0] > '9');
-    if (matrix_market && !tc.get_line(line, sizeof line)) {
-        std::free(buf);
-        return WTC_ERR_FORMAT;
-    }
-    std::vector<uint32_t> pairs;
-    pairs.reserve((size_t)std::min<long>(sz / 4, 1L << 28));
-    uint32_t max_vertex = 0;
-    const uint32_t shift = zero_indexed ? 0u : 1u;
-    int rc = WTC_OK;
-    auto take = [&](uint32_t v, uint32_t w) -> bool {
-        if (!zero_indexed && (v == 0 || w == 0)) {
-            rc = WTC_ERR_ZERO_ID;
-            return false;
-        }
-        v -= shift;
-        w -= shift;
-        if (v != w) {
-            max_vertex = std::max(max_vertex, std::max(v, w));
-            pairs.push_back(v);
-            pairs.push_back(w);
-        }
-        return true;
-    };
-    const char *lp = line, *le = line + std::strlen(line);
-    uint32_t v, w;
-    try {
-        if (scan_pair(lp, le, v, w) && take(v, w))
-            while (scan_pair(tc.p, tc.end, v, w) && take(v, w)) {
-            }
-    } catch (const std::bad_alloc &) {
-        rc = WTC_ERR_NOMEM;
-    }
-    std::free(buf);
-    if (rc) return rc;
-    if ((uint64_t)max_vertex + 1 > 0xffffffffULL) return WTC_ERR_FORMAT;
-    // -p 1/2: the sort and deduplication run on the GPU (tc.cu:628-644)
-    if (preprocess_style != WTC_PREPROCESS_CPU)
-        return wtc_gpu_symmetric_default(pairs.data(), pairs.size() / 2, max_vertex + 1, out);
-    const uint32_t *pp = pairs.data();
-    return build_symmetric((uint64_t)max_vertex + 1, pairs.size() / 2,
-                           [pp](uint64_t i, uint32_t &s, uint32_t &d) {
-                               s = pp[2 * i];
-                               d = pp[2 * i + 1];
-                               return true;
-                           },
-                           out);
 }
 
 int wtc_graph_from_pairs(const uint32_t *pairs, uint64_t npairs, wtc_big_graph **out) {

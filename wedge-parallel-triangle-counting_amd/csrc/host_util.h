@@ -74,6 +74,13 @@ int validate_oriented_rows(uint64_t n, uint64_t m, const uint64_t *rowPtr, const
 
 }  // namespace wtc
 
+namespace wtc {
+// Symmetric, deduplicated CSR of the np pairs (tok[2i] - shift,
+// tok[2i+1] - shift), self-loops skipped; ids must be < n_cap.
+int build_symmetric_tokens(uint64_t n_cap, const uint32_t *tok, uint64_t np, uint32_t shift,
+                           wtc_big_graph **out);
+}  // namespace wtc
+
 // GPU builders of the -p 1/2 styles (wtc_prep.hip), on the process-wide
 // context: symmetric CSR of loop-free pairs with ids < n, and the degree
 // orientation of a symmetric CSR.  Same results as the CPU builders.

@@ -2,6 +2,7 @@
 // WTC_BIG=1).  Same flags, usage exit and results table as the reference's
 // main (tc.cu:816-918; 64-bit table of experiments/tc_big_graphs.cu:874-891);
 // the counting itself goes through the library's C ABI.
+#include <sys/stat.h>
 #include <sys/time.h>
 
 #include <cstdio>
@@ -17,6 +18,23 @@ static double wall_seconds() {
     timeval tv;
     gettimeofday(&tv, nullptr);
     return (double)tv.tv_sec + (double)tv.tv_usec / 1e6;
+}
+
+static uint64_t mix(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ULL;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebULL;
+    return x ^ (x >> 31);
+}
+
+// Key of the oriented CSR a run builds: the text file's size and mtime and
+// the -m / -z flags (the -p style does not change the CSR).  0 if the file
+// cannot be stat'ed.
+static uint64_t source_tag(const char *file, int mm, int zero) {
+    struct stat st;
+    if (stat(file, &st) != 0) return 0;
+    const uint64_t ns = (uint64_t)st.st_mtim.tv_sec * 1000000000ull + (uint64_t)st.st_mtim.tv_nsec;
+    return mix(mix((uint64_t)st.st_size) ^ ns) ^ (uint64_t)(mm * 2 + zero + 1);
 }
 
 [[noreturn]] static void usage() {
@@ -101,25 +119,45 @@ int main(int argc, char **argv) {
     }
     if (graph_filename == nullptr || spread == 0) usage();
 
-    wtc_big_graph *original = nullptr;
-    int rc = wtc_read_graph(graph_filename, graph_mm, graph_zero_indexed, preprocess_style, &original);
-    if (rc == WTC_ERR_IO) {
-        std::fprintf(stderr, "ERROR: unable to open graph file.\n");
-        usage();
-    }
-    if (rc == WTC_ERR_ZERO_ID) {
-        std::fprintf(stderr, "ERROR: zero vertex id detected but -z was not set.\n");
-        usage();
-    }
-    if (rc == WTC_ERR_FORMAT) usage();
-    if (rc) die(rc);
-
-    double t_preprocessing = wall_seconds();
+    // WTC_CSR_CACHE=<file>: the oriented CSR is loaded from that binary file
+    // when its key matches the input (the prepro column is then the load
+    // time); otherwise it is built as usual and written there.
+    const char *cache = std::getenv("WTC_CSR_CACHE");
+    const uint64_t tag = cache && *cache ? source_tag(graph_filename, graph_mm, graph_zero_indexed) : 0;
     wtc_big_graph *graph = nullptr;
-    rc = wtc_preprocess(original, preprocess_style, &graph);
-    t_preprocessing = wall_seconds() - t_preprocessing;
-    wtc_free_graph(original);
-    if (rc) die(rc);
+    double t_preprocessing = wall_seconds();
+    int rc = WTC_OK;
+    if (tag) {
+        uint64_t got = 0;
+        if (wtc_read_csr(cache, &got, &graph) == WTC_OK && got != tag) {
+            wtc_free_graph(graph);
+            graph = nullptr;
+        }
+    }
+    if (graph) {
+        t_preprocessing = wall_seconds() - t_preprocessing;
+    } else {
+        wtc_big_graph *original = nullptr;
+        rc = wtc_read_graph(graph_filename, graph_mm, graph_zero_indexed, preprocess_style, &original);
+        if (rc == WTC_ERR_IO) {
+            std::fprintf(stderr, "ERROR: unable to open graph file.\n");
+            usage();
+        }
+        if (rc == WTC_ERR_ZERO_ID) {
+            std::fprintf(stderr, "ERROR: zero vertex id detected but -z was not set.\n");
+            usage();
+        }
+        if (rc == WTC_ERR_FORMAT) usage();
+        if (rc) die(rc);
+
+        t_preprocessing = wall_seconds();
+        rc = wtc_preprocess(original, preprocess_style, &graph);
+        t_preprocessing = wall_seconds() - t_preprocessing;
+        wtc_free_graph(original);
+        if (rc) die(rc);
+        if (tag && wtc_write_csr(cache, graph, tag) != WTC_OK)
+            std::fprintf(stderr, "WARNING: could not write the CSR cache %s\n", cache);
+    }
 
 #if !WTC_BIG
     wtc_graph *graph32 = nullptr;

@@ -63,12 +63,6 @@ hipError_t DevBuf::ensure(size_t bytes) {
     return e;
 }
 
-void DevBuf::release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-}
-
 hipError_t PinBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return hipSuccess;
     if (p) {
@@ -81,12 +75,6 @@ hipError_t PinBuf::ensure(size_t bytes) {
     if (e == hipSuccess) cap = want;
     else p = nullptr;
     return e;
-}
-
-void PinBuf::release() {
-    if (p) (void)hipHostFree(p);
-    p = nullptr;
-    cap = 0;
 }
 
 int Slot::create() {
@@ -684,6 +672,8 @@ void wtc_batch_free(wtc_batch *b) {
     delete b;
 }
 
+}  // extern "C"
+
 namespace {
 
 // Per-graph spread and bitmap length of a batch count (autotuned where 0).
@@ -755,6 +745,8 @@ int batch_capture(wtc_ctx *c, wtc_batch *b, const std::vector<uint32_t> &s, cons
 }
 
 }  // namespace
+
+extern "C" {
 
 int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint32_t *a, uint64_t *triangles,
                     wtc_count_stats *st) {

@@ -26,28 +26,55 @@ void set_error(const std::string &msg);
 // Readable bytes every device buffer carries past its requested size.
 constexpr size_t kDevTailSlack = 256;
 
-// Grow-only device buffer.
-struct DevBuf {
+// Owning, move-only buffer: freed by release() or on destruction.
+template <class Free>
+struct OwnedBuf {
     void *p = nullptr;
     size_t cap = 0;
-    hipError_t ensure(size_t bytes);
-    void release();
+    OwnedBuf() = default;
+    OwnedBuf(const OwnedBuf &) = delete;
+    OwnedBuf &operator=(const OwnedBuf &) = delete;
+    OwnedBuf(OwnedBuf &&o) noexcept : p(o.p), cap(o.cap) {
+        o.p = nullptr;
+        o.cap = 0;
+    }
+    OwnedBuf &operator=(OwnedBuf &&o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            cap = o.cap;
+            o.p = nullptr;
+            o.cap = 0;
+        }
+        return *this;
+    }
+    ~OwnedBuf() { release(); }
+    void release() {
+        if (p) Free::free(p);
+        p = nullptr;
+        cap = 0;
+    }
     template <class T>
     T *as() const {
         return static_cast<T *>(p);
     }
 };
 
-// Grow-only pinned host buffer.
-struct PinBuf {
-    void *p = nullptr;
-    size_t cap = 0;
+struct DeviceFree {
+    static void free(void *p) { (void)hipFree(p); }
+};
+struct PinnedFree {
+    static void free(void *p) { (void)hipHostFree(p); }
+};
+
+// Grow-only device buffer.
+struct DevBuf : OwnedBuf<DeviceFree> {
     hipError_t ensure(size_t bytes);
-    void release();
-    template <class T>
-    T *as() const {
-        return static_cast<T *>(p);
-    }
+};
+
+// Grow-only pinned host buffer.
+struct PinBuf : OwnedBuf<PinnedFree> {
+    hipError_t ensure(size_t bytes);
 };
 
 // One asynchronous execution slot: a stream, timing events, the device

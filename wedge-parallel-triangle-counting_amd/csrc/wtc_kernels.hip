@@ -373,6 +373,10 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
             uint32_t v = 0, d = 0, w_i = 0, u_i = 0;
             OffT vb = 0;
             uint64_t tile_cnt = 0;
+            // colInd position and value of the last w loaded: consecutive
+            // wedges of a lane mostly share w, whose id is then not reloaded
+            OffT last_wpos = ~(OffT)0;
+            uint32_t last_wv = 0;
 
             for (uint64_t c = 0; c < s; c += sc) {  // 64-bit: s may be close to 2^32
                 const uint32_t kn = (s - c) < sc ? (uint32_t)(s - c) : sc;
@@ -412,11 +416,24 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                             i++;
                         }
                     }
-                    uint32_t wv[4], uv[4];
+                    uint32_t wv[4], uv[4], wl[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {  // ok[] is a prefix of the group
+                        const OffT prev = j == 0 ? last_wpos : wpos[j - 1];
+                        wl[j] = ok[j] && wpos[j] != prev ? colInd[wpos[j]] : 0u;
+                        uv[j] = ok[j] ? colInd[upos[j]] : 0u;
+                    }
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        wv[j] = ok[j] ? colInd[wpos[j]] : kSentinel;
-                        uv[j] = ok[j] ? colInd[upos[j]] : 0u;
+                        if (ok[j]) {
+                            if (wpos[j] != last_wpos) {
+                                last_wv = wl[j];
+                                last_wpos = wpos[j];
+                            }
+                            wv[j] = last_wv;
+                        } else {
+                            wv[j] = kSentinel;
+                        }
                     }
 #pragma unroll
                     for (int j = 0; j < 4; j++) {

@@ -96,6 +96,8 @@ _sig = {
     "wtc_last_error_detail": (C.c_char_p, []),
     "wtc_read_graph": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.POINTER(_BGP)]),
     "wtc_graph_from_pairs": (C.c_int, [_P, C.c_uint64, C.POINTER(_BGP)]),
+    "wtc_write_csr": (C.c_int, [C.c_char_p, _BGP, C.c_uint64]),
+    "wtc_read_csr": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(_BGP)]),
     "wtc_preprocess": (C.c_int, [_BGP, C.c_int, C.POINTER(_BGP)]),
     "wtc_free_graph": (None, [_BGP]),
     "wtc_narrow_graph": (C.c_int, [_BGP, C.POINTER(C.POINTER(_Graph32))]),
@@ -234,6 +236,21 @@ def read_graph(path: str, matrix_market: bool = False, zero_indexed: bool = Fals
     _check(_lib.wtc_read_graph(path.encode(), int(matrix_market), int(zero_indexed), preprocess_style,
                                C.byref(out)), "read_graph")
     return _adopt(out)
+
+
+def write_csr(path: str, g: CSR, tag: int = 0):
+    """Binary CSR file with a caller tag and a checksum."""
+    s = g._struct()
+    _check(_lib.wtc_write_csr(path.encode(), C.byref(s), tag), "write_csr")
+
+
+def read_csr(path: str):
+    """(CSR, tag) from a binary CSR file; WtcError on any size, checksum or
+    structure mismatch."""
+    out = _BGP()
+    tag = C.c_uint64(0)
+    _check(_lib.wtc_read_csr(path.encode(), C.byref(tag), C.byref(out)), "read_csr")
+    return _adopt(out), int(tag.value)
 
 
 def graph_from_pairs(pairs: np.ndarray) -> CSR:
