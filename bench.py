@@ -110,7 +110,7 @@ def batch_bench(ctx, k, base_seed, check_sample, threads):
     t0 = time.perf_counter()
     graphs = batch_graphs(k, base_seed)
     tgen = time.perf_counter() - t0
-    tune = [wtc.autotune(g.n, g.m, g.wedges()) for g in graphs]
+    tune = [wtc.autotune(g.n, g.m, g.wedges(), batch=True) for g in graphs]
     spread = [s for s, _ in tune]
     alen = [a for _, a in tune]
     log(f"batch: {k} graphs, {sum(g.m for g in graphs)} edges, {sum(g.wedges() for g in graphs)} wedges, "
@@ -149,13 +149,15 @@ def batch_bench(ctx, k, base_seed, check_sample, threads):
             "sample_size": int(min(check_sample, k)) if check_sample else 0}
 
 
-def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, spread, a, ways, chunk):
+def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, window):
     """Serving workload: Poisson arrivals of count requests on seeded R-MAT
-    graphs (scale uniform in [10, 22], edge factor 16).  Graphs are registered
-    once and stay resident; requests are admitted by HBM footprint and
-    batched continuously.  Per arrival rate: p50/p99 latency, completed
-    graphs/s, rejected requests, and exactness against each graph's count
-    served alone."""
+    graphs (scale uniform in [10, 22], edge factor 16), each with the
+    autotuner's spread and bitmap length.  Graphs are registered once and stay
+    resident; requests are admitted by HBM footprint and batched continuously.
+    Per arrival rate: p50/p99 latency, completed graphs/s, rejected requests,
+    and exactness against each graph's count served alone.  The plain path on
+    the same mix is one reference-style tc_GPU call per request (alloc, copy,
+    count, copy back, free)."""
     import threading
     from concurrent.futures import ThreadPoolExecutor
 
@@ -164,12 +166,23 @@ def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, spread, a, ways,
     with ThreadPoolExecutor(max_workers=4) as ex:
         graphs = list(ex.map(lambda key: wtc.preprocess(wtc.gen_rmat(key[0], 16, seed=7000 + 31 * key[0] + key[1])),
                              keys))
-    srv = wtc.Server(0, hbm_budget_bytes=16 << 30, max_batch=256, slots=2, queue_capacity=queue_capacity,
-                     probe_ways=ways, lds_chunk=chunk)
+    tune = [wtc.autotune(g.n, g.m, g.wedges()) for g in graphs]
+    # plain path on the same mix: mean time of one reference-style call per graph
+    plain_ms = []
+    for g, (s_g, a_g) in zip(graphs, tune):
+        wtc.tc_gpu_big(g, s_g, a_g)
+        t0 = time.perf_counter()
+        for _ in range(2):
+            wtc.tc_gpu_big(g, s_g, a_g)
+        plain_ms.append((time.perf_counter() - t0) / 2 * 1e3)
+    plain_gps = 1e3 / float(np.mean(plain_ms))
+    log(f"serve: plain path {plain_gps:.1f} graphs/s on the uniform mix")
+    srv = wtc.Server(0, hbm_budget_bytes=32 << 30, max_batch=256, slots=2, queue_capacity=queue_capacity,
+                     probe_ways=ways, lds_chunk=chunk, probe_window=window if window else -1)
     gids = [srv.register(g) for g in graphs]
     alone, alone_ms = [], []
-    for gid in gids:  # each graph alone (also makes it resident)
-        cnt, st = srv.wait(srv.submit(gid, spread, a))
+    for gid, (s_g, a_g) in zip(gids, tune):  # each graph alone (also makes it resident)
+        cnt, st = srv.wait(srv.submit(gid, s_g, a_g))
         alone.append(cnt)
         alone_ms.append(st.service_ms)
     log(f"serve: {len(graphs)} graphs resident, alone service ms by scale: "
@@ -193,7 +206,7 @@ def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, spread, a, ways,
                         break
                     time.sleep(left - 2e-4 if left > 1e-3 else 0)
                 try:
-                    rids[i] = srv.submit(gids[pick[i]], spread, a)
+                    rids[i] = srv.submit(gids[pick[i]], *tune[pick[i]])
                 except wtc.WtcError:
                     rids[i] = -1  # rejected: queue full
 
@@ -221,8 +234,58 @@ def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, spread, a, ways,
         results.append(rec)
     stats = srv.stats()
     srv.close()
+    best = max(r["graphs_per_s"] for r in results) if results else 0.0
     return {"graphs": len(graphs), "scales": [scales[0], scales[-1]], "seconds_per_rate": seconds,
-            "queue_capacity": queue_capacity, "spread": spread, "a": a, "rates": results, "server": stats}
+            "queue_capacity": queue_capacity, "params": "autotuned per graph", "rates": results,
+            "plain_graphs_per_s": plain_gps, "served_max_graphs_per_s": best,
+            "served_over_plain": best / plain_gps, "server": stats}
+
+
+def ingest_bench(scale, seed):
+    """Ingestion: an R-MAT edge list as text -> read_graph (the parallel
+    reader, with every host thread and with one) -> preprocess -> binary CSR
+    file written and read back.  Host only."""
+    import subprocess
+    import tempfile
+
+    import pandas as pd
+
+    pairs = wtc.gen_rmat_pairs(scale, 16, seed=seed)
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        txt = os.path.join(d, "g.txt")
+        with open(txt, "w") as f:
+            f.write("% R-MAT edge list\n")
+        pd.DataFrame(pairs).to_csv(txt, sep=" ", header=False, index=False, mode="a")
+        size = os.path.getsize(txt)
+        threads = len(os.sched_getaffinity(0))
+        t0 = time.perf_counter()
+        g = wtc.read_graph(txt, zero_indexed=True)
+        t_read = time.perf_counter() - t0
+        code = ("import sys, time; sys.path.insert(0, %r); import wtc; t = time.perf_counter(); "
+                "g = wtc.read_graph(%r, zero_indexed=True); print(time.perf_counter() - t, g.m)" % (PKG, txt))
+        r = subprocess.run([sys.executable, "-c", code], env=dict(os.environ, WTC_HOST_THREADS="1"),
+                           capture_output=True, text=True, check=True)
+        t_read1, m1 = r.stdout.split()
+        assert int(m1) == g.m
+        t0 = time.perf_counter()
+        og = wtc.preprocess(g)
+        t_pre = time.perf_counter() - t0
+        csr = os.path.join(d, "g.csr")
+        t0 = time.perf_counter()
+        wtc.write_csr(csr, og, tag=1)
+        t_w = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        back, tag = wtc.read_csr(csr)
+        t_r = time.perf_counter() - t0
+        same = tag == 1 and back.m == og.m and np.array_equal(back.col_ind, og.col_ind) and \
+            np.array_equal(back.row_ptr, og.row_ptr)
+        csr_size = os.path.getsize(csr)
+    return {"workload": f"rmat{scale}-ef16 edge list", "text_bytes": size, "pairs": int(len(pairs)),
+            "host_threads": threads, "read_graph_s": t_read, "read_graph_1thread_s": float(t_read1),
+            "read_speedup": float(t_read1) / t_read, "text_MB_per_s": size / t_read / 1e6,
+            "preprocess_s": t_pre, "csr_bytes": csr_size, "write_csr_s": t_w, "read_csr_s": t_r,
+            "read_csr_GB_per_s": csr_size / t_r / 1e9, "csr_round_trip_equal": bool(same),
+            "text_to_oriented_s": t_read + t_pre}
 
 
 def _rocprof_rows(directory, pattern):
@@ -300,10 +363,14 @@ def main():
     ap.add_argument("--serve-only", action="store_true", help="run only the serving (Poisson arrivals) workload")
     ap.add_argument("--serve-rates", default="100,250,500,1000,2500,5000")
     ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--ingest-only", action="store_true", help="run only the text/CSR-file ingestion timing")
     args = ap.parse_args()
 
     if args.pmc_summary:
         pmc_summary(args)
+        return
+    if args.ingest_only:
+        print(json.dumps(ingest_bench(args.scale, args.seed)), flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -321,7 +388,7 @@ def main():
 
     if args.serve_only:
         res = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 2, 512,
-                          args.spread, args.a, args.ways, args.chunk)
+                          args.ways, args.chunk, args.window)
         print(json.dumps(res), flush=True)
         return
     if args.batch_only:

@@ -296,8 +296,13 @@ int wtc_server_get_stats(wtc_server *srv, wtc_server_stats *st);
 /* Finishes queued and running requests, then frees everything. */
 void wtc_server_destroy(wtc_server *srv);
 
-/* The autotuner's choice for a graph of n vertices, m edges and W wedges. */
+/* The autotuner's (spread, a) for a graph of n vertices, m oriented edges and
+ * W wedges, from per-size-bucket tables measured on MI355X: wtc_autotune for
+ * a graph counted alone, wtc_autotune_batch for a member of a batch (where
+ * many graphs share one launch).  A bitmap (a > 0) is chosen only for
+ * degree-skewed graphs (W/m >= 24). */
 void wtc_autotune(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *adjacency_matrix_len);
+void wtc_autotune_batch(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *adjacency_matrix_len);
 
 #ifdef __cplusplus
 }

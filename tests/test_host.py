@@ -143,9 +143,24 @@ def test_validate_rejects_bad_csr():
 
 
 def test_autotune_table():
+    # the measured buckets: alone, R-MAT-like skew (W/m >= 24) gets a bitmap
+    # of about n/8 top ranks once W >= 2^26, capped at 131072 / 262144
+    assert wtc.autotune(4083, 48254, 780919) == (2, 0)
+    assert wtc.autotune(65482, 909788, 33312005) == (8, 2048)
+    assert wtc.autotune(262089, 3805718, 204948349) == (12, 32768)
+    assert wtc.autotune(1048465, 15701962, 1226291937) == (12, 131072)
+    assert wtc.autotune(4193857, 64156969, 7204823400) == (12, 131072)
+    assert wtc.autotune(16775427, 260385033, 41737375097) == (12, 262144)
+    # low skew (ER/BA): no bitmap; batch members use larger tiles
+    assert wtc.autotune(40000, 600000, 4000000)[1] == 0
+    assert wtc.autotune(100, 300, 900, batch=True) == (4, 0)
+    assert wtc.autotune(200, 600, 1800, batch=True) == (8, 0)
+    assert wtc.autotune(30000, 500000, 5000000, batch=True) == (12, 0)
+    assert wtc.autotune(10, 0, 0) == (1, 0)
     for n in (64, 1000, 5000, 65536, 10 ** 6):
-        s, a = wtc.autotune(n, 0, 0)
-        assert s >= 1 and a % 32 == 0
+        for batch in (False, True):
+            s, a = wtc.autotune(n, 20 * n, 600 * n, batch=batch)
+            assert s >= 1 and a % 32 == 0
 
 
 # ------------------------------------------------------------------- CLI

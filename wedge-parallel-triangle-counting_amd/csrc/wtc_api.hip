@@ -22,6 +22,7 @@ struct wtc_batch {
     int ob;
     std::vector<uint32_t> n;
     std::vector<uint64_t> m, row_off, col_off;  // element offsets into rowPtr / colInd
+    std::vector<uint64_t> W;                    // wedges per graph (for the autotuner)
     DevBuf rowPtr, colInd;
     // Replays: the batch's own stream and workspace (a captured graph keeps
     // their addresses) and the graph of the last (spread, a, tuning) counted.
@@ -575,17 +576,56 @@ int wtc_debug_copy(wtc_ctx *c, uint64_t bytes, int reps, double *ms) {
 
 // ------------------------------------------------------------------ batches
 
-void wtc_autotune(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *a) {
-    (void)m;
-    uint32_t s, aa;
-    if (wedges < (1u << 16)) s = 2;
-    else if (wedges < (1u << 22)) s = 4;
-    else s = 8;
-    if (n <= 1024) aa = (n + 31) & ~31u;
-    else if (n <= 16384) aa = 1024;
-    else aa = 2048;
+}  // extern "C"
+
+namespace {
+
+// Per-size-bucket choices measured on MI355X (device time of prefix, bitmap
+// and count; best of a sweep of s in {1..16} and a in {0, 256 .. 2^20}):
+//  * R-MAT graphs counted alone (scales 12-24): s = 2 / 4 / 8 / 12 for W below
+//    2^20 / 2^23 / 2^26 / above; a = 0 below 2^23 wedges, 2048 below 2^26,
+//    then about n/8 capped at 131072 (n < 2^23) or 262144.
+//  * Erdos-Renyi / Barabasi-Albert graphs in a batch of 4096 (n 64..65536,
+//    degree 4..64): a = 0 in every size bucket; s = 4 / 8 / 12 for n below
+//    128 / 256 / above.
+// What separates the two families is degree skew, read as wedges per edge:
+// the bitmap over the top ranks only pays where a few vertices carry most of
+// the wedges (R-MAT: 16 at scale 12 rising to 160 at scale 24; ER/BA: below
+// 20 here).
+constexpr uint64_t kSkewWedgesPerEdge = 24;
+
+uint32_t skewed_bitmap_len(uint32_t n) {
+    const uint32_t cap = n < (1u << 23) ? 131072u : 262144u;
+    const uint32_t a = ((n / 8) + 31u) & ~31u;
+    return a < cap ? a : cap;
+}
+
+void autotune_params(uint32_t n, uint64_t m, uint64_t wedges, bool in_batch, uint32_t *spread, uint32_t *a) {
+    uint32_t s, aa = 0;
+    const bool skewed = m > 0 && wedges / m >= kSkewWedgesPerEdge;
+    if (wedges == 0) {
+        s = 1;
+    } else if (in_batch) {  // many graphs share the launch: parallelism is not scarce
+        s = n < 128 ? 4 : n < 256 ? 8 : 12;
+        if (skewed && wedges >= (1u << 23)) aa = wedges < (1u << 26) ? 2048u : skewed_bitmap_len(n);
+    } else {  // alone: small graphs need small tiles to fill the GPU
+        s = wedges < (1u << 20) ? 2 : wedges < (1u << 23) ? 4 : wedges < (1u << 26) ? 8 : 12;
+        if (skewed && wedges >= (1u << 23)) aa = wedges < (1u << 26) ? 2048u : skewed_bitmap_len(n);
+    }
     if (spread) *spread = s;
     if (a) *a = aa;
+}
+
+}  // namespace
+
+extern "C" {
+
+void wtc_autotune(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *a) {
+    autotune_params(n, m, wedges, false, spread, a);
+}
+
+void wtc_autotune_batch(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *a) {
+    autotune_params(n, m, wedges, true, spread, a);
 }
 
 int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, wtc_batch **out) {
@@ -599,6 +639,7 @@ int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, w
     b->m.resize(k);
     b->row_off.resize(k);
     b->col_off.resize(k);
+    b->W.resize(k);
     uint64_t R = 0, M = 0, max_m = 0;
     for (size_t g = 0; g < k; g++) {
         if (!graphs[g] || graphs[g]->numVertices == 0) {
@@ -615,8 +656,13 @@ int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, w
     }
     std::atomic<int> bad{0};
     par_chunks(k, 1, [&](uint64_t g0, uint64_t g1, int) {
-        for (uint64_t g = g0; g < g1; g++)
+        for (uint64_t g = g0; g < g1; g++) {
             if (validate_oriented_serial(graphs[g]) != WTC_OK) bad = 1;
+            uint64_t W = 0;
+            for (uint64_t v = 0; v < graphs[g]->numVertices; v++)
+                W += binom2(graphs[g]->rowPtr[v + 1] - graphs[g]->rowPtr[v]);
+            b->W[g] = W;
+        }
     });
     if (bad) {
         delete b;
@@ -683,7 +729,7 @@ int batch_params(const wtc_batch *b, const uint32_t *spread, const uint32_t *a, 
     a_out.resize(b->G);
     for (uint32_t g = 0; g < b->G; g++) {
         uint32_t s = spread ? spread[g] : 0, aa = a ? a[g] : 0;
-        if (s == 0) wtc_autotune(b->n[g], b->m[g], 0, &s, a ? nullptr : &aa);
+        if (s == 0) wtc_autotune_batch(b->n[g], b->m[g], b->W[g], &s, a ? nullptr : &aa);
         if (s == 0 || aa % 32 != 0) return WTC_ERR_ARG;
         s_out[g] = s;
         a_out[g] = aa;

@@ -195,10 +195,21 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_build(const GraphDesc *__rest
 
 // ------------------------------------------------------------------- count
 
+// Global-memory views of the CSR, prefix and bitmap arrays.  Their pointers
+// are read from descriptors in memory, so without the cast the compiler must
+// treat them as generic addresses: flat loads, which also count against the
+// LDS counter and make every wait cover the LDS traffic too.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(const T *p) {
+    return (gptr<T>)p;
+}
+
 // Rightmost v in [lo, hi] with ws[v] <= t, given ws[lo] <= t.  Called with
 // wave-uniform arguments (scalar loads) and per lane (vector loads).
-__device__ __forceinline__ uint32_t rightmost_le(const uint64_t *__restrict__ ws, uint32_t lo, uint32_t hi,
-                                                 uint64_t t) {
+template <class P>
+__device__ __forceinline__ uint32_t rightmost_le(P ws, uint32_t lo, uint32_t hi, uint64_t t) {
     uint32_t s = lo + 1, e = hi + 1;
     while (s < e) {
         const uint32_t mid = s + ((e - s) >> 1);
@@ -211,8 +222,8 @@ __device__ __forceinline__ uint32_t rightmost_le(const uint64_t *__restrict__ ws
 // Wave-uniform search for the tile's first vertex over [0, n) of a single
 // graph: 12 levels from the compact search tree, then wedgeSum
 // (tc.cu:139-170 semantics).
-__device__ __forceinline__ uint32_t tree_search(const uint64_t *__restrict__ tree,
-                                                const uint64_t *__restrict__ ws, uint32_t n, uint64_t t) {
+template <class P>
+__device__ __forceinline__ uint32_t tree_search(const uint64_t *__restrict__ tree, P ws, uint32_t n, uint64_t t) {
     uint32_t gs = 0, ge = n, ci = 0;
 #pragma unroll
     for (int l = 0; l < kTreeLevels; l++) {
@@ -232,8 +243,8 @@ __device__ __forceinline__ uint32_t tree_search(const uint64_t *__restrict__ tre
 }
 
 // Wave-uniform galloping search from a known lower bound.
-__device__ __forceinline__ uint32_t gallop_le(const uint64_t *__restrict__ ws, uint32_t lo, uint32_t vmax,
-                                              uint64_t t) {
+template <class P>
+__device__ __forceinline__ uint32_t gallop_le(P ws, uint32_t lo, uint32_t vmax, uint64_t t) {
     uint64_t step = 1;
     for (;;) {
         const uint64_t p = (uint64_t)lo + step;
@@ -277,10 +288,11 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
 // Tiles a wave takes per grab from its XCD's work counter.
 constexpr uint64_t kTilesPerGrab = 8;
 
+// Four ids loaded by one 16-byte vector load.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // Elements between the 16-byte aligned address at or below p and p.
-__device__ __forceinline__ uint32_t win_skew(const uint32_t *p) {
-    return (uint32_t)((reinterpret_cast<uintptr_t>(p) >> 2) & 3u);
-}
+__device__ __forceinline__ uint32_t win_skew(gptr<uint32_t> p) { return (uint32_t)(((uintptr_t)p >> 2) & 3u); }
 
 // kWays: independent probes each lane advances together.  kWin: 0 for a plain
 // binary search over adj+(w); 8 or 16 to stop the search once the remaining
@@ -324,10 +336,10 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
         uint64_t g_tile_end = 0;
         bool have_prev = false;
         // per-graph state, wave-uniform
-        const OffT *__restrict__ rowPtr = nullptr;
-        const uint32_t *__restrict__ colInd = nullptr;
-        const uint64_t *__restrict__ ws = nullptr;
-        const uint32_t *__restrict__ bitmap = nullptr;
+        gptr<OffT> rowPtr = nullptr;
+        gptr<uint32_t> colInd = nullptr;
+        gptr<uint64_t> ws = nullptr;
+        gptr<uint32_t> bitmap = nullptr;
         uint32_t s = 1, sc = 1, pad = 1, gn = 1, thr = 0;
         uint64_t wbase = 0, wend = 0, tile_base = 0, bm_size = 0;
 
@@ -335,10 +347,10 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
             if (g == kSentinel || t >= g_tile_end) {
                 g = G > 1 ? graph_of(desc, G, t, &GraphDesc::tile_base) : 0u;
                 const GraphDesc &D = desc[g];
-                rowPtr = static_cast<const OffT *>(D.rowPtr);
-                colInd = D.colInd;
-                ws = D.ws;
-                bitmap = D.bitmap;
+                rowPtr = to_global(static_cast<const OffT *>(D.rowPtr));
+                colInd = to_global(D.colInd);
+                ws = to_global(static_cast<const uint64_t *>(D.ws));
+                bitmap = to_global(static_cast<const uint32_t *>(D.bitmap));
                 s = D.spread;
                 sc = s < chunk ? s : chunk;  // wedges per lane staged at once
                 pad = sc | 1u;
@@ -360,19 +372,25 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
             // wave-uniform vertex range [v_lo, v_hi] of the tile
             const uint32_t v_lo = have_prev ? gallop_le(ws, v_prev, vmax, tile_start)
                                   : (tree != nullptr) ? tree_search(tree, ws, gn, tile_start)
-                                                      : rightmost_le(ws, 0, vmax, tile_start);
+                                                      : rightmost_le(ws, 0u, vmax, tile_start);
             const uint32_t v_hi = gallop_le(ws, v_lo, vmax, tile_end - 1);
             v_prev = v_hi;
             have_prev = true;
 
-            // lane state: wedges [first, lend)
+            // lane state: wedges [first, lend); the first one is located by a
+            // search in [v_lo, v_hi] and decoded exactly, the rest by stepping
             const uint64_t first = tile_start + (uint64_t)lane * s;
             const uint64_t lend = first + s < tile_end ? first + s : tile_end;
             uint64_t i = first;
-            bool started = false;
             uint32_t v = 0, d = 0, w_i = 0, u_i = 0;
             OffT vb = 0;
-            uint64_t tile_cnt = 0;
+            if (first < lend) {
+                v = rightmost_le(ws, v_lo, v_hi, first);
+                vb = rowPtr[v];
+                d = (uint32_t)(rowPtr[v + 1] - vb);
+                decode_wedge(d, first - ws[v], w_i, u_i);
+            }
+            uint32_t hc = 0;  // this lane's hits in the tile (a probe matches at most once)
             // colInd position and value of the last w loaded: consecutive
             // wedges of a lane mostly share w, whose id is then not reloaded
             OffT last_wpos = ~(OffT)0;
@@ -391,13 +409,10 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                         wpos[j] = 0;
                         upos[j] = 0;
                         if (ok[j]) {
-                            if (!started) {
-                                started = true;
-                                v = rightmost_le(ws, v_lo, v_hi, i);
-                                vb = rowPtr[v];
-                                d = (uint32_t)(rowPtr[v + 1] - vb);
-                                decode_wedge(d, i - ws[v], w_i, u_i);
-                            } else {
+                            wpos[j] = vb + w_i;
+                            upos[j] = vb + u_i;
+                            i++;
+                            if (i < lend) {  // step to the lane's next wedge
                                 u_i++;
                                 if (u_i >= d) {
                                     w_i++;
@@ -411,9 +426,6 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                                     }
                                 }
                             }
-                            wpos[j] = vb + w_i;
-                            upos[j] = vb + u_i;
-                            i++;
                         }
                     }
                     uint32_t wv[4], uv[4], wl[4];
@@ -451,7 +463,6 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                 for (uint32_t j0 = 0; j0 < kn; j0 += kWays) {
                     uint32_t pw[kWays], pu[kWays];
                     OffT b[kWays], len[kWays];
-                    bool hit[kWays];
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
                         const uint32_t j = j0 + k;
@@ -467,27 +478,27 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                     }
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
-                        hit[k] = false;
                         b[k] = 0;
                         len[k] = 0;
                         if (pw[k] != kSentinel) {
                             if (pw[k] >= thr) {  // top-a bitmap (tc.cu:262-272)
                                 const uint64_t idx =
                                     bm_size - binom2_d((uint64_t)gn - pw[k]) + (pu[k] - pw[k] - 1);
-                                hit[k] = (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
+                                hc += (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
                             } else {  // sorted-list probe of adj+(w) (tc.cu:273-286)
                                 b[k] = rowPtr[pw[k]];
                                 len[k] = rowPtr[pw[k] + 1] - b[k];
                             }
                         }
                     }
-                    // Binary search; with a window, only until the rest of
-                    // the range fits one aligned kWin-element window.
+                    // Binary search, updates by selects; with a window, only
+                    // until the rest of the range fits one aligned window.
                     for (;;) {
                         bool act[kWays], more = false;
 #pragma unroll
                         for (int k = 0; k < kWays; k++) {
-                            act[k] = len[k] > 0 && (kWin == 0 || (uint64_t)len[k] + win_skew(colInd + b[k]) > kWin);
+                            act[k] = kWin == 0 ? len[k] > 0
+                                               : (uint64_t)len[k] + win_skew(colInd + b[k]) > (uint64_t)kWin;
                             more |= act[k];
                         }
                         if (!more) break;
@@ -496,18 +507,18 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                         for (int k = 0; k < kWays; k++) x[k] = act[k] ? colInd[b[k] + (len[k] >> 1)] : 0u;
 #pragma unroll
                         for (int k = 0; k < kWays; k++) {
-                            if (act[k]) {
-                                const OffT half = len[k] >> 1;
-                                if (x[k] == pu[k]) {
-                                    hit[k] = true;
-                                    len[k] = 0;
-                                } else if (x[k] < pu[k]) {
-                                    b[k] += half + 1;
-                                    len[k] -= half + 1;
-                                } else {
-                                    len[k] = half;
-                                }
-                            }
+                            // go right past the probed element, or keep the left
+                            // half; no early exit on a match (the range left
+                            // holds only smaller ids, so nothing matches twice),
+                            // so the update is a few selects.  Inactive ways:
+                            // go = 0 keeps b, and len stays (0 without a window).
+                            const OffT half = len[k] >> 1;
+                            const bool lt = x[k] < pu[k];
+                            const OffT go = act[k] && lt ? half + 1 : (OffT)0;
+                            hc += act[k] && x[k] == pu[k] ? 1u : 0u;
+                            b[k] += go;
+                            const OffT nl = lt ? len[k] - go : half;
+                            len[k] = kWin == 0 ? nl : act[k] ? nl : len[k];
                         }
                     }
                     if constexpr (kWin > 0) {
@@ -516,12 +527,11 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                         // 4 elements, then compares in registers.  The window may
                         // reach up to kWin-1 elements past the end of colInd;
                         // every colInd allocation carries that much tail slack.
-                        uint4 wv4[kWays][kWin / 4];
+                        u32x4 wv4[kWays][kWin / 4];
 #pragma unroll
                         for (int k = 0; k < kWays; k++) {
                             if (len[k] > 0) {
-                                const uint4 *p = reinterpret_cast<const uint4 *>(
-                                    reinterpret_cast<uintptr_t>(colInd + b[k]) & ~(uintptr_t)15);
+                                const gptr<u32x4> p = (gptr<u32x4>)((uintptr_t)(colInd + b[k]) & ~(uintptr_t)15);
 #pragma unroll
                                 for (int q = 0; q < kWin / 4; q++) wv4[k][q] = p[q];
                             }
@@ -539,16 +549,15 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                                     f |= (wv4[k][q].z == pu[k]) & (e0 + 2 >= lo) & (e0 + 2 < hi);
                                     f |= (wv4[k][q].w == pu[k]) & (e0 + 3 >= lo) & (e0 + 3 < hi);
                                 }
-                                hit[k] = hit[k] || f;
+                                hc += f ? 1u : 0u;
                             }
                         }
                     }
-#pragma unroll
-                    for (int k = 0; k < kWays; k++) tile_cnt += (uint64_t)__popcll(__ballot(hit[k]));
                 }
                 __syncwarp();
             }
 
+            const uint64_t tile_cnt = uni64(wave_sum((uint64_t)hc));
             if (g != cur_g) {
                 if (cur_cnt != 0 && lane == 0) atomicAdd(&counts[cur_g], (unsigned long long)cur_cnt);
                 cur_g = g;

@@ -139,6 +139,8 @@ _sig = {
     "wtc_batch_count": (C.c_int, [_P, _P, _P, _P, _P, C.POINTER(CountStats)]),
     "wtc_batch_count_host": (C.c_int, [_P, C.POINTER(_BGP), C.c_size_t, _P, _P, _P, C.POINTER(CountStats)]),
     "wtc_autotune": (None, [C.c_uint32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "wtc_autotune_batch": (None, [C.c_uint32, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint32),
+                                  C.POINTER(C.c_uint32)]),
     "wtc_server_create": (C.c_int, [C.c_int, C.POINTER(ServerConfig), C.POINTER(_P)]),
     "wtc_server_register": (C.c_int, [_P, _BGP, C.POINTER(C.c_uint64)]),
     "wtc_server_submit": (C.c_int, [_P, C.c_uint64, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
@@ -305,9 +307,11 @@ def gen_barabasi_albert(n: int, m0: int, seed: int = 1) -> CSR:
     return _adopt(out)
 
 
-def autotune(n: int, m: int = 0, wedges: int = 0):
+def autotune(n: int, m: int = 0, wedges: int = 0, batch: bool = False):
+    """(spread, a) for a graph counted alone, or as a batch member."""
     s, a = C.c_uint32(), C.c_uint32()
-    _lib.wtc_autotune(n, m, wedges, C.byref(s), C.byref(a))
+    fn = _lib.wtc_autotune_batch if batch else _lib.wtc_autotune
+    fn(n, m, wedges, C.byref(s), C.byref(a))
     return int(s.value), int(a.value)
 
 
