@@ -72,8 +72,15 @@ def cpu_baseline(g, dg, spread, a, target_s, threads):
     cpu_count = O.tc_merge_range(g, v0, v1, threads)
     dt = time.perf_counter() - t0
     gpu_count = dg.count(spread, a, int(ws[v0]), int(ws[v1]))
+    # the same oracle on one thread, over the first eighth of the sample's rows
+    v1s = int(np.searchsorted(ws, ws[v0] + sample_w // 8, side="right"))
+    v1s = min(max(v1s, v0 + 1), v1)
+    t0 = time.perf_counter()
+    O.tc_merge_range(g, v0, v1s, 1)
+    dt1 = time.perf_counter() - t0
     return {
         "value": sample_w / dt, "unit": "wedge checks/s", "cores": threads, "kind": "port",
+        "value_1thread": int(ws[v1s] - ws[v0]) / dt1,
         "sample": f"oracle merge-intersection over rows [{v0},{v1}) = {sample_w} of {W} wedges "
                   f"({100.0 * sample_w / W:.2f}%), {dt:.1f} s on {threads} host threads",
         "seconds": dt, "sample_triangles": cpu_count, "gpu_sample_triangles": gpu_count,
@@ -345,14 +352,18 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--spread", type=int, default=8)
-    ap.add_argument("--a", type=int, default=16384)
+    ap.add_argument("--spread", type=int, default=0, help="0: the autotuner's choice")
+    ap.add_argument("--a", type=int, default=-1, help="-1: the autotuner's choice")
     ap.add_argument("--ways", type=int, default=4, help="probes advanced together per lane")
-    ap.add_argument("--chunk", type=int, default=16, help="wedges per lane staged in LDS per phase")
+    ap.add_argument("--chunk", type=int, default=12, help="wedges per lane staged in LDS per phase")
     ap.add_argument("--window", type=int, default=0, help="probe window in elements (0, 8 or 16)")
+    ap.add_argument("--builder", choices=("auto", "host", "device"), default="auto",
+                    help="build the oriented CSR on the host or on the GPU (auto: GPU from scale 26)")
     ap.add_argument("--offset-bytes", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-range-check", action="store_true",
+                    help="skip the four partial-range counts (keeps profiler runs to whole-graph launches)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run the calibration copy of this many MiB (for profiler counters)")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("KFETCH", "KWRITE", "CFETCH", "CWRITE"),
@@ -400,21 +411,33 @@ def main():
         return
 
     workload = f"rmat{args.scale}"
-    t0 = time.perf_counter()
-    sym = wtc.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
-    t1 = time.perf_counter()
-    g = wtc.preprocess(sym)
-    t2 = time.perf_counter()
-    del sym
-    W = g.wedges()
-    log(f"{workload}: n={g.n} m={g.m} W={W} gen {t1 - t0:.1f}s prepro {t2 - t1:.1f}s")
-    ob = args.offset_bytes or (4 if g.m < 2 ** 32 - 1 else 8)
-    b_alg, b_min = g.alg_bytes(ob)
-
     ctx = wtc.Context(0)
     ctx.set_tuning(args.ways, args.chunk, args.window)
-    dg = ctx.upload(g, offset_bytes=ob)
-    log(f"uploaded {dg.nbytes / 1e9:.2f} GB")
+    device_build = args.builder == "device" or (args.builder == "auto" and args.scale >= 26)
+    t0 = time.perf_counter()
+    if device_build:  # generate, sort, deduplicate and orient on the GPU; keep it resident
+        dg = wtc.DeviceGraph.gen_rmat(ctx, args.scale, args.edge_factor, seed=args.seed,
+                                      offset_bytes=args.offset_bytes)
+        t1 = t2 = time.perf_counter()
+        log(f"{workload}: built on the GPU in {t2 - t0:.1f}s ({dg.nbytes / 1e9:.2f} GB resident)")
+        g = dg.download()  # host copy for W, B_alg and the CPU baseline
+        ob = dg.offset_bytes
+    else:
+        sym = wtc.gen_rmat(args.scale, args.edge_factor, seed=args.seed)
+        t1 = time.perf_counter()
+        g = wtc.preprocess(sym)
+        t2 = time.perf_counter()
+        del sym
+        ob = args.offset_bytes or (4 if g.m < 2 ** 32 - 1 else 8)
+        dg = ctx.upload(g, offset_bytes=ob)
+        log(f"uploaded {dg.nbytes / 1e9:.2f} GB")
+    W = g.wedges()
+    log(f"{workload}: n={g.n} m={g.m} W={W} gen {t1 - t0:.1f}s prepro {t2 - t1:.1f}s")
+    b_alg, b_min = g.alg_bytes(ob)
+    auto_s, auto_a = wtc.autotune(g.n, g.m, W)
+    args.spread = args.spread or auto_s
+    args.a = auto_a if args.a < 0 else args.a
+    log(f"spread={args.spread} a={args.a} ways={args.ways} chunk={args.chunk} window={args.window}")
     for _ in range(args.warmup):
         dg.count(args.spread, args.a)
     counts, stats, wall = [], [], []
@@ -460,7 +483,15 @@ def main():
                      "frac": achieved / HBM_PEAK, "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "b_alg": b_alg, "b_min": b_min},
         "prepro_s": t2 - t1,
+        "builder": "device" if device_build else "host",
+        "tuning": {"probe_ways": args.ways, "lds_chunk": args.chunk, "probe_window": args.window},
     }
+    # size-independent check at full size: four wedge ranges sum to the whole count
+    if not args.no_range_check:
+        cuts = [0, W // 5, W // 2, (4 * W) // 5, W]
+        parts = [dg.count(args.spread, args.a, cuts[i], cuts[i + 1]) for i in range(4)]
+        out["range_split"] = {"cuts": cuts, "parts": parts, "sum_equals_total": sum(parts) == counts[0]}
+        log(f"range split: {parts} sum ok = {sum(parts) == counts[0]}")
     if not args.no_cpu:
         threads = min(len(os.sched_getaffinity(0)), 16)
         out["cpu_baseline"] = cpu_baseline(g, dg, args.spread, args.a, args.cpu_seconds, threads)

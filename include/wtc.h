@@ -191,6 +191,8 @@ int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int pro
 int wtc_dgraph_upload(wtc_ctx *ctx, const wtc_big_graph *g, uint32_t offset_bytes, wtc_dgraph **out);
 void wtc_dgraph_free(wtc_dgraph *dg);
 uint64_t wtc_dgraph_bytes(const wtc_dgraph *dg);
+/* Vertices, oriented edges and row-offset width (4 or 8) of a device graph. */
+int wtc_dgraph_info(const wtc_dgraph *dg, uint32_t *n, uint64_t *m, uint32_t *offset_bytes);
 
 /* Builds the oriented CSR on the device from (src,dst) host pairs, as
  * read_graph -z then preprocess would (GPU preprocessing, result identical to

@@ -490,6 +490,14 @@ void wtc_dgraph_free(wtc_dgraph *dg) {
     delete dg;
 }
 
+int wtc_dgraph_info(const wtc_dgraph *dg, uint32_t *n, uint64_t *m, uint32_t *offset_bytes) {
+    if (!dg) return WTC_ERR_ARG;
+    if (n) *n = dg->n;
+    if (m) *m = dg->m;
+    if (offset_bytes) *offset_bytes = (uint32_t)dg->ob;
+    return WTC_OK;
+}
+
 uint64_t wtc_dgraph_bytes(const wtc_dgraph *dg) {
     return dg ? ((uint64_t)dg->n + 1) * dg->ob + dg->m * 4u : 0;
 }

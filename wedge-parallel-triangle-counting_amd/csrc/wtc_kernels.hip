@@ -20,6 +20,9 @@
 
 namespace wtc {
 
+// Tiles a wave of k_count takes per grab from its XCD's work counter.
+constexpr uint64_t kTilesPerGrab = 8;
+
 // Rightmost graph g with desc[g].*field <= key (wave-uniform, scalar loads).
 __device__ __forceinline__ uint32_t graph_of(const GraphDesc *__restrict__ desc, uint32_t G, uint64_t key,
                                             uint64_t GraphDesc::*field) {
@@ -128,6 +131,12 @@ __global__ __launch_bounds__(kBlock) void k_setup(GraphDesc *__restrict__ desc, 
     if (threadIdx.x == 0) {
         meta[kMetaTiles] = carry;
         meta[kMetaWedges] = wsum;
+        // super-chunks of the XCD interleave: 2^k grabs of kTilesPerGrab tiles,
+        // the largest power of two (at most 64 grabs) that still gives every
+        // XCD at least 64 super-chunks
+        uint64_t lg = 0;
+        while (lg < 6 && (carry / kTilesPerGrab) >> (lg + 1) >= 8ull * 64) lg++;
+        meta[kMetaChunkLog] = lg;
     }
     if (threadIdx.x < 8) meta[kMetaNext + threadIdx.x] = 0;  // per-XCD tile counters of k_count
     if (tree != nullptr) {  // single-graph sets only
@@ -285,9 +294,6 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
     u_out[i] = u;
 }
 
-// Tiles a wave takes per grab from its XCD's work counter.
-constexpr uint64_t kTilesPerGrab = 8;
-
 // Four ids loaded by one 16-byte vector load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -312,14 +318,18 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
     uint32_t *const lw = lds + wid * (2u * kWave * pad_max);
     uint32_t *const lu = lw + kWave * pad_max;
 
-    // XCD partition: blocks go round-robin to the 8 XCDs, so blocks with the
-    // same blockIdx % 8 share an L2; each such group sweeps one contiguous
-    // eighth of the tiles, handed out kTilesPerGrab at a time by a counter
-    // (dynamic balance; consecutive tiles let a wave gallop from the vertex
-    // where its previous tile ended instead of searching from the root).
+    // XCD interleave: blocks go round-robin to the 8 XCDs, so blocks with the
+    // same blockIdx % 8 share an L2.  The tiles are cut into super-chunks of
+    // 2^lg grabs (kTilesPerGrab consecutive tiles each), dealt to the XCDs in
+    // turn: XCD x owns super-chunks x, x+8, x+16, ..., so every XCD gets an
+    // even mix of cheap and expensive regions of the wedge space (the cost per
+    // wedge varies several-fold along it), while each super-chunk stays
+    // contiguous for its L2.  Inside an XCD, waves take grabs from the XCD's
+    // own counter (dynamic balance); consecutive tiles of a grab let a wave
+    // gallop from the vertex where its previous tile ended.
     const uint64_t ntiles = meta[kMetaTiles];
+    const uint32_t lg = (uint32_t)meta[kMetaChunkLog];
     const uint32_t xcd = blockIdx.x & 7u;
-    const uint64_t t_lo = ntiles * xcd / 8, t_hi = ntiles * (xcd + 1) / 8;
     unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaNext + xcd);
 
     uint32_t cur_g = kSentinel;
@@ -328,9 +338,11 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
     for (;;) {
         unsigned long long grab = 0;
         if (lane == 0) grab = atomicAdd(ctr, 1ull);
-        const uint64_t tb = t_lo + uni64(grab) * kTilesPerGrab;
-        if (tb >= t_hi) break;
-        const uint64_t te = tb + kTilesPerGrab < t_hi ? tb + kTilesPerGrab : t_hi;
+        const uint64_t gi = uni64(grab);
+        const uint64_t chunk_id = xcd + 8 * (gi >> lg);  // this XCD's super-chunk
+        const uint64_t tb = ((chunk_id << lg) + (gi & ((1ull << lg) - 1))) * kTilesPerGrab;
+        if (tb >= ntiles) break;  // tb grows with gi, so every later grab is past the end too
+        const uint64_t te = tb + kTilesPerGrab < ntiles ? tb + kTilesPerGrab : ntiles;
 
         uint32_t g = kSentinel, v_prev = 0;
         uint64_t g_tile_end = 0;
@@ -476,20 +488,32 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                             pu[k] = lu[q * pad + r];
                         }
                     }
+                    // Every way's first load is issued before any is used, so
+                    // the ways' round trips overlap: a bitmap word for w in the
+                    // top-a ranks (tc.cu:262-272), else rowPtr[w], rowPtr[w+1]
+                    // for the sorted-list probe of adj+(w) (tc.cu:273-286).
+                    bool isb[kWays], isl[kWays];
+                    uint64_t bidx[kWays];
+                    uint32_t bw[kWays];
+                    OffT r0[kWays], r1[kWays];
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
-                        b[k] = 0;
-                        len[k] = 0;
-                        if (pw[k] != kSentinel) {
-                            if (pw[k] >= thr) {  // top-a bitmap (tc.cu:262-272)
-                                const uint64_t idx =
-                                    bm_size - binom2_d((uint64_t)gn - pw[k]) + (pu[k] - pw[k] - 1);
-                                hc += (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
-                            } else {  // sorted-list probe of adj+(w) (tc.cu:273-286)
-                                b[k] = rowPtr[pw[k]];
-                                len[k] = rowPtr[pw[k] + 1] - b[k];
-                            }
-                        }
+                        isb[k] = pw[k] != kSentinel && pw[k] >= thr;
+                        isl[k] = pw[k] < thr;
+                        bidx[k] = isb[k] ? bm_size - binom2_d((uint64_t)gn - pw[k]) + (pu[k] - pw[k] - 1) : 0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) bw[k] = isb[k] ? bitmap[bidx[k] >> 5] : 0u;
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) {
+                        r0[k] = isl[k] ? rowPtr[pw[k]] : (OffT)0;
+                        r1[k] = isl[k] ? rowPtr[pw[k] + 1] : (OffT)0;
+                    }
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) {
+                        hc += (bw[k] >> (bidx[k] & 31u)) & 1u;
+                        b[k] = r0[k];
+                        len[k] = r1[k] - r0[k];
                     }
                     // Binary search, updates by selects; with a window, only
                     // until the rest of the range fits one aligned window.

@@ -122,6 +122,7 @@ _sig = {
     "wtc_dgraph_upload": (C.c_int, [_P, _BGP, C.c_uint32, C.POINTER(_P)]),
     "wtc_dgraph_free": (None, [_P]),
     "wtc_dgraph_bytes": (C.c_uint64, [_P]),
+    "wtc_dgraph_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint32)]),
     "wtc_dgraph_from_pairs": (C.c_int, [_P, _P, C.c_uint64, C.c_uint32, C.POINTER(_P)]),
     "wtc_dgraph_gen_rmat": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                      C.c_uint32, C.POINTER(_P)]),
@@ -407,12 +408,12 @@ class DeviceGraph:
         self._p = C.c_void_p()
         if _handle is not None:
             self._p = _handle
-            host = self.download()
-            self.n, self.m = host.n, host.m
-            return
-        self.n, self.m = g.n, g.m
-        s = g._struct()
-        _check(_lib.wtc_dgraph_upload(ctx._p, C.byref(s), offset_bytes, C.byref(self._p)), "dgraph_upload")
+        else:
+            s = g._struct()
+            _check(_lib.wtc_dgraph_upload(ctx._p, C.byref(s), offset_bytes, C.byref(self._p)), "dgraph_upload")
+        n, m, ob = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        _check(_lib.wtc_dgraph_info(self._p, C.byref(n), C.byref(m), C.byref(ob)), "dgraph_info")
+        self.n, self.m, self.offset_bytes = int(n.value), int(m.value), int(ob.value)
 
     @classmethod
     def from_pairs(cls, ctx: Context, pairs: np.ndarray, offset_bytes: int = 0) -> "DeviceGraph":
