@@ -590,9 +590,9 @@ namespace {
 
 // Per-size-bucket choices measured on MI355X (device time of prefix, bitmap
 // and count; best of a sweep of s in {1..16} and a in {0, 256 .. 2^20}):
-//  * R-MAT graphs counted alone (scales 12-24): s = 2 / 4 / 8 / 12 for W below
-//    2^20 / 2^23 / 2^26 / above; a = 0 below 2^23 wedges, 2048 below 2^26,
-//    then about n/8 capped at 131072 (n < 2^23) or 262144.
+//  * R-MAT graphs counted alone (scales 12-24): s = 2 / 4 / 8 / 16 / 12 for W
+//    below 2^20 / 2^23 / 2^26 / 2^31 / above; a = 0 below 2^23 wedges, 2048
+//    below 2^26, then about n/32 capped at 32768 (n < 2^22) or 131072.
 //  * Erdos-Renyi / Barabasi-Albert graphs in a batch of 4096 (n 64..65536,
 //    degree 4..64): a = 0 in every size bucket; s = 4 / 8 / 12 for n below
 //    128 / 256 / above.
@@ -603,8 +603,8 @@ namespace {
 constexpr uint64_t kSkewWedgesPerEdge = 24;
 
 uint32_t skewed_bitmap_len(uint32_t n) {
-    const uint32_t cap = n < (1u << 23) ? 131072u : 262144u;
-    const uint32_t a = ((n / 8) + 31u) & ~31u;
+    const uint32_t cap = n < (1u << 22) ? 32768u : 131072u;
+    const uint32_t a = ((n / 32) + 31u) & ~31u;
     return a < cap ? a : cap;
 }
 
@@ -617,7 +617,7 @@ void autotune_params(uint32_t n, uint64_t m, uint64_t wedges, bool in_batch, uin
         s = n < 128 ? 4 : n < 256 ? 8 : 12;
         if (skewed && wedges >= (1u << 23)) aa = wedges < (1u << 26) ? 2048u : skewed_bitmap_len(n);
     } else {  // alone: small graphs need small tiles to fill the GPU
-        s = wedges < (1u << 20) ? 2 : wedges < (1u << 23) ? 4 : wedges < (1u << 26) ? 8 : 12;
+        s = wedges < (1u << 20) ? 2 : wedges < (1u << 23) ? 4 : wedges < (1u << 26) ? 8 : wedges < (1ull << 31) ? 16 : 12;
         if (skewed && wedges >= (1u << 23)) aa = wedges < (1u << 26) ? 2048u : skewed_bitmap_len(n);
     }
     if (spread) *spread = s;
