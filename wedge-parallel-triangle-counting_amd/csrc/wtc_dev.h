@@ -9,6 +9,9 @@ __device__ __forceinline__ uint64_t binom2_d(uint64_t x) {
     return (x & 1u) ? x * ((x - 1u) >> 1) : (x >> 1) * (x - 1u);
 }
 
+// C(x,2) of a 32-bit x: x(x-1) < 2^64, so one 32x32->64 multiply and a shift.
+__device__ __forceinline__ uint64_t binom2_u32(uint32_t x) { return ((uint64_t)x * (uint64_t)(x - 1u)) >> 1; }
+
 __device__ __forceinline__ uint32_t uni32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {

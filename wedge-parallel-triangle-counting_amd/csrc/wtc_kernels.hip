@@ -193,7 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_build(const GraphDesc *__rest
         const uint32_t *__restrict__ colInd = D.colInd;
         uint32_t *__restrict__ bm = D.bitmap;
         const uint32_t v = D.n - D.bm_rows + (uint32_t)(r - D.row_base);
-        const uint64_t rowbits = D.bm_size - binom2_d((uint64_t)D.n - v) - v - 1;
+        const uint64_t rowbits = D.bm_size - binom2_u32(D.n - v) - v - 1;
         const OffT b = rowPtr[v], e = rowPtr[v + 1];
         for (OffT j = b + lane; j < e; j += kWave) {
             const uint64_t idx = rowbits + colInd[j];
@@ -393,10 +393,12 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
             // search in [v_lo, v_hi] and decoded exactly, the rest by stepping
             const uint64_t first = tile_start + (uint64_t)lane * s;
             const uint64_t lend = first + s < tile_end ? first + s : tile_end;
-            uint64_t i = first;
+            // wedges of this lane not yet staged (<= s, so 32 bits); the 64-bit
+            // index of the next one is lend - left
+            uint32_t left = first < lend ? (uint32_t)(lend - first) : 0u;
             uint32_t v = 0, d = 0, w_i = 0, u_i = 0;
             OffT vb = 0;
-            if (first < lend) {
+            if (left) {
                 v = rightmost_le(ws, v_lo, v_hi, first);
                 vb = rowPtr[v];
                 d = (uint32_t)(rowPtr[v + 1] - vb);
@@ -417,19 +419,20 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                     bool ok[4];
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
-                        ok[j] = (k0 + j < kn) && (i < lend);
+                        ok[j] = (k0 + j < kn) && left != 0;
                         wpos[j] = 0;
                         upos[j] = 0;
                         if (ok[j]) {
                             wpos[j] = vb + w_i;
                             upos[j] = vb + u_i;
-                            i++;
-                            if (i < lend) {  // step to the lane's next wedge
+                            left--;
+                            if (left) {  // step to the lane's next wedge
                                 u_i++;
                                 if (u_i >= d) {
                                     w_i++;
                                     u_i = w_i + 1;
                                     if (u_i >= d) {  // vertex exhausted: next vertex with wedges
+                                        const uint64_t i = lend - left;  // the next wedge
                                         v = (ws[v + 2] > i) ? v + 1 : rightmost_le(ws, v + 2, v_hi, i);
                                         vb = rowPtr[v];
                                         d = (uint32_t)(rowPtr[v + 1] - vb);
@@ -462,8 +465,8 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
                         if (k0 + j < kn) {
-                            lw[lane * pad + k0 + j] = wv[j];
-                            lu[lane * pad + k0 + j] = uv[j];
+                            lw[__umul24(lane, pad) + k0 + j] = wv[j];
+                            lu[__umul24(lane, pad) + k0 + j] = uv[j];
                         }
                     }
                 }
@@ -481,11 +484,15 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                         pw[k] = kSentinel;
                         pu[k] = 0;
                         if (j < kn) {
+                            // p < 64 * kChunkMax and magic <= 2^16: operands
+                            // below 2^24 and products below 2^32, so the
+                            // full-rate 24-bit multiplies are exact
                             const uint32_t p = j * kWave + lane;
-                            const uint32_t q = (p * magic) >> 16;
-                            const uint32_t r = p - q * kn;
-                            pw[k] = lw[q * pad + r];
-                            pu[k] = lu[q * pad + r];
+                            const uint32_t q = __umul24(p, magic) >> 16;
+                            const uint32_t r = p - __umul24(q, kn);
+                            const uint32_t slot = __umul24(q, pad) + r;
+                            pw[k] = lw[slot];
+                            pu[k] = lu[slot];
                         }
                     }
                     // Every way's first load is issued before any is used, so
@@ -500,7 +507,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                     for (int k = 0; k < kWays; k++) {
                         isb[k] = pw[k] != kSentinel && pw[k] >= thr;
                         isl[k] = pw[k] < thr;
-                        bidx[k] = isb[k] ? bm_size - binom2_d((uint64_t)gn - pw[k]) + (pu[k] - pw[k] - 1) : 0;
+                        bidx[k] = isb[k] ? bm_size - binom2_u32(gn - pw[k]) + (pu[k] - pw[k] - 1) : 0;
                     }
 #pragma unroll
                     for (int k = 0; k < kWays; k++) bw[k] = isb[k] ? bitmap[bidx[k] >> 5] : 0u;
