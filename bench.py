@@ -369,11 +369,14 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, metavar=("KFETCH", "KWRITE", "CFETCH", "CWRITE"),
                     help="summarise rocprofv3 --pmc output dirs into profiles/ (no GPU use)")
     ap.add_argument("--pmc-workload", default="rmat24")
-    ap.add_argument("--batch-graphs", type=int, default=0, help="also run batched vs plain on this many graphs")
+    ap.add_argument("--batch-graphs", type=int, default=4096, help="batched vs plain on this many graphs (0: skip)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="only the RMAT count: no batched-vs-plain and no serving legs (profiler runs)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched vs plain comparison")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving (Poisson arrivals) workload")
-    ap.add_argument("--serve-rates", default="100,250,500,1000,2500,5000")
-    ap.add_argument("--serve-seconds", type=float, default=5.0)
+    ap.add_argument("--serve-rates", default="100,1000",
+                    help="Poisson arrival rates of the serving leg (empty: skip it)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--ingest-only", action="store_true", help="run only the text/CSR-file ingestion timing")
     args = ap.parse_args()
 
@@ -497,11 +500,15 @@ def main():
         out["cpu_baseline"] = cpu_baseline(g, dg, args.spread, args.a, args.cpu_seconds, threads)
         log(f"cpu baseline: {out['cpu_baseline']['sample']}")
     dg.free()
-    if args.batch_graphs:
+    if args.batch_graphs and not args.no_extras:
         out["batched_vs_plain"] = batch_bench(ctx, args.batch_graphs, 1000, 32,
                                               min(len(os.sched_getaffinity(0)), 16))
         log(f"batched/plain = {out['batched_vs_plain']['batched_over_plain']:.2f}x")
     ctx.close()
+    if args.serve_rates and not args.no_extras:
+        out["serving"] = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 2, 512,
+                                     args.ways, args.chunk, args.window)
+        log(f"served/plain = {out['serving']['served_over_plain']:.2f}x")
     print(json.dumps(out), flush=True)
 
 

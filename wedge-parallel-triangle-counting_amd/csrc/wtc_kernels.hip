@@ -404,7 +404,11 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                 d = (uint32_t)(rowPtr[v + 1] - vb);
                 decode_wedge(d, first - ws[v], w_i, u_i);
             }
-            uint32_t hc = 0;  // this lane's hits in the tile (a probe matches at most once)
+            // hits of the wave in this tile: popcount(ballot(match)) as each
+            // probe step resolves (a probe matches at most once); the ballot
+            // is the compare's own lane mask, so counting costs scalar
+            // instructions only
+            uint64_t tile_cnt = 0;
             // colInd position and value of the last w loaded: consecutive
             // wedges of a lane mostly share w, whose id is then not reloaded
             OffT last_wpos = ~(OffT)0;
@@ -518,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                     }
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
-                        hc += (bw[k] >> (bidx[k] & 31u)) & 1u;
+                        tile_cnt += (uint64_t)__popcll(__ballot((bw[k] >> (bidx[k] & 31u)) & 1u));
                         b[k] = r0[k];
                         len[k] = r1[k] - r0[k];
                     }
@@ -532,7 +536,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                                                : (uint64_t)len[k] + win_skew(colInd + b[k]) > (uint64_t)kWin;
                             more |= act[k];
                         }
-                        if (!more) break;
+                        // wave-uniform exit: every lane stays in the loop until
+                        // all are done, so the ballots below see the whole wave
+                        if (__ballot(more) == 0) break;
                         uint32_t x[kWays];
 #pragma unroll
                         for (int k = 0; k < kWays; k++) x[k] = act[k] ? colInd[b[k] + (len[k] >> 1)] : 0u;
@@ -546,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                             const OffT half = len[k] >> 1;
                             const bool lt = x[k] < pu[k];
                             const OffT go = act[k] && lt ? half + 1 : (OffT)0;
-                            hc += act[k] && x[k] == pu[k] ? 1u : 0u;
+                            tile_cnt += (uint64_t)__popcll(__ballot(act[k] && x[k] == pu[k]));
                             b[k] += go;
                             const OffT nl = lt ? len[k] - go : half;
                             len[k] = kWin == 0 ? nl : act[k] ? nl : len[k];
@@ -569,9 +575,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                         }
 #pragma unroll
                         for (int k = 0; k < kWays; k++) {
+                            bool f = false;
                             if (len[k] > 0) {
                                 const uint32_t lo = win_skew(colInd + b[k]), hi = lo + (uint32_t)len[k];
-                                bool f = false;
 #pragma unroll
                                 for (int q = 0; q < kWin / 4; q++) {
                                     const uint32_t e0 = 4 * q;
@@ -580,15 +586,14 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                                     f |= (wv4[k][q].z == pu[k]) & (e0 + 2 >= lo) & (e0 + 2 < hi);
                                     f |= (wv4[k][q].w == pu[k]) & (e0 + 3 >= lo) & (e0 + 3 < hi);
                                 }
-                                hc += f ? 1u : 0u;
                             }
+                            tile_cnt += (uint64_t)__popcll(__ballot(f));  // converged: outside the branch
                         }
                     }
                 }
                 __syncwarp();
             }
 
-            const uint64_t tile_cnt = uni64(wave_sum((uint64_t)hc));
             if (g != cur_g) {
                 if (cur_cnt != 0 && lane == 0) atomicAdd(&counts[cur_g], (unsigned long long)cur_cnt);
                 cur_g = g;
