@@ -198,3 +198,30 @@ def test_batch_graph_replays(gpu_ctx):
     finally:
         gpu_ctx.set_tuning()
         b.free()
+
+
+def test_config1_chung_lu_full_size(gpu_ctx):
+    """Config 1 at its real size: Chung-Lu, n = 262,144, ~900k unique edges,
+    power-law expected degrees (exponent 2.7 on [1, 420]); host preprocessing,
+    the oracle's count, then s=5, a=8192 on the GPU (and the tc_GPU drop-ins)."""
+    g = wtc.preprocess(wtc.gen_chung_lu(262144, 2.7, 1, 420, 900000, seed=1))
+    assert g.m == 900000
+    expect = O.tc_merge(g)
+    assert expect > 0
+    dg = gpu_ctx.upload(g)
+    assert dg.count(5, 8192) == expect
+    dg.free()
+    assert wtc.tc_gpu(g, 5, 8192) == expect
+    assert wtc.tc_gpu_big(g, 5, 8192) == expect
+
+
+def test_config2_rmat20_sweep(gpu_ctx):
+    """Config 2 at its real size: R-MAT scale 20, edge factor 16, u32 ids and
+    offsets, every (s, a) of the sweep exact against the oracle."""
+    g = wtc.preprocess(wtc.gen_rmat(20, 16, seed=1))
+    expect = O.tc_merge(g)
+    dg = gpu_ctx.upload(g, offset_bytes=4)
+    for s in (1, 3, 5, 8):
+        for a in (0, 4096, 8192, 16384):
+            assert dg.count(s, a) == expect, (s, a)
+    dg.free()
