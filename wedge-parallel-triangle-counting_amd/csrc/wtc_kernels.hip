@@ -297,21 +297,6 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
 // Four ids loaded by one 16-byte vector load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// rowPtr[w] and rowPtr[w+1] in one vector load (element-aligned, which the
-// global memory instructions accept for multi-dword loads).
-typedef uint32_t u32x2_e __attribute__((ext_vector_type(2), aligned(4)));
-typedef uint64_t u64x2_e __attribute__((ext_vector_type(2), aligned(8)));
-__device__ __forceinline__ void row_bounds(gptr<uint32_t> rowPtr, uint32_t w, uint32_t &b, uint32_t &e) {
-    const u32x2_e p = *(gptr<u32x2_e>)(rowPtr + w);
-    b = p.x;
-    e = p.y;
-}
-__device__ __forceinline__ void row_bounds(gptr<uint64_t> rowPtr, uint32_t w, uint64_t &b, uint64_t &e) {
-    const u64x2_e p = *(gptr<u64x2_e>)(rowPtr + w);
-    b = p.x;
-    e = p.y;
-}
-
 // Elements between the 16-byte aligned address at or below p and p.
 __device__ __forceinline__ uint32_t win_skew(gptr<uint32_t> p) { return (uint32_t)(((uintptr_t)p >> 2) & 3u); }
 
@@ -415,9 +400,8 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
             OffT vb = 0;
             if (left) {
                 v = rightmost_le(ws, v_lo, v_hi, first);
-                OffT ve;
-                row_bounds(rowPtr, v, vb, ve);
-                d = (uint32_t)(ve - vb);
+                vb = rowPtr[v];
+                d = (uint32_t)(rowPtr[v + 1] - vb);
                 decode_wedge(d, first - ws[v], w_i, u_i);
             }
             // hits of the wave in this tile: popcount(ballot(match)) as each
@@ -454,9 +438,8 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                                     if (u_i >= d) {  // vertex exhausted: next vertex with wedges
                                         const uint64_t i = lend - left;  // the next wedge
                                         v = (ws[v + 2] > i) ? v + 1 : rightmost_le(ws, v + 2, v_hi, i);
-                                        OffT ve;
-                                        row_bounds(rowPtr, v, vb, ve);
-                                        d = (uint32_t)(ve - vb);
+                                        vb = rowPtr[v];
+                                        d = (uint32_t)(rowPtr[v + 1] - vb);
                                         w_i = 0;
                                         u_i = 1;
                                     }
@@ -534,9 +517,8 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                     for (int k = 0; k < kWays; k++) bw[k] = isb[k] ? bitmap[bidx[k] >> 5] : 0u;
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
-                        r0[k] = 0;
-                        r1[k] = 0;
-                        if (isl[k]) row_bounds(rowPtr, pw[k], r0[k], r1[k]);
+                        r0[k] = isl[k] ? rowPtr[pw[k]] : (OffT)0;
+                        r1[k] = isl[k] ? rowPtr[pw[k] + 1] : (OffT)0;
                     }
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
