@@ -296,6 +296,9 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
 
 // Four ids loaded by one 16-byte vector load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// The same at any element (4-byte) alignment, which the global memory
+// instructions accept for multi-dword loads.
+typedef uint32_t u32x4_e __attribute__((ext_vector_type(4), aligned(4)));
 
 // Elements between the 16-byte aligned address at or below p and p.
 __device__ __forceinline__ uint32_t win_skew(gptr<uint32_t> p) { return (uint32_t)(((uintptr_t)p >> 2) & 3u); }
@@ -448,12 +451,32 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                         }
                     }
                     uint32_t wv[4], uv[4], wl[4];
+                    // Inside a row the group's u positions are consecutive:
+                    // then one 16-byte load per lane fetches all four (one
+                    // vL1D access instead of four; up to three elements past a
+                    // lane's last wedge are read, which stay inside colInd or
+                    // its allocation's tail slack).  Across a row end the ids
+                    // are loaded one by one.
+                    bool cont = ok[0];
+#pragma unroll
+                    for (int j = 1; j < 4; j++) cont = cont && (!ok[j] || upos[j] == upos[0] + j);
 #pragma unroll
                     for (int j = 0; j < 4; j++) {  // ok[] is a prefix of the group
                         const OffT prev = j == 0 ? last_wpos : wpos[j - 1];
                         wl[j] = ok[j] && wpos[j] != prev ? colInd[wpos[j]] : 0u;
-                        uv[j] = ok[j] ? colInd[upos[j]] : 0u;
                     }
+                    if (cont) {
+                        const u32x4_e q = *(gptr<u32x4_e>)(colInd + upos[0]);
+                        uv[0] = q.x;
+                        uv[1] = q.y;
+                        uv[2] = q.z;
+                        uv[3] = q.w;
+                    } else {
+#pragma unroll
+                        for (int j = 0; j < 4; j++) uv[j] = ok[j] ? colInd[upos[j]] : 0u;
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) uv[j] = ok[j] ? uv[j] : 0u;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
                         if (ok[j]) {
