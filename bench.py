@@ -149,6 +149,57 @@ def batch_bench(ctx, k, base_seed):
             "per_graph_counts_equal": exact}
 
 
+def small_batch_graphs(k, base_seed, n_max):
+    """A batch of small graphs: k seeded graphs, Erdos-Renyi or
+    Barabasi-Albert (50/50), n log-uniform in [32, n_max], average degree
+    uniform in [4, 32]; returned oriented."""
+    def one(i):
+        r = np.random.default_rng(base_seed + i)
+        n = int(round(np.exp(r.uniform(np.log(32), np.log(n_max)))))
+        kdeg = r.uniform(4, 32)
+        if i % 2:
+            g = wtc.gen_erdos_renyi(n, min(1.0, kdeg / (n - 1)), seed=base_seed + i)
+        else:
+            g = wtc.gen_barabasi_albert(n, max(1, min(n - 1, int(np.ceil(kdeg / 2)))), seed=base_seed + i)
+        return wtc.preprocess(g)
+    return [one(i) for i in range(k)]
+
+
+def small_kernel_bench(ctx, k, base_seed, reps=7):
+    """Kernel choice for sets of small graphs: graph-replay time of one
+    resident batch counted by the small-graph kernel (CSR in LDS, one block
+    per slice of a graph) and by the wedge-parallel kernel, for batches of
+    graphs up to n = 512 and n = 1024."""
+    default, _ = ctx.small_graphs()
+    res = {}
+    try:
+        for n_max in (512, 1024):
+            graphs = small_batch_graphs(k, base_seed, n_max)
+            tune = [wtc.autotune(g.n, g.m, g.wedges(), batch=True) for g in graphs]
+            spread = [s for s, _ in tune]
+            alen = [a for _, a in tune]
+            row = {"graphs": k, "n_max": n_max, "wedges": int(sum(g.wedges() for g in graphs))}
+            counts = {}
+            for name, limit in (("small_kernel", default), ("wedge_kernel", 0)):
+                ctx.set_small_graphs(limit)
+                b = wtc.Batch(ctx, graphs)
+                b.count(spread, alen)  # capture
+                ms = []
+                for _ in range(reps):
+                    got, st = b.count(spread, alen, stats=True)
+                    ms.append(st.count_ms)
+                b.free()
+                counts[name] = got
+                row[f"{name}_replay_ms"] = float(np.median(ms))
+                row[f"{name}_graphs"] = int(st.small_graphs)
+            row["counts_equal"] = bool(np.array_equal(counts["small_kernel"], counts["wedge_kernel"]))
+            row["speedup"] = row["wedge_kernel_replay_ms"] / row["small_kernel_replay_ms"]
+            res[f"n_max_{n_max}"] = row
+    finally:
+        ctx.set_small_graphs(default)
+    return res
+
+
 def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, window):
     """Serving workload: Poisson arrivals of count requests on seeded R-MAT
     graphs (scale uniform in [10, 22], edge factor 16), each with the
@@ -366,6 +417,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="only the RMAT count: no batched-vs-plain and no serving legs (profiler runs)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched vs plain comparison")
+    ap.add_argument("--small-batches-only", action="store_true",
+                    help="run only the small-graph kernel vs wedge kernel comparison on batches of small graphs")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving (Poisson arrivals) workload")
     ap.add_argument("--serve-rates", default="100,1000",
                     help="Poisson arrival rates of the serving leg (empty: skip it)")
@@ -398,10 +451,18 @@ def main():
                           args.ways, args.chunk, args.window)
         print(json.dumps(res), flush=True)
         return
+    if args.small_batches_only:
+        ctx = wtc.Context(0)
+        ctx.set_tuning(args.ways, args.chunk, args.window)
+        res = small_kernel_bench(ctx, args.batch_graphs or 4096, 7000)
+        ctx.close()
+        print(json.dumps(res), flush=True)
+        return
     if args.batch_only:
         ctx = wtc.Context(0)
         ctx.set_tuning(args.ways, args.chunk, args.window)
         res = batch_bench(ctx, args.batch_graphs or 4096, 1000)
+        res["small_graph_batches"] = small_kernel_bench(ctx, args.batch_graphs or 4096, 7000)
         ctx.close()
         print(json.dumps(res), flush=True)
         return
@@ -496,6 +557,9 @@ def main():
     if args.batch_graphs and not args.no_extras:
         out["batched_vs_plain"] = batch_bench(ctx, args.batch_graphs, 1000)
         log(f"batched/plain = {out['batched_vs_plain']['batched_over_plain']:.2f}x")
+        out["small_graph_batches"] = small_kernel_bench(ctx, args.batch_graphs, 7000)
+        log("small-graph kernel speedup: " + ", ".join(
+            f"{k} {v['speedup']:.2f}x" for k, v in out["small_graph_batches"].items()))
     ctx.close()
     if args.serve_rates and not args.no_extras:
         out["serving"] = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 2, 512,

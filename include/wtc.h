@@ -173,6 +173,8 @@ typedef struct {
     uint64_t wedges;   /* W counted                                          */
     uint32_t blocks;   /* counting-kernel grid                               */
     uint32_t launches; /* counting-kernel launches                           */
+    uint32_t small_graphs; /* graphs counted by the small-graph kernel       */
+    uint32_t reserved;
 } wtc_count_stats;
 
 int wtc_device_count(int *count);
@@ -185,6 +187,18 @@ void wtc_ctx_destroy(wtc_ctx *ctx);
  * the search with one aligned window of that many elements tested by vector
  * loads (default 0 or $WTC_PROBE_WINDOW).  Counts do not depend on these. */
 int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int probe_window);
+/* Kernel choice for sets of small graphs (the role of the vertex- and
+ * edge-parallel kernels, experiments/tc_vertex.cu:113-274 and
+ * experiments/tc_edge.cu:179-352, for small graphs in a batch).  A count over
+ * two or more graphs (a batch, a serving batch) in which every graph has
+ * 2(n+1) + m <= max_words is done by the small-graph kernel: one block per
+ * graph holds the graph's CSR and wedge prefix in LDS.  Any other set is
+ * done by the wedge-parallel kernel.  0 turns the small-graph kernel off.
+ * max_words may not exceed the device's LDS per block in words
+ * (device_max_words; 40960 on MI355X).  Default 20480 or
+ * $WTC_SMALL_GRAPH_WORDS.  Counts do not depend on it. */
+int wtc_ctx_set_small_graphs(wtc_ctx *ctx, uint32_t max_words);
+int wtc_ctx_get_small_graphs(const wtc_ctx *ctx, uint32_t *max_words, uint32_t *device_max_words);
 
 /* Copies an oriented CSR to the device.  offset_bytes 4 or 8 selects the
  * device rowPtr width (0: 4 if numEdges < 2^32 else 8). */
@@ -269,6 +283,8 @@ typedef struct {
     int32_t probe_ways;        /* counting-kernel tuning, 0 = default             */
     uint32_t lds_chunk;        /* counting-kernel tuning, 0 = default             */
     int32_t probe_window;      /* 0 = default, -1 = none, else 8 or 16            */
+    int32_t small_graph_words; /* small-graph kernel limit (wtc_ctx_set_small_graphs):
+                                  0 = default, -1 = off, else the limit in words */
 } wtc_server_config;
 typedef struct {
     double queue_ms;      /* submit -> batch start                                */

@@ -225,3 +225,69 @@ def test_config2_rmat20_sweep(gpu_ctx):
         for a in (0, 4096, 8192, 16384):
             assert dg.count(s, a) == expect, (s, a)
     dg.free()
+
+
+def _small_set():
+    """Small graphs of every shape the small-graph kernel must handle: empty,
+    wedge-free, complete, star, sparse and dense random graphs."""
+    k = lambda n: wtc.preprocess(wtc.graph_from_pairs(  # noqa: E731
+        np.array([[i, j] for i in range(n) for j in range(i + 1, n)], np.uint32).reshape(-1, 2)))
+    graphs = [
+        k(3),
+        wtc.preprocess(wtc.graph_from_pairs(np.zeros((0, 2), np.uint32))),           # n = 1, no edges
+        wtc.preprocess(wtc.graph_from_pairs(np.array([[0, 1], [2, 3]], np.uint32))),  # no wedges
+        k(140),                                                                       # W = C(140,3) on one block
+        wtc.preprocess(wtc.graph_from_pairs(np.array([[0, i] for i in range(1, 900)], np.uint32))),  # star
+    ]
+    for i in range(30):
+        n = 40 + 37 * i
+        if i % 2:
+            graphs.append(wtc.preprocess(wtc.gen_erdos_renyi(n, min(1.0, (4 + i % 12) / (n - 1)), seed=500 + i)))
+        else:
+            graphs.append(wtc.preprocess(wtc.gen_barabasi_albert(n, 1 + i % 9, seed=500 + i)))
+    graphs.append(k(40))
+    return graphs
+
+
+def test_small_graph_kernel(gpu_ctx):
+    """Kernel choice per set: a set of small graphs that all fit the limit is
+    counted by the LDS small-graph kernel, any other set by the wedge-parallel
+    kernel; both give the oracle's counts (graph replays and eager launches),
+    at every limit from off to the device maximum.  The limit is inclusive;
+    a set with one larger graph, and a single graph, never take it."""
+    small = _small_set()
+    words = [2 * (g.n + 1) + g.m for g in small]
+    wmax = max(words)
+    big = _rmat(12, seed=5)
+    mixed = small[:10] + [big] + small[10:]
+    expect = [O.tc_merge(g) if g.m else 0 for g in small]
+    big_expect = O.tc_merge(big)
+    mixed_expect = expect[:10] + [big_expect] + expect[10:]
+    default, dev_max = gpu_ctx.small_graphs()
+    assert 0 < default <= dev_max
+    assert dev_max * 4 <= 160 * 1024
+    assert wmax <= default < 2 * (big.n + 1) + big.m
+    try:
+        for limit in sorted({0, 64, 4096, wmax - 1, wmax, default, dev_max}):
+            gpu_ctx.set_small_graphs(limit)
+            assert gpu_ctx.small_graphs() == (limit, dev_max)
+            n_small = len(small) if 0 < limit and wmax <= limit else 0
+            for graphs, exp, ns in ((small, expect, n_small), (mixed, mixed_expect, 0)):
+                b = wtc.Batch(gpu_ctx, graphs)
+                for spread, a in ((None, None), (3, 64)):
+                    for rep in range(2):
+                        got, st = b.count(spread, a, stats=True)
+                        assert got.tolist() == exp, (limit, spread, a, rep)
+                        assert st.small_graphs == ns, (limit, st.small_graphs, ns)
+                b.free()
+                got, st = wtc.batch_count_host(gpu_ctx, graphs, stats=True)
+                assert got.tolist() == exp, limit
+                assert st.small_graphs == ns
+        # a single graph never takes it (one block would hold the whole count)
+        gpu_ctx.set_small_graphs(dev_max)
+        got, st = wtc.batch_count_host(gpu_ctx, [small[3]], stats=True)
+        assert got.tolist() == [expect[3]] and st.small_graphs == 0
+        with pytest.raises(wtc.WtcError):
+            gpu_ctx.set_small_graphs(dev_max + 1)
+    finally:
+        gpu_ctx.set_small_graphs(default)

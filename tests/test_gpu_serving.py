@@ -21,10 +21,11 @@ def _pool():
     return graphs
 
 
-def test_alone_equals_mix():
+@pytest.mark.parametrize("small_graph_words", [0, -1])  # small-graph kernel at its default limit; off
+def test_alone_equals_mix(small_graph_words):
     graphs = _pool()
     expect = [O.tc_merge(g) for g in graphs]
-    srv = wtc.Server(0, hbm_budget_bytes=256 << 20, max_batch=16, slots=2)
+    srv = wtc.Server(0, hbm_budget_bytes=256 << 20, max_batch=16, slots=2, small_graph_words=small_graph_words)
     ids = [srv.register(g) for g in graphs]
     # alone: one request at a time
     alone = []

@@ -31,7 +31,7 @@ struct wtc_batch {
     wtc::SetPlan plan;
     std::vector<uint32_t> key_s, key_a;
     int key_ways = 0, key_win = 0;
-    uint32_t key_chunk = 0;
+    uint32_t key_chunk = 0, key_small = 0;
 };
 
 namespace wtc {
@@ -134,17 +134,50 @@ GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint
     return d;
 }
 
+// Blocks of the small-graph kernel for a graph of W wedges (W < 2^31).
+static uint32_t small_parts_of(uint64_t W) {
+    const uint64_t p = (W + kSmallBlockWedges - 1) / kSmallBlockWedges;
+    return p < 1 ? 1u : (uint32_t)(p > 65535 ? 65535 : p);
+}
+
 static int ways_index(int w) { return w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3; }
 static int win_index(int v) { return v == 8 ? 1 : v == 16 ? 2 : 0; }
 
-int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan) {
+int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan, const uint64_t *m,
+              const uint64_t *W) {
     const uint32_t G = (uint32_t)hd.size();
     if (G == 0) return WTC_ERR_ARG;
+    // Kernel choice per set: a set of two or more graphs that all fit the
+    // small-graph limit is counted by the small-graph kernel (one block per
+    // graph, CSR in LDS, so no bitmap); any other set by the wedge-parallel
+    // kernel.  Mixing the two in one set runs them back to back, and the
+    // small kernel's per-graph tail then costs more than it saves.
+    uint32_t small_words = 0, small_n = 0;
+    uint64_t small_parts = 0;
+    const uint32_t small_cap = std::min(t.small_words, t.small_words_max);
+    bool all_small = m && W && G >= 2 && small_cap > 0;
+    for (uint32_t g = 0; g < G && all_small; g++)
+        all_small = hd[g].n > 0 && 2 * ((uint64_t)hd[g].n + 1) + m[g] <= small_cap;
+    for (uint32_t g = 0; g < G; g++) {
+        GraphDesc &d = hd[g];
+        d.mode = kModeTiles;
+        if (!all_small) continue;
+        const uint64_t words = 2 * ((uint64_t)d.n + 1) + m[g];
+        d.mode = kModeSmall;
+        d.a = 0;
+        d.bm_rows = 0;
+        d.thr = d.n;
+        d.bm_size = 0;
+        d.bitmap = nullptr;
+        small_n++;
+        small_parts += small_parts_of(W[g]);
+        small_words = std::max(small_words, (uint32_t)words);
+    }
     uint32_t max_spread = 1;
     uint64_t ws_n = 0, bm_words = 0, rows = 0, blocks = 0;
     for (const auto &d : hd) {
         if (d.n == 0 || d.spread == 0) return WTC_ERR_ARG;
-        max_spread = std::max(max_spread, d.spread);
+        if (d.mode == kModeTiles) max_spread = std::max(max_spread, d.spread);
         if (!d.ws) ws_n += (uint64_t)d.n + 1;
         if (d.a && !d.bitmap) bm_words += bm_words_for(d.a);
         rows += d.bm_rows;
@@ -155,10 +188,13 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     if (ws_n) WTC_HIPC(s.ws.ensure(ws_n * sizeof(uint64_t)));
     WTC_HIPC(s.partials.ensure(blocks * sizeof(uint64_t)));
     if (single) WTC_HIPC(s.tree.ensure(kTreeSize * sizeof(uint64_t)));
-    WTC_HIPC(s.desc.ensure((size_t)G * sizeof(GraphDesc)));
+    // descriptors, then the block list of the small-graph kernel
+    if (small_parts > 0x7fffffffULL) return WTC_ERR_RANGE;
+    const size_t desc_bytes = (size_t)G * sizeof(GraphDesc) + (size_t)small_parts * sizeof(SmallPart);
+    WTC_HIPC(s.desc.ensure(desc_bytes));
     WTC_HIPC(s.out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
     if (bm_words) WTC_HIPC(s.bm.ensure(bm_words * sizeof(uint32_t)));
-    WTC_HIPC(s.h_desc.ensure((size_t)G * sizeof(GraphDesc)));
+    WTC_HIPC(s.h_desc.ensure(desc_bytes));
     WTC_HIPC(s.h_out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
     uint64_t ws_off = 0, bm_off = 0, row_off = 0, blk_off = 0, clear_off = 0;
     for (auto &d : hd) {
@@ -178,6 +214,12 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
         blk_off += prefix_blocks(d.n);
     }
     std::memcpy(s.h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
+    SmallPart *h_parts = reinterpret_cast<SmallPart *>(s.h_desc.as<GraphDesc>() + G);
+    for (uint32_t g = 0, k = 0; g < G; g++) {
+        if (hd[g].mode != kModeSmall) continue;
+        const uint32_t np = small_parts_of(W[g]);
+        for (uint32_t p = 0; p < np; p++) h_parts[k++] = SmallPart{g, (uint16_t)p, (uint16_t)np};
+    }
 
     const int oi = ob == 8 ? 1 : 0;
     const uint32_t staged = std::min<uint32_t>(max_spread, t.chunk);
@@ -185,6 +227,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     if (!gc) WTC_HIPC(count_grid(ob, staged, t.ways, t.win, t.num_cus, &gc));
     s.G = G;
     s.grid = gc;
+    s.small = small_n;
     plan->G = G;
     plan->ob = ob;
     plan->blocks = (uint32_t)blocks;
@@ -195,6 +238,10 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->ways = t.ways;
     plan->win = t.win;
     plan->chunk = t.chunk;
+    plan->desc_bytes = desc_bytes;
+    plan->small_count = (uint32_t)small_parts;
+    plan->tile_graphs = G - small_n;
+    plan->small_lds = std::min((small_words * 4u + 255u) & ~255u, t.small_words_max * 4u);
     return WTC_OK;
 }
 
@@ -209,21 +256,38 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
     if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
     WTC_HIPC(launch_bitmap(ddesc, p.G, p.ob, p.rows, p.clear_words, s.stream));
     if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
-    WTC_HIPC(launch_count(ddesc, p.G, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways, p.win, s.grid,
-                          s.stream));
+    WTC_HIPC(launch_count_small(ddesc, reinterpret_cast<const SmallPart *>(ddesc + p.G), p.small_count, p.ob,
+                                dcounts, p.small_lds, s.stream));
+    if (p.tile_graphs)  // a set counted wholly by the small-graph kernel has no tiles
+        WTC_HIPC(launch_count(ddesc, p.G, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways, p.win, s.grid,
+                              s.stream));
     if (timed) WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
     WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + p.G) * sizeof(uint64_t), hipMemcpyDeviceToHost,
                             s.stream));
     return WTC_OK;
 }
 
-int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd) {
+int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, const uint64_t *m, const uint64_t *W) {
     SetPlan p;
-    int rc = stage_set(s, t, ob, hd, &p);
+    int rc = stage_set(s, t, ob, hd, &p, m, W);
     if (rc) return rc;
-    WTC_HIPC(hipMemcpyAsync(s.desc.p, s.h_desc.p, (size_t)p.G * sizeof(GraphDesc), hipMemcpyHostToDevice,
+    WTC_HIPC(hipMemcpyAsync(s.desc.p, s.h_desc.p, p.desc_bytes, hipMemcpyHostToDevice,
                             s.stream));
     return launch_set(s, p, true);
+}
+
+uint32_t default_small_words() {
+    const char *e = std::getenv("WTC_SMALL_GRAPH_WORDS");
+    if (!e) return kSmallWordsDefault;
+    const long v = std::atol(e);
+    return v < 0 ? 0u : (uint32_t)std::min<long>(v, kSmallWordsMax);
+}
+
+int init_small_graphs(Tuning &t, const hipDeviceProp_t &prop, uint32_t want) {
+    t.small_words_max = (uint32_t)std::min<size_t>(prop.sharedMemPerBlock / 4, kSmallWordsMax);
+    t.small_words = std::min(want, t.small_words_max);
+    const hipError_t e = allow_small_lds(t.small_words_max * 4u);
+    return e == hipSuccess ? WTC_OK : hip_fail("small-graph kernel LDS limit", e);
 }
 
 bool slot_done(Slot &s) { return hipStreamQuery(s.stream) == hipSuccess; }
@@ -277,9 +341,9 @@ int set_device(int dev) {
 
 // Synchronous count of a graph set on the context's slot.
 int run_set(wtc_ctx *c, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, wtc_count_stats *st,
-            RunTimes *rt_out = nullptr) {
+            RunTimes *rt_out = nullptr, const uint64_t *m = nullptr, const uint64_t *wedges = nullptr) {
     auto t0 = std::chrono::steady_clock::now();
-    int rc = enqueue_set(c->slot, c->tune, ob, hd);
+    int rc = enqueue_set(c->slot, c->tune, ob, hd, m, wedges);
     if (rc) return rc;
     RunTimes rt;
     uint64_t W = 0;
@@ -295,6 +359,7 @@ int run_set(wtc_ctx *c, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, wt
         st->wedges = W;
         st->blocks = c->slot.grid;
         st->launches = 1;
+        st->small_graphs = c->slot.small;
     }
     return WTC_OK;
 }
@@ -408,11 +473,28 @@ int wtc_ctx_create(int device, wtc_ctx **out) {
     c->tune.num_cus = prop.multiProcessorCount;
     c->tune.ways = default_probe_ways();
     c->tune.win = default_probe_window();
+    if ((rc = init_small_graphs(c->tune, prop, default_small_words()))) {
+        delete c;
+        return rc;
+    }
     if ((rc = c->slot.create())) {
         wtc_ctx_destroy(c);
         return rc;
     }
     *out = c;
+    return WTC_OK;
+}
+
+int wtc_ctx_set_small_graphs(wtc_ctx *c, uint32_t max_words) {
+    if (!c || max_words > c->tune.small_words_max) return WTC_ERR_ARG;
+    c->tune.small_words = max_words;
+    return WTC_OK;
+}
+
+int wtc_ctx_get_small_graphs(const wtc_ctx *c, uint32_t *max_words, uint32_t *device_max_words) {
+    if (!c) return WTC_ERR_ARG;
+    if (max_words) *max_words = c->tune.small_words;
+    if (device_max_words) *device_max_words = c->tune.small_words_max;
     return WTC_OK;
 }
 
@@ -770,10 +852,10 @@ int batch_capture(wtc_ctx *c, wtc_batch *b, const std::vector<uint32_t> &s, cons
         if (rc) return rc;
     }
     std::vector<GraphDesc> hd = batch_descs(b, s, a);
-    int rc = stage_set(b->slot, c->tune, b->ob, hd, &b->plan);
+    int rc = stage_set(b->slot, c->tune, b->ob, hd, &b->plan, b->m.data(), b->W.data());
     if (rc) return rc;
     hipStream_t st = b->slot.stream;
-    WTC_HIPC(hipMemcpyAsync(b->slot.desc.p, b->slot.h_desc.p, (size_t)b->G * sizeof(GraphDesc),
+    WTC_HIPC(hipMemcpyAsync(b->slot.desc.p, b->slot.h_desc.p, b->plan.desc_bytes,
                             hipMemcpyHostToDevice, st));
     WTC_HIPC(hipStreamSynchronize(st));
     WTC_HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -795,6 +877,7 @@ int batch_capture(wtc_ctx *c, wtc_batch *b, const std::vector<uint32_t> &s, cons
     b->key_ways = c->tune.ways;
     b->key_win = c->tune.win;
     b->key_chunk = c->tune.chunk;
+    b->key_small = c->tune.small_words;
     return WTC_OK;
 }
 
@@ -812,7 +895,7 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
     int rc = batch_params(b, spread, a, s, aa);
     if (rc) return rc;
     if (!b->exec || s != b->key_s || aa != b->key_a || c->tune.ways != b->key_ways || c->tune.win != b->key_win ||
-        c->tune.chunk != b->key_chunk) {
+        c->tune.chunk != b->key_chunk || c->tune.small_words != b->key_small) {
         if ((rc = batch_capture(c, b, s, aa))) return rc;
     }
     Slot &sl = b->slot;
@@ -832,6 +915,7 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
         st->wedges = h[kMetaWedges];
         st->blocks = sl.grid;
         st->launches = 1;
+        st->small_graphs = sl.small;
     }
     return WTC_OK;
 }
@@ -847,7 +931,7 @@ int wtc_batch_count_host(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t 
     rc = batch_params(b, spread, a, s, aa);
     if (rc == WTC_OK && b->G) {
         std::vector<GraphDesc> hd = batch_descs(b, s, aa);
-        rc = run_set(c, b->ob, hd, triangles, st);
+        rc = run_set(c, b->ob, hd, triangles, st, nullptr, b->m.data(), b->W.data());
     }
     wtc_batch_free(b);
     auto t1 = std::chrono::steady_clock::now();

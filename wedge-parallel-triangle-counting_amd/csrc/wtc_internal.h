@@ -86,6 +86,7 @@ struct Slot {
     PinBuf h_desc, h_out;
     uint32_t G = 0;       // graphs of the last enqueue
     uint32_t grid = 0;    // counting grid of the last enqueue
+    uint32_t small = 0;   // graphs of the last enqueue taken by the small-graph kernel
     int create();
     void release_device();  // frees the device workspace (keeps stream/events)
     void destroy();
@@ -97,8 +98,22 @@ struct Tuning {
     int ways = 4;         // probes advanced together per lane
     int win = 0;          // probe window in elements: 0, 8 or 16
     uint32_t chunk = 16;  // wedges per lane staged in LDS per phase
+    // In sets of two or more graphs, a graph with 2(n+1) + m <= small_words
+    // is counted by the small-graph kernel from LDS (0: never);
+    // small_words_max is the device's LDS per block in words (at most
+    // kSmallWordsMax).
+    uint32_t small_words = 0;
+    uint32_t small_words_max = 0;
     uint32_t grid_cache[2][4][3][kChunkMax + 2] = {};
 };
+
+// Default small-graph limit of new contexts and servers, in LDS words
+// (2(n+1) + m): $WTC_SMALL_GRAPH_WORDS, else kSmallWordsDefault.
+constexpr uint32_t kSmallWordsDefault = 20480;
+uint32_t default_small_words();
+// Sets the tuning's small-graph limits for a device (want clamped to its
+// LDS per block) and raises the kernel's dynamic-LDS limit to match.
+int init_small_graphs(Tuning &t, const hipDeviceProp_t &prop, uint32_t want);
 
 struct RunTimes {
     float prefix_ms = 0, bitmap_ms = 0, count_ms = 0;
@@ -120,10 +135,18 @@ struct SetPlan {
     int ob = 4, ways = 4, win = 0;
     uint64_t rows = 0, clear_words = 0;
     bool single = false;
+    // small_count blocks of the small-graph kernel, whose SmallPart list
+    // follows the descriptors; desc_bytes covers both
+    uint32_t small_count = 0, small_lds = 0;
+    uint32_t tile_graphs = 0;  // graphs counted by the wedge-parallel kernel
+    size_t desc_bytes = 0;
 };
 // Sizes the slot's workspace for the set, assigns each graph its prefix and
 // bitmap (unless given), fills the pinned host descriptors and the plan.
-int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan);
+// m and W (edges and wedges per graph, or null) let a set whose graphs all
+// fit the tuning's small-graph limit take the small-graph kernel.
+int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan,
+              const uint64_t *m = nullptr, const uint64_t *W = nullptr);
 // Enqueues the set's kernels and the copy of meta + counts to the host; no
 // host-to-device copies, allocations or memsets, so it can be captured into
 // a graph once the descriptors are on the device.  timed: record ev[0..3].
@@ -132,7 +155,8 @@ int launch_set(Slot &s, const SetPlan &p, bool timed);
 // pipeline for a graph set on the slot's stream and the copy of its counts
 // into the slot's pinned buffer; returns without waiting.  All graphs of a
 // set share the row-offset width ob.
-int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd);
+int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, const uint64_t *m = nullptr,
+                const uint64_t *W = nullptr);
 // Waits for the slot (or, if it is done, does not wait) and reads results.
 int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt);
 // True when the slot's last enqueue has completed.

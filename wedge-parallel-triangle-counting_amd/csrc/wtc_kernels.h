@@ -15,6 +15,29 @@ constexpr uint32_t kTreeSize = (1u << kTreeLevels) - 1u;
 constexpr int kScanItems = 16;          // vertices per thread in the prefix scan
 constexpr uint32_t kScanChunk = kBlock * kScanItems;  // vertices per scan block
 constexpr uint32_t kSentinel = 0xffffffffu;           // empty LDS wedge slot
+constexpr int kSmallBlock = 512;        // small-graph kernel block: one graph per block
+// Largest LDS footprint, in 32-bit words, of a graph the small-graph kernel
+// takes: 2(n+1) + m <= 2^16 keeps m < 2^16, so W <= C(m,2) < 2^31 and every
+// wedge index of such a graph fits 32 bits.
+constexpr uint32_t kSmallWordsMax = 1u << 16;
+
+// Wedges per block the host aims at when it splits a graph of a small-graph
+// set over several blocks of the small-graph kernel.
+constexpr uint64_t kSmallBlockWedges = 16384;
+
+// One block of the small-graph kernel: wedges [wcount*part/parts,
+// wcount*(part+1)/parts) of graph g's range.
+struct SmallPart {
+    uint32_t g;
+    uint16_t part, parts;
+};
+static_assert(sizeof(SmallPart) == 8, "SmallPart layout");
+
+// GraphDesc::mode
+enum : uint32_t {
+    kModeTiles = 0,  // counted by k_count in tiles of 64*s wedges
+    kModeSmall = 1,  // counted by k_count_small, one block with the CSR in LDS
+};
 
 // One graph of a graph set.  A graph set is any number of oriented CSRs
 // resident in HBM (ids local to each graph, row offsets relative to the
@@ -38,7 +61,7 @@ struct GraphDesc {
     uint32_t a;              // bitmap length (top-a ranks), multiple of 32
     uint32_t thr;            // bitmap threshold max(n,a) - a (a = 0: n)
     uint32_t bm_rows;        // min(n, a)
-    uint32_t pad_;
+    uint32_t mode;           // kModeTiles or kModeSmall
     // device-filled by the setup kernel
     uint64_t wbase;          // first counted wedge (graph-local index)
     uint64_t wcount;         // wedges counted

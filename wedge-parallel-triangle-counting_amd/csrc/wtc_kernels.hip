@@ -86,6 +86,7 @@ __global__ __launch_bounds__(kBlock) void k_wedge_scan(const GraphDesc *__restri
     uint64_t carry = partials[blockIdx.x] - partials[D.blk_base];  // graph-local prefix
     const uint64_t base = lb * kScanChunk;
     for (int k = 0; k < kScanItems; k++) {
+        if (base + (uint64_t)k * kBlock >= n) break;  // block-uniform: past the graph's last vertex
         const uint64_t v = base + (uint64_t)k * kBlock + threadIdx.x;
         const uint64_t x = v < n ? binom2_d((uint64_t)(rowPtr[v + 1] - rowPtr[v])) : 0;
         uint64_t tot;
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_setup(GraphDesc *__restrict__ desc, 
             D.wbase = b;
             D.wcount = wc;
             const uint64_t per = (uint64_t)kWave * D.spread;
-            tiles = (wc + per - 1) / per;
+            tiles = D.mode == kModeSmall ? 0 : (wc + per - 1) / per;  // small graphs: k_count_small
             counts[g] = 0;
         }
         uint64_t tot, wtot;
@@ -646,6 +647,168 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
     }
 }
 
+// ------------------------------------------------------ small-graph count
+
+// Independent wedge streams each lane of k_count_small advances together.
+constexpr int kSmallWays = 4;
+
+// The same count for a graph small enough to keep in LDS (the role of the
+// vertex- and edge-parallel kernels of experiments/tc_vertex.cu:113-274 and
+// experiments/tc_edge.cu:179-352 for small graphs): one block per graph
+// copies its rowPtr, wedge prefix and colInd into LDS once, then every probe
+// is an LDS read instead of an L1 lookup, and no tile bookkeeping is paid.
+// Block b takes part parts[b] of a graph of mode kModeSmall: a slice of its
+// wedge range (graphs with many wedges are split over several blocks, each
+// staging its own copy).  The CSR is staged with eight loads in flight per
+// lane.  The block's wedge slice is cut into kSmallBlock * kSmallWays
+// contiguous streams; lane t advances streams t, t + kSmallBlock, ... in
+// lockstep so that their LDS round trips overlap.  A stream's first wedge is
+// located by a search of the prefix and decoded exactly, the rest by
+// stepping.  The probe of u in adj+(w) is a lower-bound search that starts
+// from the previous probe's position while w stays (u only grows along a
+// row); a present u is always probed once (the last probe at or above u is
+// the lower bound), so a hit is counted when seen.  Every loop exits by a
+// wave-wide vote, so lanes never wait in nested per-lane loops.  Counts of
+// a mode-small graph need 2(n+1) + m <= kSmallWordsMax words of LDS, which
+// the host checks against the launch's dynamic LDS size.
+template <typename OffT>
+__global__ __launch_bounds__(kSmallBlock) void k_count_small(const GraphDesc *__restrict__ desc,
+                                                             const SmallPart *__restrict__ parts,
+                                                             unsigned long long *__restrict__ counts) {
+    extern __shared__ uint32_t lds[];
+    const SmallPart sp = parts[blockIdx.x];
+    const uint32_t g = sp.g;
+    const GraphDesc &D = desc[g];
+    // this block's slice of the graph's range [wbase, wbase + wcount); wb + wc <= W < 2^31
+    const uint32_t wtot = (uint32_t)D.wcount;
+    const uint32_t s0 = (uint32_t)((uint64_t)wtot * sp.part / sp.parts);
+    const uint32_t s1 = (uint32_t)((uint64_t)wtot * (sp.part + 1u) / sp.parts);
+    if (s1 == s0) return;  // block-uniform
+    const uint32_t n = D.n;
+    const gptr<OffT> rowPtr = to_global(static_cast<const OffT *>(D.rowPtr));
+    const gptr<uint64_t> ws = to_global(static_cast<const uint64_t *>(D.ws));
+    const OffT base = rowPtr[0];
+    const uint32_t m = (uint32_t)(rowPtr[n] - base);
+    const gptr<uint32_t> colInd = to_global(D.colInd) + base;
+    uint32_t *const rp = lds;              // n+1 row offsets relative to rowPtr[0]
+    uint32_t *const wp = lds + (n + 1);    // n+1 wedge prefix; W <= C(m,2) < 2^31
+    uint32_t *const col = lds + 2 * (n + 1);
+    constexpr uint32_t kStage = 8;  // loads in flight per lane
+    for (uint32_t i0 = 0; i0 <= n; i0 += kStage * kSmallBlock) {
+        OffT r[kStage];
+        uint64_t x[kStage];
+#pragma unroll
+        for (uint32_t q = 0; q < kStage; q++) {
+            const uint32_t i = i0 + q * kSmallBlock + threadIdx.x;
+            r[q] = i <= n ? rowPtr[i] : base;
+            x[q] = i <= n ? ws[i] : 0;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kStage; q++) {
+            const uint32_t i = i0 + q * kSmallBlock + threadIdx.x;
+            if (i <= n) {
+                rp[i] = (uint32_t)(r[q] - base);
+                wp[i] = (uint32_t)x[q];
+            }
+        }
+    }
+    for (uint32_t i0 = 0; i0 < m; i0 += kStage * kSmallBlock) {
+        uint32_t c[kStage];
+#pragma unroll
+        for (uint32_t q = 0; q < kStage; q++) {
+            const uint32_t i = i0 + q * kSmallBlock + threadIdx.x;
+            c[q] = i < m ? colInd[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kStage; q++) {
+            const uint32_t i = i0 + q * kSmallBlock + threadIdx.x;
+            if (i < m) col[i] = c[q];
+        }
+    }
+    __syncthreads();
+
+    // streams of `per` wedges: stream j = k * kSmallBlock + lane covers
+    // [wb + j*per, min(wb + (j+1)*per, wb + wc))
+    const uint32_t wb = (uint32_t)D.wbase + s0, wc = s1 - s0;
+    const uint32_t per = (wc + kSmallBlock * kSmallWays - 1) / (kSmallBlock * kSmallWays);
+    uint32_t i[kSmallWays], iend[kSmallWays], v[kSmallWays], vb[kSmallWays], d[kSmallWays];
+    uint32_t w_i[kSmallWays], u_i[kSmallWays], lo[kSmallWays], hi[kSmallWays];
+#pragma unroll
+    for (int k = 0; k < kSmallWays; k++) {
+        const uint32_t off = (k * kSmallBlock + threadIdx.x) * per;  // < kSmallBlock*kSmallWays*per < 2^32
+        i[k] = iend[k] = 0;
+        v[k] = vb[k] = d[k] = w_i[k] = u_i[k] = lo[k] = hi[k] = 0;
+        if (off < wc) {
+            i[k] = wb + off;
+            iend[k] = wb + (wc - off < per ? wc : off + per);
+            v[k] = rightmost_le(wp, 0u, n - 1, i[k]);
+            vb[k] = rp[v[k]];
+            d[k] = rp[v[k] + 1] - vb[k];
+            decode_wedge(d[k], i[k] - wp[v[k]], w_i[k], u_i[k]);
+            const uint32_t w = col[vb[k] + w_i[k]];
+            lo[k] = rp[w];
+            hi[k] = rp[w + 1];
+        }
+    }
+    uint32_t cnt = 0;
+    for (;;) {
+        bool act[kSmallWays], any = false;
+#pragma unroll
+        for (int k = 0; k < kSmallWays; k++) {
+            act[k] = i[k] < iend[k];
+            any |= act[k];
+        }
+        if (__ballot(any) == 0) break;
+        uint32_t u[kSmallWays], len[kSmallWays];
+#pragma unroll
+        for (int k = 0; k < kSmallWays; k++) {
+            u[k] = act[k] ? col[vb[k] + u_i[k]] : 0u;
+            len[k] = act[k] ? hi[k] - lo[k] : 0u;
+        }
+        for (;;) {
+            bool more = false;
+#pragma unroll
+            for (int k = 0; k < kSmallWays; k++) more |= len[k] > 0;
+            if (__ballot(more) == 0) break;
+            uint32_t x[kSmallWays];
+#pragma unroll
+            for (int k = 0; k < kSmallWays; k++) x[k] = len[k] > 0 ? col[lo[k] + (len[k] >> 1)] : 0u;
+#pragma unroll
+            for (int k = 0; k < kSmallWays; k++) {
+                const uint32_t half = len[k] >> 1;
+                const bool lt = x[k] < u[k];
+                const uint32_t go = len[k] > 0 && lt ? half + 1 : 0u;
+                cnt += (len[k] > 0 && x[k] == u[k]) ? 1u : 0u;
+                lo[k] += go;
+                len[k] = lt ? len[k] - go : half;
+            }
+        }
+        // step every active stream to its next wedge
+#pragma unroll
+        for (int k = 0; k < kSmallWays; k++) {
+            const bool next = act[k] && ++i[k] < iend[k];
+            const bool new_w = next && ++u_i[k] == d[k];
+            if (new_w) {
+                w_i[k]++;
+                u_i[k] = w_i[k] + 1;
+                if (u_i[k] == d[k]) {
+                    // a wedge i < W remains, so a later vertex holds it: v + 2 <= n
+                    v[k] = (wp[v[k] + 2] > i[k]) ? v[k] + 1 : rightmost_le(wp, v[k] + 2, n - 1, i[k]);
+                    vb[k] = rp[v[k]];
+                    d[k] = rp[v[k] + 1] - vb[k];
+                    w_i[k] = 0;
+                    u_i[k] = 1;
+                }
+                const uint32_t w = col[vb[k] + w_i[k]];
+                lo[k] = rp[w];
+                hi[k] = rp[w + 1];
+            }
+        }
+    }
+    const uint64_t tot = wave_sum((uint64_t)cnt);
+    if ((threadIdx.x & (kWave - 1)) == 0 && tot != 0) atomicAdd(&counts[g], (unsigned long long)tot);
+}
+
 // Plain streaming copy (16-byte vectors), the reference kernel for scaling
 // the HBM byte counters of a profiler run.
 __global__ __launch_bounds__(kBlock) void k_copy_u32x4(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
@@ -724,6 +887,21 @@ hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t row
     const uint64_t bb = (rows + kWavesPerBlock - 1) / kWavesPerBlock;
     return WTC_LAUNCH_OFF(ob, k_bitmap_build, dim3((uint32_t)(bb < 8192 ? bb : 8192)), dim3(kBlock), 0, st, desc, G,
                           rows);
+}
+
+hipError_t launch_count_small(const GraphDesc *desc, const SmallPart *parts, uint32_t nparts, int ob,
+                              unsigned long long *counts, uint32_t lds_bytes, hipStream_t st) {
+    if (nparts == 0) return hipSuccess;
+    if (lds_bytes == 0 || lds_bytes > kSmallWordsMax * 4u) return hipErrorInvalidValue;
+    return WTC_LAUNCH_OFF(ob, k_count_small, dim3(nparts), dim3(kSmallBlock), lds_bytes, st, desc, parts, counts);
+}
+
+hipError_t allow_small_lds(uint32_t lds_bytes) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_count_small<uint32_t>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void *>(&k_count_small<uint64_t>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
 size_t count_lds_bytes(uint32_t max_spread) {

@@ -24,6 +24,13 @@ hipError_t launch_setup(GraphDesc *desc, uint32_t G, uint64_t *tree, uint64_t *m
 // out by bm_clear_base) and sets its bits; rows = sum of bm_rows.
 hipError_t launch_bitmap(const GraphDesc *desc, uint32_t G, int ob, uint64_t rows, uint64_t clear_words,
                          hipStream_t st);
+// Counts the mode-small graphs named by parts[0..nparts) with one block per
+// part; lds_bytes (<= 4*kSmallWordsMax) must hold 4*(2(n+1) + m) for each
+// of them.
+hipError_t launch_count_small(const GraphDesc *desc, const SmallPart *parts, uint32_t nparts, int ob,
+                              unsigned long long *counts, uint32_t lds_bytes, hipStream_t st);
+// Raises the small-graph kernel's dynamic-LDS limit to lds_bytes.
+hipError_t allow_small_lds(uint32_t lds_bytes);
 // LDS bytes of a counting block that stages `staged` wedges per lane.
 size_t count_lds_bytes(uint32_t staged);
 // `ways`: independent probes each lane advances together (1, 2, 4 or 8);

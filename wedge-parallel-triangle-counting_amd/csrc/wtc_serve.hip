@@ -93,6 +93,7 @@ struct SrvGraph {
     uint32_t pins = 0;  // admitted requests using it
     uint64_t last_use = 0;
     uint64_t first_rid = UINT64_MAX;  // oldest queued request naming it
+    uint64_t W = 0;                    // wedges
     uint64_t cost = 0;                 // batching cost class: max(W, kCostFloor)
 };
 
@@ -238,6 +239,7 @@ bool schedule(wtc_server *s) {
     for (auto &x : s->slots) {
         if (x.busy || s->queue.empty()) continue;
         std::vector<GraphDesc> hd;
+        std::vector<uint64_t> hm, hw;  // edges and wedges per graph of hd
         uint64_t c0 = 0;
         for (auto it = s->queue.begin(); it != s->queue.end() && hd.size() < s->cfg.max_batch;) {
             SrvReq *r = *it;
@@ -270,13 +272,15 @@ bool schedule(wtc_server *s) {
             d.ws = reinterpret_cast<uint64_t *>(pool_ptr(s, r->ws_off));
             d.bitmap = r->bm_bytes ? reinterpret_cast<uint32_t *>(pool_ptr(s, r->bm_off)) : nullptr;
             hd.push_back(d);
+            hm.push_back(g->m);
+            hw.push_back(g->W);
             r->state = kRunning;
             r->t_start = Clock::now();
             x.reqs.push_back(r);
             it = s->queue.erase(it);
         }
         if (hd.empty()) break;
-        const int rc = enqueue_set(x.slot, s->tune, 4, hd);
+        const int rc = enqueue_set(x.slot, s->tune, 4, hd, hm.data(), hw.data());
         x.batch_id = ++s->batches;
         for (SrvReq *r : x.reqs) {
             r->batch_graphs = (uint32_t)hd.size();
@@ -338,6 +342,7 @@ int wtc_server_create(int device, const wtc_server_config *cfg, wtc_server **out
     s->cfg.probe_ways = cfg && cfg->probe_ways ? cfg->probe_ways : 4;
     s->cfg.lds_chunk = cfg && cfg->lds_chunk ? cfg->lds_chunk : 16;
     s->cfg.probe_window = cfg && cfg->probe_window ? cfg->probe_window : -1;
+    s->cfg.small_graph_words = cfg ? cfg->small_graph_words : 0;
     if (s->cfg.probe_window != -1 && s->cfg.probe_window != 8 && s->cfg.probe_window != 16) {
         delete s;
         return WTC_ERR_ARG;
@@ -352,7 +357,15 @@ int wtc_server_create(int device, const wtc_server_config *cfg, wtc_server **out
     s->tune.ways = s->cfg.probe_ways;
     s->tune.chunk = s->cfg.lds_chunk;
     s->tune.win = s->cfg.probe_window > 0 ? s->cfg.probe_window : 0;
-    int rc = WTC_OK;
+    int rc = init_small_graphs(s->tune, prop, s->cfg.small_graph_words < 0    ? 0u
+                                              : s->cfg.small_graph_words == 0 ? default_small_words()
+                                                                              : (uint32_t)s->cfg.small_graph_words);
+    if (!rc && s->cfg.small_graph_words > 0 && (uint32_t)s->cfg.small_graph_words > s->tune.small_words_max)
+        rc = WTC_ERR_ARG;
+    if (rc) {
+        delete s;
+        return rc;
+    }
     if ((e = s->pool_mem.ensure(s->cfg.hbm_budget_bytes)) != hipSuccess)
         rc = hip_fail("server pool", e);
     if (!rc && (e = hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking)) != hipSuccess)
@@ -397,6 +410,7 @@ int wtc_server_register(wtc_server *s, const wtc_big_graph *g, uint64_t *graph_i
         rp[v] = (uint32_t)g->rowPtr[v];
         if (v < sg->n) W += binom2(g->rowPtr[v + 1] - g->rowPtr[v]);
     }
+    sg->W = W;
     sg->cost = std::max<uint64_t>(W, kCostFloor);
     if (sg->m) std::memcpy(rp + sg->n + 1, g->colInd, sg->m * 4);
     std::lock_guard<std::mutex> lk(s->mu);

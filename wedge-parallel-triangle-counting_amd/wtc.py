@@ -63,7 +63,7 @@ class GpuTime(C.Structure):
 class CountStats(C.Structure):
     _fields_ = [("prefix_ms", C.c_double), ("bitmap_ms", C.c_double), ("count_ms", C.c_double),
                 ("total_ms", C.c_double), ("wedges", C.c_uint64), ("blocks", C.c_uint32),
-                ("launches", C.c_uint32)]
+                ("launches", C.c_uint32), ("small_graphs", C.c_uint32), ("reserved", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -72,7 +72,7 @@ class CountStats(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("slots", C.c_uint32),
                 ("queue_capacity", C.c_uint32), ("probe_ways", C.c_int32), ("lds_chunk", C.c_uint32),
-                ("probe_window", C.c_int32)]
+                ("probe_window", C.c_int32), ("small_graph_words", C.c_int32)]
 
 
 class RequestStats(C.Structure):
@@ -134,6 +134,8 @@ _sig = {
     "wtc_debug_decode": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _P]),
     "wtc_debug_copy": (C.c_int, [_P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "wtc_ctx_set_tuning": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_int]),
+    "wtc_ctx_set_small_graphs": (C.c_int, [_P, C.c_uint32]),
+    "wtc_ctx_get_small_graphs": (C.c_int, [_P, _P, _P]),
     "wtc_alg_bytes": (None, [_BGP, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "wtc_batch_upload": (C.c_int, [_P, C.POINTER(_BGP), C.c_size_t, C.POINTER(_P)]),
     "wtc_batch_free": (None, [_P]),
@@ -380,6 +382,17 @@ class Context:
         per lane staged in LDS per phase, probe window (0, 8 or 16)."""
         _check(_lib.wtc_ctx_set_tuning(self._p, probe_ways, lds_chunk, probe_window), "set_tuning")
 
+    def set_small_graphs(self, max_words: int):
+        """In counts over two or more graphs, graphs with 2(n+1) + m <=
+        max_words are counted from LDS by the small-graph kernel (0: off)."""
+        _check(_lib.wtc_ctx_set_small_graphs(self._p, max_words), "set_small_graphs")
+
+    def small_graphs(self):
+        """(current small-graph limit, device maximum), in LDS words."""
+        cur, dev = C.c_uint32(), C.c_uint32()
+        _check(_lib.wtc_ctx_get_small_graphs(self._p, C.byref(cur), C.byref(dev)), "get_small_graphs")
+        return int(cur.value), int(dev.value)
+
     def debug_copy(self, nbytes: int, reps: int = 1) -> float:
         ms = C.c_double()
         _check(_lib.wtc_debug_copy(self._p, nbytes, reps, C.byref(ms)), "debug_copy")
@@ -522,8 +535,10 @@ class Server:
     """Request loop with continuous batching over device-resident CSRs."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, max_batch: int = 256, slots: int = 2,
-                 queue_capacity: int = 65536, probe_ways: int = 0, lds_chunk: int = 0, probe_window: int = 0):
-        cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, queue_capacity, probe_ways, lds_chunk, probe_window)
+                 queue_capacity: int = 65536, probe_ways: int = 0, lds_chunk: int = 0, probe_window: int = 0,
+                 small_graph_words: int = 0):
+        cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, queue_capacity, probe_ways, lds_chunk, probe_window,
+                           small_graph_words)
         self._p = C.c_void_p()
         _check(_lib.wtc_server_create(device, C.byref(cfg), C.byref(self._p)), "server_create")
 
