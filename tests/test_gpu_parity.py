@@ -291,3 +291,36 @@ def test_small_graph_kernel(gpu_ctx):
             gpu_ctx.set_small_graphs(dev_max + 1)
     finally:
         gpu_ctx.set_small_graphs(default)
+
+
+def test_batch_upload_checks_and_reuse(gpu_ctx):
+    """The batch upload checks every CSR while packing it (a graph that is not
+    an oriented CSR fails the whole call with WTC_ERR_CSR, and the context
+    stays usable), and repeated host-batch counts of different sizes reuse
+    the context's device buffers with exact results."""
+    graphs = [wtc.preprocess(wtc.gen_barabasi_albert(200 + 300 * i, 2 + i % 5, seed=900 + i)) for i in range(12)]
+    expect = [O.tc_merge(g) for g in graphs]
+    good = graphs[3]
+    bad_cases = []
+    ci = good.col_ind.copy()
+    r0, r1 = int(good.row_ptr[5]), int(good.row_ptr[6])
+    assert r1 - r0 >= 2
+    ci[r0], ci[r0 + 1] = ci[r0 + 1], ci[r0]                    # a row out of order
+    bad_cases.append(wtc.CSR(good.n, good.m, good.row_ptr.copy(), ci))
+    ci = good.col_ind.copy()
+    ci[r0] = 5                                                  # an id not above its row
+    bad_cases.append(wtc.CSR(good.n, good.m, good.row_ptr.copy(), ci))
+    ci = good.col_ind.copy()
+    ci[-1] = good.n                                             # an id out of range
+    bad_cases.append(wtc.CSR(good.n, good.m, good.row_ptr.copy(), ci))
+    for bad in bad_cases:
+        for where in (0, 6, 12):
+            mix = graphs[:where] + [bad] + graphs[where:]
+            with pytest.raises(wtc.WtcError) as ei:
+                wtc.batch_count_host(gpu_ctx, mix)
+            assert ei.value.code == -10
+            with pytest.raises(wtc.WtcError):
+                wtc.Batch(gpu_ctx, mix)
+        assert wtc.batch_count_host(gpu_ctx, graphs).tolist() == expect
+    for k in (12, 3, 12, 1, 7):
+        assert wtc.batch_count_host(gpu_ctx, graphs[:k]).tolist() == expect[:k], k

@@ -512,6 +512,8 @@ void wtc_ctx_destroy(wtc_ctx *c) {
     (void)hipSetDevice(c->device);
     c->slot.destroy();
     c->h_stage.release();
+    c->batch_row.release();
+    c->batch_col.release();
     delete c;
 }
 
@@ -718,12 +720,51 @@ void wtc_autotune_batch(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *sprea
     autotune_params(n, m, wedges, true, spread, a);
 }
 
-int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, wtc_batch **out) {
-    if (!c || !out || (k && !graphs) || k > 0xffffffffULL) return WTC_ERR_ARG;
-    *out = nullptr;
-    wtc_batch *b = new (std::nothrow) wtc_batch();
-    if (!b) return WTC_ERR_NOMEM;
-    b->ctx = c;
+}  // extern "C"
+
+namespace {
+
+// Copies graph gr into pinned staging (row offsets at element ro in the
+// set's width ob, colInd at element co), checks that it is an oriented CSR
+// (rows strictly ascending, every id above its row and below n) and returns
+// its wedge count: one pass over the graph's data.
+int pack_graph(const wtc_big_graph *gr, int ob, char *row_stage, uint32_t *col_stage, uint64_t ro, uint64_t co,
+               uint64_t *W) {
+    if (!gr || !gr->rowPtr || (gr->numEdges && !gr->colInd)) return WTC_ERR_ARG;
+    const uint64_t n = gr->numVertices, m = gr->numEdges;
+    const uint64_t *rp = gr->rowPtr;
+    const uint32_t *ci = gr->colInd;
+    if (rp[0] != 0 || rp[n] != m) return WTC_ERR_CSR;
+    if (m) std::memcpy(col_stage + co, ci, m * 4);
+    uint32_t *r32 = ob == 4 ? reinterpret_cast<uint32_t *>(row_stage) + ro : nullptr;
+    if (ob == 8) std::memcpy(reinterpret_cast<uint64_t *>(row_stage) + ro, rp, (n + 1) * 8);
+    uint64_t w = 0;
+    for (uint64_t v = 0; v < n; v++) {
+        const uint64_t s = rp[v], t = rp[v + 1];
+        if (t < s || t > m) return WTC_ERR_CSR;
+        if (r32) r32[v] = (uint32_t)s;
+        w += binom2(t - s);
+        uint64_t prev = v;
+        for (uint64_t j = s; j < t; j++) {
+            const uint64_t x = ci[j];
+            if (x <= prev || x >= n) return WTC_ERR_CSR;
+            prev = x;
+        }
+    }
+    if (r32) r32[n] = (uint32_t)m;
+    *W = w;
+    return WTC_OK;
+}
+
+// Bytes of packed graphs after which a host-to-device copy is started, so
+// that packing the next graphs overlaps the copy of the previous ones.
+constexpr uint64_t kUploadChunkBytes = 256ull << 20;
+
+// Packs k host CSRs into b's device rowPtr/colInd (grown as needed): sizes
+// and offsets, then one pass per graph (copy into the context's pinned
+// staging + check + wedge count) in chunks whose copies to the device run
+// while the next chunk is packed.
+int batch_fill(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, wtc_batch *b) {
     b->G = (uint32_t)k;
     b->n.resize(k);
     b->m.resize(k);
@@ -732,10 +773,7 @@ int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, w
     b->W.resize(k);
     uint64_t R = 0, M = 0, max_m = 0;
     for (size_t g = 0; g < k; g++) {
-        if (!graphs[g] || graphs[g]->numVertices == 0) {
-            delete b;
-            return WTC_ERR_ARG;
-        }
+        if (!graphs[g] || graphs[g]->numVertices == 0) return WTC_ERR_ARG;
         b->n[g] = graphs[g]->numVertices;
         b->m[g] = graphs[g]->numEdges;
         b->row_off[g] = R;
@@ -744,55 +782,59 @@ int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, w
         M += graphs[g]->numEdges;
         max_m = std::max<uint64_t>(max_m, graphs[g]->numEdges);
     }
-    std::atomic<int> bad{0};
-    par_chunks(k, 1, [&](uint64_t g0, uint64_t g1, int) {
-        for (uint64_t g = g0; g < g1; g++) {
-            if (validate_oriented_serial(graphs[g]) != WTC_OK) bad = 1;
-            uint64_t W = 0;
-            for (uint64_t v = 0; v < graphs[g]->numVertices; v++)
-                W += binom2(graphs[g]->rowPtr[v + 1] - graphs[g]->rowPtr[v]);
-            b->W[g] = W;
-        }
-    });
-    if (bad) {
-        delete b;
-        return WTC_ERR_CSR;
-    }
     b->ob = max_m < 0xffffffffULL ? 4 : 8;
     const int ob = b->ob;
-    hipError_t e = hipSetDevice(c->device);
+    WTC_HIPC(hipSetDevice(c->device));
     const size_t rbytes = R * (size_t)ob, cbytes = std::max<uint64_t>(M, 1) * 4;
-    if (e == hipSuccess) e = b->rowPtr.ensure(rbytes);
-    if (e == hipSuccess) e = b->colInd.ensure(cbytes);
-    if (e == hipSuccess) e = c->h_stage.ensure(rbytes + cbytes);
-    if (e != hipSuccess) {
-        wtc_batch_free(b);
-        return hip_fail("batch alloc", e);
-    }
-    // pack every graph into pinned staging (row offsets narrowed when they fit)
+    WTC_HIPC(b->rowPtr.ensure(rbytes));
+    WTC_HIPC(b->colInd.ensure(cbytes));
+    WTC_HIPC(c->h_stage.ensure(rbytes + cbytes));
     char *stage = c->h_stage.as<char>();
     uint32_t *sc = reinterpret_cast<uint32_t *>(stage + rbytes);
-    par_chunks(k, 1, [&](uint64_t g0, uint64_t g1, int) {
-        for (uint64_t g = g0; g < g1; g++) {
-            const wtc_big_graph *gr = graphs[g];
-            const uint64_t ro = b->row_off[g];
-            if (ob == 8) {
-                std::memcpy(reinterpret_cast<uint64_t *>(stage) + ro, gr->rowPtr,
-                            ((uint64_t)gr->numVertices + 1) * 8);
-            } else {
-                uint32_t *dst = reinterpret_cast<uint32_t *>(stage) + ro;
-                for (uint64_t v = 0; v <= gr->numVertices; v++) dst[v] = (uint32_t)gr->rowPtr[v];
-            }
-            if (gr->numEdges) std::memcpy(sc + b->col_off[g], gr->colInd, gr->numEdges * 4);
-        }
-    });
     hipStream_t st = c->slot.stream;
-    e = hipMemcpyAsync(b->rowPtr.p, stage, rbytes, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && M) e = hipMemcpyAsync(b->colInd.p, sc, M * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) {
+    std::atomic<int> bad{WTC_OK};
+    hipError_t e = hipSuccess;
+    for (size_t g0 = 0; g0 < k && e == hipSuccess;) {
+        size_t g1 = g0;
+        uint64_t bytes = 0;
+        while (g1 < k && (g1 == g0 || bytes < kUploadChunkBytes)) {
+            bytes += (b->n[g1] + 1) * (uint64_t)ob + b->m[g1] * 4;
+            g1++;
+        }
+        par_chunks(g1 - g0, 1, [&](uint64_t i0, uint64_t i1, int) {
+            for (uint64_t i = g0 + i0; i < g0 + i1; i++) {
+                const int rc = pack_graph(graphs[i], ob, stage, sc, b->row_off[i], b->col_off[i], &b->W[i]);
+                if (rc) bad = rc;
+            }
+        });
+        const uint64_t r0 = b->row_off[g0], r1 = b->row_off[g1 - 1] + b->n[g1 - 1] + 1;
+        const uint64_t c0 = b->col_off[g0], c1 = b->col_off[g1 - 1] + b->m[g1 - 1];
+        e = hipMemcpyAsync(b->rowPtr.as<char>() + r0 * ob, stage + r0 * ob, (r1 - r0) * ob, hipMemcpyHostToDevice,
+                           st);
+        if (e == hipSuccess && c1 > c0)
+            e = hipMemcpyAsync(b->colInd.as<uint32_t>() + c0, sc + c0, (c1 - c0) * 4, hipMemcpyHostToDevice, st);
+        g0 = g1;
+    }
+    const hipError_t es = hipStreamSynchronize(st);  // the staging is reused by the next call
+    if (e == hipSuccess) e = es;
+    if (e != hipSuccess) return hip_fail("batch upload", e);
+    return bad.load();
+}
+
+}  // namespace
+
+extern "C" {
+
+int wtc_batch_upload(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, wtc_batch **out) {
+    if (!c || !out || (k && !graphs) || k > 0xffffffffULL) return WTC_ERR_ARG;
+    *out = nullptr;
+    wtc_batch *b = new (std::nothrow) wtc_batch();
+    if (!b) return WTC_ERR_NOMEM;
+    b->ctx = c;
+    const int rc = batch_fill(c, graphs, k, b);
+    if (rc) {
         wtc_batch_free(b);
-        return hip_fail("batch upload", e);
+        return rc;
     }
     *out = b;
     return WTC_OK;
@@ -923,16 +965,24 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
 int wtc_batch_count_host(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, const uint32_t *spread,
                          const uint32_t *a, uint64_t *triangles, wtc_count_stats *st) {
     auto t0 = std::chrono::steady_clock::now();
-    wtc_batch *b = nullptr;
-    int rc = wtc_batch_upload(c, graphs, k, &b);
-    if (rc) return rc;
+    if (!c || !triangles || (k && !graphs) || k > 0xffffffffULL) return WTC_ERR_ARG;
+    wtc_batch *b = new (std::nothrow) wtc_batch();
+    if (!b) return WTC_ERR_NOMEM;
+    b->ctx = c;
+    // the graphs go into the context's grow-only device buffers, so repeated
+    // calls allocate no device memory once the buffers are large enough
+    b->rowPtr = std::move(c->batch_row);
+    b->colInd = std::move(c->batch_col);
+    int rc = batch_fill(c, graphs, k, b);
     // one pass over freshly uploaded graphs: launched directly, no capture
     std::vector<uint32_t> s, aa;
-    rc = batch_params(b, spread, a, s, aa);
+    if (rc == WTC_OK) rc = batch_params(b, spread, a, s, aa);
     if (rc == WTC_OK && b->G) {
         std::vector<GraphDesc> hd = batch_descs(b, s, aa);
         rc = run_set(c, b->ob, hd, triangles, st, nullptr, b->m.data(), b->W.data());
     }
+    c->batch_row = std::move(b->rowPtr);
+    c->batch_col = std::move(b->colInd);
     wtc_batch_free(b);
     auto t1 = std::chrono::steady_clock::now();
     if (st && rc == WTC_OK) st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();

@@ -180,7 +180,8 @@ struct wtc_ctx {
     int device = 0;
     wtc::Slot slot;
     wtc::Tuning tune;
-    wtc::PinBuf h_stage;
+    wtc::PinBuf h_stage;              // pinned staging of uploads
+    wtc::DevBuf batch_row, batch_col;  // device CSRs of wtc_batch_count_host, kept between calls
 };
 
 struct wtc_dgraph {
