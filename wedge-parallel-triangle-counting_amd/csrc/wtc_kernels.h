@@ -71,7 +71,6 @@ static_assert(sizeof(GraphDesc) == 128, "GraphDesc layout");
 
 // meta[] slots written by the setup kernel
 // (kMetaNext..kMetaNext+7: per-XCD work counters of the counting kernel)
-// (kMetaChunkLog: log2 of the grabs per super-chunk of the XCD interleave)
-enum : int { kMetaTiles = 0, kMetaWedges = 1, kMetaChunkLog = 2, kMetaNext = 8, kMetaSlots = 16 };
+enum : int { kMetaTiles = 0, kMetaWedges = 1, kMetaNext = 8, kMetaSlots = 16 };
 
 }  // namespace wtc

@@ -20,8 +20,13 @@
 
 namespace wtc {
 
-// Tiles a wave of k_count takes per grab from its XCD's work counter.
-constexpr uint64_t kTilesPerGrab = 8;
+// Most tiles a wave of k_count takes per grab from its XCD's work counter.
+// Every grab is one 64-bit atomic on the counter its XCD's waves share, and
+// those atomics are serialised at the L2: with 8 tiles per grab they alone
+// set a floor of ~80 ms on RMAT-24 (54M tiles); with 32, ~20 ms.  Launches
+// with few tiles take smaller grabs (about two per wave), so that a small
+// graph still spreads over many waves.
+constexpr uint64_t kTilesPerGrabMax = 32;
 
 // Rightmost graph g with desc[g].*field <= key (wave-uniform, scalar loads).
 __device__ __forceinline__ uint32_t graph_of(const GraphDesc *__restrict__ desc, uint32_t G, uint64_t key,
@@ -132,12 +137,6 @@ __global__ __launch_bounds__(kBlock) void k_setup(GraphDesc *__restrict__ desc, 
     if (threadIdx.x == 0) {
         meta[kMetaTiles] = carry;
         meta[kMetaWedges] = wsum;
-        // super-chunks of the XCD interleave: 2^k grabs of kTilesPerGrab tiles,
-        // the largest power of two (at most 64 grabs) that still gives every
-        // XCD at least 64 super-chunks
-        uint64_t lg = 0;
-        while (lg < 6 && (carry / kTilesPerGrab) >> (lg + 1) >= 8ull * 64) lg++;
-        meta[kMetaChunkLog] = lg;
     }
     if (threadIdx.x < 8) meta[kMetaNext + threadIdx.x] = 0;  // per-XCD tile counters of k_count
     if (tree != nullptr) {  // single-graph sets only
@@ -295,6 +294,18 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
     u_out[i] = u;
 }
 
+// Value of x in lane src (ds_bpermute), for 64- and 32-bit values; every
+// lane of the wave must execute it.
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, uint32_t src) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)x, (int)src, kWave);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(x >> 32), (int)src, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t shfl_off(uint32_t x, uint32_t src) {
+    return (uint32_t)__shfl((int)x, (int)src, kWave);
+}
+__device__ __forceinline__ uint64_t shfl_off(uint64_t x, uint32_t src) { return shfl_u64(x, src); }
+
 // Four ids loaded by one 16-byte vector load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // The same at any element (4-byte) alignment, which the global memory
@@ -313,18 +324,19 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                                                   const uint64_t *__restrict__ tree, uint64_t *__restrict__ meta,
                                                   unsigned long long *__restrict__ counts, uint32_t pad_max,
                                                   uint32_t chunk) {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ uint2 lds_wu[];
     __shared__ uint32_t s_g[kWavesPerBlock];
     __shared__ uint64_t s_c[kWavesPerBlock];
 
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wid = uni32(threadIdx.x >> 6);
-    uint32_t *const lw = lds + wid * (2u * kWave * pad_max);
-    uint32_t *const lu = lw + kWave * pad_max;
+    // this wave's staging: (w, u) of a staged wedge as one 8-byte entry,
+    // lane l's k-th wedge at l * pad + k
+    uint2 *const lwu = lds_wu + wid * (kWave * pad_max);
 
     // XCD interleave: blocks go round-robin to the 8 XCDs, so blocks with the
     // same blockIdx % 8 share an L2.  The tiles are cut into super-chunks of
-    // 2^lg grabs (kTilesPerGrab consecutive tiles each), dealt to the XCDs in
+    // 2^lg grabs (tpg consecutive tiles each), dealt to the XCDs in
     // turn: XCD x owns super-chunks x, x+8, x+16, ..., so every XCD gets an
     // even mix of cheap and expensive regions of the wedge space (the cost per
     // wedge varies several-fold along it), while each super-chunk stays
@@ -332,7 +344,15 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
     // own counter (dynamic balance); consecutive tiles of a grab let a wave
     // gallop from the vertex where its previous tile ended.
     const uint64_t ntiles = meta[kMetaTiles];
-    const uint32_t lg = (uint32_t)meta[kMetaChunkLog];
+    // grab size: about two grabs per wave, at most kTilesPerGrabMax tiles;
+    // super-chunks of 2^lg grabs, the largest power of two (at most 64) that
+    // still gives every XCD at least 64 super-chunks
+    const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
+    uint64_t tpg = ntiles / (2 * waves);
+    tpg = tpg < 1 ? 1 : tpg > kTilesPerGrabMax ? kTilesPerGrabMax : tpg;
+    const uint64_t ngrabs = (ntiles + tpg - 1) / tpg;
+    uint32_t lg = 0;
+    while (lg < 6 && (ngrabs >> (lg + 1)) >= 8ull * 64) lg++;
     const uint32_t xcd = blockIdx.x & 7u;
     unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaNext + xcd);
 
@@ -344,9 +364,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
         if (lane == 0) grab = atomicAdd(ctr, 1ull);
         const uint64_t gi = uni64(grab);
         const uint64_t chunk_id = xcd + 8 * (gi >> lg);  // this XCD's super-chunk
-        const uint64_t tb = ((chunk_id << lg) + (gi & ((1ull << lg) - 1))) * kTilesPerGrab;
+        const uint64_t tb = ((chunk_id << lg) + (gi & ((1ull << lg) - 1))) * tpg;
         if (tb >= ntiles) break;  // tb grows with gi, so every later grab is past the end too
-        const uint64_t te = tb + kTilesPerGrab < ntiles ? tb + kTilesPerGrab : ntiles;
+        const uint64_t te = tb + tpg < ntiles ? tb + tpg : ntiles;
 
         uint32_t g = kSentinel, v_prev = 0;
         uint64_t g_tile_end = 0;
@@ -385,28 +405,69 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
             const uint64_t tile_end =
                 tile_start + (uint64_t)kWave * s < wend ? tile_start + (uint64_t)kWave * s : wend;
 
-            // wave-uniform vertex range [v_lo, v_hi] of the tile
-            const uint32_t v_lo = have_prev ? gallop_le(ws, v_prev, vmax, tile_start)
-                                  : (tree != nullptr) ? tree_search(tree, ws, gn, tile_start)
-                                                      : rightmost_le(ws, 0u, vmax, tile_start);
-            const uint32_t v_hi = gallop_le(ws, v_lo, vmax, tile_end - 1);
+            // Vertex range [v_lo, v_hi] of the tile and each lane's first
+            // vertex from one coalesced window: wedgeSum and rowPtr of the 64
+            // vertices from `base` (the previous tile's last vertex, or a
+            // search for the first tile of a grab), lane l holding base + l.
+            // wedgeSum is non-decreasing, so ballots of `<= key` are prefixes
+            // whose top bit gives the range; a lane finds its vertex by a
+            // 6-step search over the window's lanes and its row bounds by
+            // shuffles.  A tile that reaches past the window falls back to
+            // searches in global memory.
+            uint32_t base = have_prev ? v_prev
+                            : (tree != nullptr) ? tree_search(tree, ws, gn, tile_start)
+                                                : rightmost_le(ws, 0u, vmax, tile_start);
+            uint64_t wwin, m_lo;
+            OffT rwin;
+            for (;;) {  // ws[base] <= tile_start; at most one skip ahead
+                const uint32_t x = base + lane;
+                wwin = x <= vmax ? ws[x] : ~0ull;
+                rwin = x <= gn ? rowPtr[x] : (OffT)0;
+                m_lo = __ballot(wwin <= tile_start);
+                if (~m_lo != 0) break;
+                base = gallop_le(ws, base + kWave - 1, vmax, tile_start);  // all 64 at or below tile_start
+            }
+            const uint32_t lo_l = 63u - (uint32_t)__clzll(m_lo);
+            const uint64_t m_hi = __ballot(wwin <= tile_end - 1);
+            const bool hi_in = ~m_hi != 0;  // wave-uniform
+            const uint32_t v_hi = hi_in ? base + (63u - (uint32_t)__clzll(m_hi))
+                                        : gallop_le(ws, base + kWave - 1, vmax, tile_end - 1);
             v_prev = v_hi;
             have_prev = true;
 
-            // lane state: wedges [first, lend); the first one is located by a
-            // search in [v_lo, v_hi] and decoded exactly, the rest by stepping
+            // lane state: wedges [first, lend); the first one is located in
+            // the window (or past it) and decoded exactly, the rest by stepping
             const uint64_t first = tile_start + (uint64_t)lane * s;
             const uint64_t lend = first + s < tile_end ? first + s : tile_end;
             // wedges of this lane not yet staged (<= s, so 32 bits); the 64-bit
             // index of the next one is lend - left
             uint32_t left = first < lend ? (uint32_t)(lend - first) : 0u;
+            // rightmost window index in [lo_l, 63] with wedgeSum <= first
+            // (binary lifting; every lane takes part in the shuffles)
+            uint32_t pos = lo_l;
+#pragma unroll
+            for (uint32_t step = 32; step > 0; step >>= 1) {
+                const uint32_t cand = pos + step;
+                const uint64_t wc = shfl_u64(wwin, cand < (uint32_t)kWave ? cand : kWave - 1);
+                pos = cand < (uint32_t)kWave && wc <= first ? cand : pos;
+            }
+            const uint64_t ws_v = shfl_u64(wwin, pos);
+            const OffT rb = shfl_off(rwin, pos);
+            const OffT re = shfl_off(rwin, pos + 1 < (uint32_t)kWave ? pos + 1 : kWave - 1);
             uint32_t v = 0, d = 0, w_i = 0, u_i = 0;
             OffT vb = 0;
             if (left) {
-                v = rightmost_le(ws, v_lo, v_hi, first);
-                vb = rowPtr[v];
-                d = (uint32_t)(rowPtr[v + 1] - vb);
-                decode_wedge(d, first - ws[v], w_i, u_i);
+                if (pos == kWave - 1 && !hi_in) {  // the lane's vertex lies past the window
+                    v = rightmost_le(ws, base + kWave - 1, v_hi, first);
+                    vb = rowPtr[v];
+                    d = (uint32_t)(rowPtr[v + 1] - vb);
+                    decode_wedge(d, first - ws[v], w_i, u_i);
+                } else {
+                    v = base + pos;
+                    vb = rb;
+                    d = (uint32_t)((pos + 1 < (uint32_t)kWave ? re : rowPtr[v + 1]) - vb);
+                    decode_wedge(d, first - ws_v, w_i, u_i);
+                }
             }
             // hits of the wave in this tile: popcount(ballot(match)) as each
             // probe step resolves (a probe matches at most once); the ballot
@@ -491,36 +552,40 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                         }
                     }
 #pragma unroll
-                    for (int j = 0; j < 4; j++) {
-                        if (k0 + j < kn) {
-                            lw[__umul24(lane, pad) + k0 + j] = wv[j];
-                            lu[__umul24(lane, pad) + k0 + j] = uv[j];
-                        }
-                    }
+                    for (int j = 0; j < 4; j++)
+                        if (k0 + j < kn) lwu[__umul24(lane, pad) + k0 + j] = make_uint2(wv[j], uv[j]);
                 }
                 __syncwarp();
                 // phase 2: consecutive lanes probe consecutive wedges; each lane
                 // advances kWays independent searches in lockstep so that their
                 // dependent loads overlap (memory-level parallelism per lane).
+                // Step j of lane l reads staged wedge p = 64j + l, which lane
+                // p / kn wrote as its (p % kn)-th: slot (p / kn) * pad + p % kn.
+                // The slot is advanced from step to step (p grows by 64 =
+                // sa*kn + sb) instead of dividing again.  Division of lane by
+                // kn once: lane < 64 and magic <= 2^16, so the 24-bit
+                // multiplies are exact.
                 const uint32_t magic = (65536u + kn - 1) / kn;
+                const uint32_t sa = kWave / kn, sb = kWave % kn;
+                const uint32_t step0 = sa * pad + sb, step1 = step0 + pad - kn;
+                uint32_t q0 = __umul24(lane, magic) >> 16;
+                uint32_t rem = lane - __umul24(q0, kn);
+                uint32_t slot = __umul24(q0, pad) + rem;
                 for (uint32_t j0 = 0; j0 < kn; j0 += kWays) {
                     uint32_t pw[kWays], pu[kWays];
                     OffT b[kWays], len[kWays];
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
-                        const uint32_t j = j0 + k;
                         pw[k] = kSentinel;
                         pu[k] = 0;
-                        if (j < kn) {
-                            // p < 64 * kChunkMax and magic <= 2^16: operands
-                            // below 2^24 and products below 2^32, so the
-                            // full-rate 24-bit multiplies are exact
-                            const uint32_t p = j * kWave + lane;
-                            const uint32_t q = __umul24(p, magic) >> 16;
-                            const uint32_t r = p - __umul24(q, kn);
-                            const uint32_t slot = __umul24(q, pad) + r;
-                            pw[k] = lw[slot];
-                            pu[k] = lu[slot];
+                        if (j0 + k < kn) {
+                            const uint2 e = lwu[slot];
+                            pw[k] = e.x;
+                            pu[k] = e.y;
+                            rem += sb;
+                            const bool carry = rem >= kn;
+                            rem = carry ? rem - kn : rem;
+                            slot += carry ? step1 : step0;
                         }
                     }
                     // Every way's first load is issued before any is used, so
@@ -907,7 +972,7 @@ hipError_t allow_small_lds(uint32_t lds_bytes) {
 size_t count_lds_bytes(uint32_t max_spread) {
     const uint32_t sc = max_spread < kChunkMax ? max_spread : kChunkMax;
     const uint32_t pad = sc | 1u;
-    return (size_t)kWavesPerBlock * 2u * kWave * pad * sizeof(uint32_t);
+    return (size_t)kWavesPerBlock * kWave * pad * sizeof(uint2);
 }
 
 static uint32_t count_pad(uint32_t max_spread) {
