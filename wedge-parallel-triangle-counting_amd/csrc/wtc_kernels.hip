@@ -484,6 +484,46 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                 // phase 1: this lane's next kn wedges into LDS, positions first
                 // and then the colInd loads of a group together
                 for (uint32_t k0 = 0; k0 < kn; k0 += 4) {
+                    // Fast path, taken by the whole wave when every lane that
+                    // still has wedges has its next four in the current row
+                    // (one w, four consecutive u) -- nearly always where the
+                    // rows are long, which is where the wedges are: one 16-byte
+                    // load of the four ids, w only when its position changed,
+                    // no per-wedge stepping.  Waves with short rows stay on the
+                    // general path, so they never run both.
+                    if (k0 + 4 <= kn && __ballot(left != 0 && (left < 4 || d - u_i < 4)) == 0) {
+                        uint2 *const dst = lwu + __umul24(lane, pad) + k0;
+                        if (left == 0) {  // a finished lane: empty slots
+                            dst[0] = dst[1] = dst[2] = dst[3] = make_uint2(kSentinel, 0u);
+                            continue;
+                        }
+                        const OffT wp = vb + w_i;
+                        if (wp != last_wpos) {
+                            last_wv = colInd[wp];
+                            last_wpos = wp;
+                        }
+                        const u32x4_e q = *(gptr<u32x4_e>)(colInd + (vb + u_i));
+                        dst[0] = make_uint2(last_wv, q.x);
+                        dst[1] = make_uint2(last_wv, q.y);
+                        dst[2] = make_uint2(last_wv, q.z);
+                        dst[3] = make_uint2(last_wv, q.w);
+                        left -= 4;
+                        u_i += 4;
+                        if (left && u_i >= d) {  // the row ended with the group
+                            w_i++;
+                            u_i = w_i + 1;
+                            if (u_i >= d) {  // vertex exhausted: next vertex with wedges
+                                const uint64_t i = lend - left;  // the next wedge
+                                v = (ws[v + 2] > i) ? v + 1 : rightmost_le(ws, v + 2, v_hi, i);
+                                vb = rowPtr[v];
+                                d = (uint32_t)(rowPtr[v + 1] - vb);
+                                w_i = 0;
+                                u_i = 1;
+                            }
+                        }
+                        continue;
+                    }
+                    // general path: wedge by wedge across row and vertex ends
                     OffT wpos[4], upos[4];
                     bool ok[4];
 #pragma unroll
