@@ -324,3 +324,33 @@ def test_batch_upload_checks_and_reuse(gpu_ctx):
         assert wtc.batch_count_host(gpu_ctx, graphs).tolist() == expect
     for k in (12, 3, 12, 1, 7):
         assert wtc.batch_count_host(gpu_ctx, graphs[:k]).tolist() == expect[:k], k
+
+
+def test_tiles_across_wedge_free_runs(gpu_ctx):
+    """Tiles whose vertex range crosses runs of more than 64 wedge-free
+    vertices (the tile setup's coalesced 64-vertex window then skips ahead
+    or falls back to searches in global memory): 40 disjoint K4 blocks, each
+    followed in rank order by 200 degree-3 claw centres that have no wedges,
+    counted alone at several spreads and bitmap lengths and inside a batch."""
+    B, C = 40, 200
+    pairs, leaf = [], 10 ** 6
+    for b in range(B):
+        base = b * (4 + C)
+        pairs += [(base + i, base + j) for i in range(4) for j in range(i + 1, 4)]
+        for c in range(C):
+            for _ in range(3):
+                pairs.append((base + 4 + c, leaf))
+                leaf += 1
+    g = wtc.preprocess(wtc.graph_from_pairs(np.array(pairs, np.uint32)))
+    d = np.diff(np.asarray(g.row_ptr, np.int64))
+    gaps = np.diff(np.nonzero(d * (d - 1) // 2)[0])
+    assert (gaps > 64).sum() >= 30  # the runs the test is about
+    assert O.tc_merge(g) == 4 * B
+    for ob in (4, 8):
+        dg = gpu_ctx.upload(g, offset_bytes=ob)
+        for s, a in ((1, 0), (2, 32), (3, 0), (5, 4096), (1, 64)):
+            assert dg.count(s, a) == 4 * B, (ob, s, a)
+        dg.free()
+    other = wtc.preprocess(wtc.gen_barabasi_albert(500, 3, seed=77))
+    got = wtc.batch_count_host(gpu_ctx, [other, g, other], [2, 1, 3], [0, 0, 32])
+    assert got.tolist() == [O.tc_merge(other), 4 * B, O.tc_merge(other)]
