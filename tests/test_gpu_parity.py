@@ -354,3 +354,31 @@ def test_tiles_across_wedge_free_runs(gpu_ctx):
     other = wtc.preprocess(wtc.gen_barabasi_albert(500, 3, seed=77))
     got = wtc.batch_count_host(gpu_ctx, [other, g, other], [2, 1, 3], [0, 0, 32])
     assert got.tolist() == [O.tc_merge(other), 4 * B, O.tc_merge(other)]
+
+
+def test_host_batch_chunked_pipeline(gpu_ctx, monkeypatch):
+    """A host batch counted chunk by chunk (upload chunk forced down to a few
+    KiB, so each chunk holds one or a few graphs and the two execution slots
+    alternate): per-graph counts equal the oracle, stats sum over the chunks,
+    and an invalid CSR in a late chunk still fails the whole call."""
+    graphs = [wtc.preprocess(wtc.gen_erdos_renyi(100 + 90 * i, 8.0 / (100 + 90 * i), seed=1200 + i)) if i % 2 else
+              wtc.preprocess(wtc.gen_barabasi_albert(120 + 70 * i, 2 + i % 4, seed=1200 + i)) for i in range(30)]
+    expect = [O.tc_merge(g) for g in graphs]
+    whole, st1 = wtc.batch_count_host(gpu_ctx, graphs, stats=True)
+    assert whole.tolist() == expect and st1.launches == 1
+    for chunk in (4096, 65536):
+        monkeypatch.setenv("WTC_UPLOAD_CHUNK_BYTES", str(chunk))
+        got, st = wtc.batch_count_host(gpu_ctx, graphs, stats=True)
+        assert got.tolist() == expect, chunk
+        assert st.launches > 2 and st.wedges == st1.wedges
+        got = wtc.batch_count_host(gpu_ctx, graphs, [1 + i % 5 for i in range(30)], [32 * (i % 3) for i in range(30)])
+        assert got.tolist() == expect
+        bad = graphs[25]
+        ci = bad.col_ind.copy()
+        ci[-1] = bad.n  # an id out of range
+        mix = graphs[:25] + [wtc.CSR(bad.n, bad.m, bad.row_ptr.copy(), ci)] + graphs[26:]
+        with pytest.raises(wtc.WtcError) as ei:
+            wtc.batch_count_host(gpu_ctx, mix)
+        assert ei.value.code == -10
+        assert wtc.batch_count_host(gpu_ctx, graphs).tolist() == expect
+    monkeypatch.delenv("WTC_UPLOAD_CHUNK_BYTES")

@@ -81,6 +81,7 @@ hipError_t PinBuf::ensure(size_t bytes) {
 int Slot::create() {
     WTC_HIPC(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto &e : ev) WTC_HIPC(hipEventCreate(&e));
+    WTC_HIPC(hipEventCreateWithFlags(&ev_ready, hipEventDisableTiming));
     return WTC_OK;
 }
 
@@ -100,8 +101,10 @@ void Slot::destroy() {
     h_out.release();
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
+    if (ev_ready) (void)hipEventDestroy(ev_ready);
     if (stream) (void)hipStreamDestroy(stream);
     stream = nullptr;
+    ev_ready = nullptr;
     for (auto &e : ev) e = nullptr;
 }
 
@@ -511,6 +514,7 @@ void wtc_ctx_destroy(wtc_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     c->slot.destroy();
+    for (auto &p : c->pipe) p.destroy();
     c->h_stage.release();
     c->batch_row.release();
     c->batch_col.release();
@@ -757,8 +761,14 @@ int pack_graph(const wtc_big_graph *gr, int ob, char *row_stage, uint32_t *col_s
 }
 
 // Bytes of packed graphs after which a host-to-device copy is started, so
-// that packing the next graphs overlaps the copy of the previous ones.
+// that packing the next graphs overlaps the copy of the previous ones:
+// $WTC_UPLOAD_CHUNK_BYTES, else 256 MiB.
 constexpr uint64_t kUploadChunkBytes = 256ull << 20;
+uint64_t upload_chunk_bytes() {
+    const char *e = std::getenv("WTC_UPLOAD_CHUNK_BYTES");
+    const long long v = e ? std::atoll(e) : 0;
+    return v > 0 ? (uint64_t)v : kUploadChunkBytes;
+}
 
 // Packs k host CSRs into b's device rowPtr/colInd (grown as needed): sizes
 // and offsets, then one pass per graph (copy into the context's pinned
@@ -794,10 +804,11 @@ int batch_fill(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, wtc_bat
     hipStream_t st = c->slot.stream;
     std::atomic<int> bad{WTC_OK};
     hipError_t e = hipSuccess;
+    const uint64_t chunk_bytes = upload_chunk_bytes();
     for (size_t g0 = 0; g0 < k && e == hipSuccess;) {
         size_t g1 = g0;
         uint64_t bytes = 0;
-        while (g1 < k && (g1 == g0 || bytes < kUploadChunkBytes)) {
+        while (g1 < k && (g1 == g0 || bytes < chunk_bytes)) {
             bytes += (b->n[g1] + 1) * (uint64_t)ob + b->m[g1] * 4;
             g1++;
         }
@@ -855,11 +866,13 @@ void wtc_batch_free(wtc_batch *b) {
 namespace {
 
 // Per-graph spread and bitmap length of a batch count (autotuned where 0).
+// Per-graph spread and bitmap length of graphs [g0, g1) of a batch count
+// (autotuned where 0); s_out and a_out hold the whole batch.
 int batch_params(const wtc_batch *b, const uint32_t *spread, const uint32_t *a, std::vector<uint32_t> &s_out,
-                 std::vector<uint32_t> &a_out) {
+                 std::vector<uint32_t> &a_out, uint32_t g0 = 0, uint32_t g1 = UINT32_MAX) {
     s_out.resize(b->G);
     a_out.resize(b->G);
-    for (uint32_t g = 0; g < b->G; g++) {
+    for (uint32_t g = g0; g < std::min(g1, b->G); g++) {
         uint32_t s = spread ? spread[g] : 0, aa = a ? a[g] : 0;
         if (s == 0) wtc_autotune_batch(b->n[g], b->m[g], b->W[g], &s, a ? nullptr : &aa);
         if (s == 0 || aa % 32 != 0) return WTC_ERR_ARG;
@@ -869,12 +882,14 @@ int batch_params(const wtc_batch *b, const uint32_t *spread, const uint32_t *a, 
     return WTC_OK;
 }
 
+// Descriptors of graphs [g0, g1) of a batch (the whole batch by default).
 std::vector<GraphDesc> batch_descs(const wtc_batch *b, const std::vector<uint32_t> &s,
-                                   const std::vector<uint32_t> &a) {
-    std::vector<GraphDesc> hd(b->G);
-    for (uint32_t g = 0; g < b->G; g++) {
+                                   const std::vector<uint32_t> &a, uint32_t g0 = 0, uint32_t g1 = UINT32_MAX) {
+    g1 = std::min(g1, b->G);
+    std::vector<GraphDesc> hd(g1 > g0 ? g1 - g0 : 0);
+    for (uint32_t g = g0; g < g1; g++) {
         const void *rp = b->rowPtr.as<char>() + b->row_off[g] * b->ob;
-        hd[g] = make_desc(rp, b->colInd.as<uint32_t>() + b->col_off[g], b->n[g], s[g], a[g], 0, UINT64_MAX);
+        hd[g - g0] = make_desc(rp, b->colInd.as<uint32_t>() + b->col_off[g], b->n[g], s[g], a[g], 0, UINT64_MAX);
     }
     return hd;
 }
@@ -923,6 +938,119 @@ int batch_capture(wtc_ctx *c, wtc_batch *b, const std::vector<uint32_t> &s, cons
     return WTC_OK;
 }
 
+// Host CSRs counted as they arrive: the graphs are packed into pinned
+// staging and copied to the device in chunks of about upload_chunk_bytes() on
+// the context's stream (the upload stream here), and every chunk is counted
+// as its own graph set on one of two execution slots as soon as its copy has
+// landed, so counting overlaps packing and copying the later chunks.  A
+// graph that is not an oriented CSR fails the whole call.
+int batch_count_pipelined(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t k, const uint32_t *spread,
+                          const uint32_t *a, wtc_batch *b, uint64_t *triangles, RunTimes *sum, uint64_t *wsum,
+                          uint32_t *launches, uint32_t *small) {
+    b->G = (uint32_t)k;
+    b->n.resize(k);
+    b->m.resize(k);
+    b->row_off.resize(k);
+    b->col_off.resize(k);
+    b->W.resize(k);
+    uint64_t R = 0, M = 0, max_m = 0;
+    for (size_t g = 0; g < k; g++) {
+        if (!graphs[g] || graphs[g]->numVertices == 0) return WTC_ERR_ARG;
+        b->n[g] = graphs[g]->numVertices;
+        b->m[g] = graphs[g]->numEdges;
+        b->row_off[g] = R;
+        b->col_off[g] = M;
+        R += (uint64_t)graphs[g]->numVertices + 1;
+        M += graphs[g]->numEdges;
+        max_m = std::max<uint64_t>(max_m, graphs[g]->numEdges);
+    }
+    b->ob = max_m < 0xffffffffULL ? 4 : 8;
+    const int ob = b->ob;
+    WTC_HIPC(hipSetDevice(c->device));
+    const size_t rbytes = R * (size_t)ob, cbytes = std::max<uint64_t>(M, 1) * 4;
+    WTC_HIPC(b->rowPtr.ensure(rbytes));
+    WTC_HIPC(b->colInd.ensure(cbytes));
+    WTC_HIPC(c->h_stage.ensure(rbytes + cbytes));
+    for (auto &p : c->pipe)
+        if (!p.stream) {
+            const int rc = p.create();
+            if (rc) return rc;
+        }
+    char *stage = c->h_stage.as<char>();
+    uint32_t *sc = reinterpret_cast<uint32_t *>(stage + rbytes);
+    hipStream_t up = c->slot.stream;
+    std::vector<uint32_t> s_all, a_all;
+    uint32_t pending_g0[2] = {0, 0};
+    bool pending[2] = {false, false};
+    int rc = WTC_OK;
+    // collects the counts of the set in flight on pipe slot i
+    auto drain = [&](int i) -> int {
+        if (!pending[i]) return WTC_OK;
+        pending[i] = false;
+        RunTimes rt;
+        uint64_t W = 0;
+        const int r = finish_set(c->pipe[i], triangles + pending_g0[i], &W, &rt);
+        if (r) return r;
+        sum->prefix_ms += rt.prefix_ms;
+        sum->bitmap_ms += rt.bitmap_ms;
+        sum->count_ms += rt.count_ms;
+        *wsum += W;
+        *small += c->pipe[i].small;
+        return WTC_OK;
+    };
+    int next = 0;
+    const uint64_t chunk_bytes = upload_chunk_bytes();
+    for (size_t g0 = 0; g0 < k && rc == WTC_OK;) {
+        size_t g1 = g0;
+        uint64_t bytes = 0;
+        while (g1 < k && (g1 == g0 || bytes < chunk_bytes)) {
+            bytes += (b->n[g1] + 1) * (uint64_t)ob + b->m[g1] * 4;
+            g1++;
+        }
+        std::atomic<int> bad{WTC_OK};
+        par_chunks(g1 - g0, 1, [&](uint64_t i0, uint64_t i1, int) {
+            for (uint64_t i = g0 + i0; i < g0 + i1; i++) {
+                const int r = pack_graph(graphs[i], ob, stage, sc, b->row_off[i], b->col_off[i], &b->W[i]);
+                if (r) bad = r;
+            }
+        });
+        if ((rc = bad.load())) break;
+        const uint64_t r0 = b->row_off[g0], r1 = b->row_off[g1 - 1] + b->n[g1 - 1] + 1;
+        const uint64_t c0 = b->col_off[g0], c1 = b->col_off[g1 - 1] + b->m[g1 - 1];
+        hipError_t e = hipMemcpyAsync(b->rowPtr.as<char>() + r0 * ob, stage + r0 * ob, (r1 - r0) * ob,
+                                      hipMemcpyHostToDevice, up);
+        if (e == hipSuccess && c1 > c0)
+            e = hipMemcpyAsync(b->colInd.as<uint32_t>() + c0, sc + c0, (c1 - c0) * 4, hipMemcpyHostToDevice, up);
+        Slot &x = c->pipe[next];
+        if (e == hipSuccess) e = hipEventRecord(x.ev_ready, up);
+        if (e != hipSuccess) {
+            rc = hip_fail("batch upload", e);
+            break;
+        }
+        if ((rc = drain(next))) break;  // the slot's previous set (two chunks back)
+        if ((rc = batch_params(b, spread, a, s_all, a_all, (uint32_t)g0, (uint32_t)g1))) break;
+        std::vector<GraphDesc> hd = batch_descs(b, s_all, a_all, (uint32_t)g0, (uint32_t)g1);
+        if ((e = hipStreamWaitEvent(x.stream, x.ev_ready, 0)) != hipSuccess) {
+            rc = hip_fail("batch upload wait", e);
+            break;
+        }
+        if ((rc = enqueue_set(x, c->tune, ob, hd, b->m.data() + g0, b->W.data() + g0))) break;
+        pending[next] = true;
+        pending_g0[next] = (uint32_t)g0;
+        ++*launches;
+        next ^= 1;
+        g0 = g1;
+    }
+    // every copy and launch has finished before the staging or buffers are reused
+    (void)hipStreamSynchronize(up);
+    for (int i = 0; i < 2; i++) {
+        const int r = drain(i);
+        if (rc == WTC_OK) rc = r;
+        if (c->pipe[i].stream) (void)hipStreamSynchronize(c->pipe[i].stream);
+    }
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -966,6 +1094,10 @@ int wtc_batch_count_host(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t 
                          const uint32_t *a, uint64_t *triangles, wtc_count_stats *st) {
     auto t0 = std::chrono::steady_clock::now();
     if (!c || !triangles || (k && !graphs) || k > 0xffffffffULL) return WTC_ERR_ARG;
+    if (k == 0) {
+        if (st) *st = wtc_count_stats{};
+        return WTC_OK;
+    }
     wtc_batch *b = new (std::nothrow) wtc_batch();
     if (!b) return WTC_ERR_NOMEM;
     b->ctx = c;
@@ -973,19 +1105,23 @@ int wtc_batch_count_host(wtc_ctx *c, const wtc_big_graph *const *graphs, size_t 
     // calls allocate no device memory once the buffers are large enough
     b->rowPtr = std::move(c->batch_row);
     b->colInd = std::move(c->batch_col);
-    int rc = batch_fill(c, graphs, k, b);
-    // one pass over freshly uploaded graphs: launched directly, no capture
-    std::vector<uint32_t> s, aa;
-    if (rc == WTC_OK) rc = batch_params(b, spread, a, s, aa);
-    if (rc == WTC_OK && b->G) {
-        std::vector<GraphDesc> hd = batch_descs(b, s, aa);
-        rc = run_set(c, b->ob, hd, triangles, st, nullptr, b->m.data(), b->W.data());
-    }
+    RunTimes sum{};
+    uint64_t wsum = 0;
+    uint32_t launches = 0, small = 0;
+    int rc = batch_count_pipelined(c, graphs, k, spread, a, b, triangles, &sum, &wsum, &launches, &small);
     c->batch_row = std::move(b->rowPtr);
     c->batch_col = std::move(b->colInd);
     wtc_batch_free(b);
-    auto t1 = std::chrono::steady_clock::now();
-    if (st && rc == WTC_OK) st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    if (rc == WTC_OK && st) {
+        st->prefix_ms = sum.prefix_ms;
+        st->bitmap_ms = sum.bitmap_ms;
+        st->count_ms = sum.count_ms;
+        st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        st->wedges = wsum;
+        st->blocks = c->pipe[0].grid;
+        st->launches = launches;
+        st->small_graphs = small;
+    }
     return rc;
 }
 

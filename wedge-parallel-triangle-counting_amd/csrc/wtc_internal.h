@@ -82,6 +82,7 @@ struct PinBuf : OwnedBuf<PinnedFree> {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};  // prefix start, bitmap start, count start, count end
+    hipEvent_t ev_ready = nullptr;  // input copied (waited on by the slot's stream)
     DevBuf ws, partials, tree, bm, desc, out;
     PinBuf h_desc, h_out;
     uint32_t G = 0;       // graphs of the last enqueue
@@ -182,6 +183,7 @@ struct wtc_ctx {
     wtc::Tuning tune;
     wtc::PinBuf h_stage;              // pinned staging of uploads
     wtc::DevBuf batch_row, batch_col;  // device CSRs of wtc_batch_count_host, kept between calls
+    wtc::Slot pipe[2];                 // execution slots of wtc_batch_count_host's chunks
 };
 
 struct wtc_dgraph {
