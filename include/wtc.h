@@ -260,9 +260,13 @@ void wtc_batch_free(wtc_batch *b);
  * count); prefix_ms and bitmap_ms are 0. */
 int wtc_batch_count(wtc_ctx *ctx, wtc_batch *b, const uint32_t *spread, const uint32_t *adjacency_matrix_len,
                     uint64_t *triangles, wtc_count_stats *st);
-/* Host CSRs in, counts out: upload + count + free in one call (the batched
- * counterpart of calling wtc_tc_gpu once per graph); launched directly on
- * the context's stream, without a graph. */
+/* Host CSRs in, counts out in one call (the batched counterpart of calling
+ * wtc_tc_gpu once per graph): each graph is checked and packed into pinned
+ * staging, every chunk of $WTC_UPLOAD_CHUNK_BYTES (default 256 MiB) is copied
+ * to the device and counted as its own graph set as soon as it lands, on two
+ * execution slots of the context, so counting overlaps the upload.  The
+ * device buffers stay in the context for the next call.  A graph that is not
+ * an oriented CSR fails the whole call with WTC_ERR_CSR. */
 int wtc_batch_count_host(wtc_ctx *ctx, const wtc_big_graph *const *graphs, size_t k, const uint32_t *spread,
                          const uint32_t *adjacency_matrix_len, uint64_t *triangles, wtc_count_stats *st);
 /* ---------------------------------------------------------- GPU: serving
