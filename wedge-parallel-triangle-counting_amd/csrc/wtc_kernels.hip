@@ -7,13 +7,15 @@
 //       tc.cu:348-361, as one reduce-then-scan over all graphs' vertices)
 //   k_setup       per-graph wedge ranges, tile bases, the 12-level search
 //                 tree over wedgeSum (tc.cu:306-313) and zeroed counts
-//   k_fill_u32 / k_bitmap_build
+//   k_bitmap_clear / k_bitmap_build
 //                 the strict-upper-triangular bitmap over each graph's top-a
 //                 ranks (tc.cu:329-346), built with atomicOr
 //   k_count       the wedge-parallel count (tc.cu:173-293): persistent,
 //                 XCD-partitioned grid; every wave owns tiles of 64*s wedges
 //                 and keeps its own LDS staging, so waves never wait on each
 //                 other inside a tile.
+//   k_count_small the same count for a set of small graphs, one block per
+//                 slice of a graph's wedges with the graph's CSR in LDS.
 #include "wtc_dev.h"
 #include "wtc_kernels.h"
 #include "wtc_launch.h"
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
     // wedge varies several-fold along it), while each super-chunk stays
     // contiguous for its L2.  Inside an XCD, waves take grabs from the XCD's
     // own counter (dynamic balance); consecutive tiles of a grab let a wave
-    // gallop from the vertex where its previous tile ended.
+    // start its vertex window where its previous tile ended.
     const uint64_t ntiles = meta[kMetaTiles];
     // grab size: about two grabs per wave, at most kTilesPerGrabMax tiles;
     // super-chunks of 2^lg grabs, the largest power of two (at most 64) that
