@@ -200,6 +200,60 @@ def small_kernel_bench(ctx, k, base_seed, reps=7):
     return res
 
 
+def reference_configs(iters=10):
+    """The reference's own measurement configurations through the tc_GPU
+    drop-in (per call: allocate, copy, count, copy back, free): config 1
+    (Chung-Lu n = 2^18, ~900k edges, s = 5, a = 8192) and the config-2 sweep
+    (R-MAT-20, s in {1,3,5,8} x a in {0,4096,8192,16384}).  Per (s, a): the
+    median over `iters` calls of GPU_time's copy and exec (ms, tc.cu:76-79)
+    and of the call's wall time, and the device time of the same count on a
+    resident copy of the graph."""
+    ctx = wtc.Context(0)
+
+    def run(g, params):
+        rows = []
+        dg = ctx.upload(g)
+        for s_, a_ in params:
+            ref = wtc.tc_gpu(g, s_, a_)  # warm-up
+            cp, ex, wall, res = [], [], [], []
+            for _ in range(iters):
+                t = wtc.GpuTime()
+                t0 = time.perf_counter()
+                c = wtc.tc_gpu(g, s_, a_, t)
+                wall.append((time.perf_counter() - t0) * 1e3)
+                assert c == ref
+                cp.append(t.copy)
+                ex.append(t.exec)
+                c2, st = dg.count(s_, a_, stats=True)
+                assert c2 == ref
+                res.append(st.prefix_ms + st.bitmap_ms + st.count_ms)
+            rows.append({"spread": s_, "a": a_, "triangles": ref, "copy_ms": float(np.median(cp)),
+                         "exec_ms": float(np.median(ex)), "call_ms": float(np.median(wall)),
+                         "resident_ms": float(np.median(res))})
+        dg.free()
+        return {"n": g.n, "m": g.m, "wedges": g.wedges(), "runs": rows}
+
+    def prep(raw):
+        """The preprocessing drop-in (tc.cu:687-787): median of 3 wall times
+        for -p 0 (host builder) and -p 2 (GPU builder); both give one CSR."""
+        res = {}
+        for style in (0, 2):
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                g = wtc.preprocess(raw, style)
+                ts.append((time.perf_counter() - t0) * 1e3)
+            res[f"prepro_p{style}_ms"] = float(np.median(ts))
+        return g, res
+
+    g1, p1 = prep(wtc.gen_chung_lu())
+    g2, p2 = prep(wtc.gen_rmat(20, 16, seed=1))
+    out = {"config1": dict(run(g1, [(5, 8192)]), **p1),
+           "config2": dict(run(g2, [(s_, a_) for s_ in (1, 3, 5, 8) for a_ in (0, 4096, 8192, 16384)]), **p2)}
+    ctx.close()
+    return out
+
+
 def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, window):
     """Serving workload: Poisson arrivals of count requests on seeded R-MAT
     graphs (scale uniform in [10, 22], edge factor 16), each with the
@@ -417,6 +471,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="only the RMAT count: no batched-vs-plain and no serving legs (profiler runs)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched vs plain comparison")
+    ap.add_argument("--reference-configs", action="store_true",
+                    help="run only the reference's config 1 and config-2 sweep through the tc_GPU drop-in")
     ap.add_argument("--small-batches-only", action="store_true",
                     help="run only the small-graph kernel vs wedge kernel comparison on batches of small graphs")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving (Poisson arrivals) workload")
@@ -450,6 +506,9 @@ def main():
         res = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 2, 512,
                           args.ways, args.chunk, args.window)
         print(json.dumps(res), flush=True)
+        return
+    if args.reference_configs:
+        print(json.dumps(reference_configs()), flush=True)
         return
     if args.small_batches_only:
         ctx = wtc.Context(0)
