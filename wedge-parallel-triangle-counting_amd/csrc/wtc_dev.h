@@ -19,6 +19,24 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x) {
            __builtin_amdgcn_readfirstlane((uint32_t)x);
 }
 
+// Value of x in lane src (ds_bpermute), for 64- and 32-bit values; every
+// lane of the wave must execute it.
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, uint32_t src) {
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)x, (int)src, kWave);
+    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(x >> 32), (int)src, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t shfl_off(uint32_t x, uint32_t src) {
+    return (uint32_t)__shfl((int)x, (int)src, kWave);
+}
+__device__ __forceinline__ uint64_t shfl_off(uint64_t x, uint32_t src) { return shfl_u64(x, src); }
+// Value of x in lane (lane - d), or the lane's own x for lanes below d.
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t x, int d) {
+    const uint32_t lo = (uint32_t)__shfl_up((int)(uint32_t)x, d, kWave);
+    const uint32_t hi = (uint32_t)__shfl_up((int)(uint32_t)(x >> 32), d, kWave);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ uint64_t wave_incl_scan(uint64_t x) {
     const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll

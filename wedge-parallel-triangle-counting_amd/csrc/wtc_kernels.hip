@@ -196,10 +196,40 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_build(const GraphDesc *__rest
         uint32_t *__restrict__ bm = D.bitmap;
         const uint32_t v = D.n - D.bm_rows + (uint32_t)(r - D.row_base);
         const uint64_t rowbits = D.bm_size - binom2_u32(D.n - v) - v - 1;
+        // the row's bits are [rowbits + v + 1, rowbits + n - 1]: its first and
+        // last words may hold bits of the neighbouring rows, every word in
+        // between belongs to this row (and so to this wave) alone
+        const uint64_t fw = (rowbits + v + 1) >> 5, lw = (rowbits + D.n - 1) >> 5;
         const OffT b = rowPtr[v], e = rowPtr[v + 1];
-        for (OffT j = b + lane; j < e; j += kWave) {
-            const uint64_t idx = rowbits + colInd[j];
-            atomicOr(&bm[idx >> 5], 1u << (idx & 31u));
+        for (OffT j0 = b; j0 < e; j0 += kWave) {  // wave-uniform
+            const OffT j = j0 + lane;
+            uint64_t word = ~0ull;  // no lane's word: inactive lanes never match
+            uint32_t acc = 0;
+            if (j < e) {
+                const uint64_t idx = rowbits + colInd[j];
+                word = idx >> 5;
+                acc = 1u << (idx & 31u);
+            }
+            // ids ascend along the lanes, so lanes sharing a word are
+            // consecutive: OR each run into its last lane (segmented scan)
+#pragma unroll
+            for (int dd = 1; dd < kWave; dd <<= 1) {
+                const uint32_t ov = (uint32_t)__shfl_up((int)acc, dd, kWave);
+                const uint64_t ow = shfl_up_u64(word, dd);
+                if ((int)lane >= dd && ow == word) acc |= ov;
+            }
+            const uint64_t next_w = shfl_u64(word, lane + 1 < (uint32_t)kWave ? lane + 1 : lane);
+            const uint64_t first_w = shfl_u64(word, 0);
+            const bool run_end = j < e && (lane == kWave - 1 || next_w != word);
+            if (run_end) {
+                // shared with a neighbouring row, or possibly continued in the
+                // next chunk (run reaching lane 63) or begun in the previous
+                // one (run holding lane 0): atomic; otherwise a plain store
+                if (word == fw || word == lw || lane == kWave - 1 || word == first_w)
+                    atomicOr(&bm[word], acc);
+                else
+                    bm[word] = acc;
+            }
         }
     }
 }
@@ -295,18 +325,6 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
     w_out[i] = w;
     u_out[i] = u;
 }
-
-// Value of x in lane src (ds_bpermute), for 64- and 32-bit values; every
-// lane of the wave must execute it.
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t x, uint32_t src) {
-    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)x, (int)src, kWave);
-    const uint32_t hi = (uint32_t)__shfl((int)(uint32_t)(x >> 32), (int)src, kWave);
-    return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ uint32_t shfl_off(uint32_t x, uint32_t src) {
-    return (uint32_t)__shfl((int)x, (int)src, kWave);
-}
-__device__ __forceinline__ uint64_t shfl_off(uint64_t x, uint32_t src) { return shfl_u64(x, src); }
 
 // Four ids loaded by one 16-byte vector load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
