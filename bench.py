@@ -285,8 +285,10 @@ def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, win
     srv = wtc.Server(0, hbm_budget_bytes=32 << 30, max_batch=256, slots=2, queue_capacity=queue_capacity,
                      probe_ways=ways, lds_chunk=chunk, probe_window=window if window else -1)
     gids = [srv.register(g) for g in graphs]
+    for gid, (s_g, a_g) in zip(gids, tune):  # each graph alone once: resident, server warmed up
+        srv.wait(srv.submit(gid, s_g, a_g))
     alone, alone_ms = [], []
-    for gid, (s_g, a_g) in zip(gids, tune):  # each graph alone (also makes it resident)
+    for gid, (s_g, a_g) in zip(gids, tune):  # then timed, each graph alone
         cnt, st = srv.wait(srv.submit(gid, s_g, a_g))
         alone.append(cnt)
         alone_ms.append(st.service_ms)
