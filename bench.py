@@ -599,7 +599,11 @@ def main():
         "wedge_checks_per_s_kernel": W / (count_ms / 1e3),
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "b_alg": b_alg, "b_min": b_min},
+                     "b_alg": b_alg, "b_min": b_min,
+                     # measured HBM bytes per launch over the same launch time: the rate the memory
+                     # actually ran at (achieved/frac count every probe as HBM bytes, most hit caches)
+                     "traffic_GBps": traffic["bytes_per_launch"] / (count_ms / 1e3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic["bytes_per_launch"] / (count_ms / 1e3) / HBM_PEAK if traffic else None},
         "prepro_s": t2 - t1,
         "builder": "device" if device_build else "host",
         "tuning": {"probe_ways": args.ways, "lds_chunk": args.chunk, "probe_window": args.window},
