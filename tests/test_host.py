@@ -272,6 +272,9 @@ def test_csr_file_round_trip_and_corruption(tmp_path):
     magic[0] ^= 0xFF
     expect_format(magic, "magic.csr")
     expect_format(raw[:40], "tiny.csr")
+    reserved = bytearray(raw)
+    reserved[48] ^= 0x10  # header bytes 40..63 are reserved, written as zero
+    expect_format(reserved, "reserved.csr")
     # a checksum-valid file whose ids run past n is still refused
     bad = wtc.CSR(g.n, g.m, g.row_ptr.copy(), np.array(g.col_ind).copy())
     bad.col_ind[-1] = g.n + 5

@@ -309,7 +309,10 @@ int wtc_read_csr(const char *path, uint64_t *tag, wtc_big_graph **out) {
     CsrHeader h;
     if (f.size < sizeof h) return WTC_ERR_FORMAT;
     std::memcpy(&h, f.p, sizeof h);
-    if (std::memcmp(h.magic, kMagic, 8) != 0 || h.n == 0 || h.n > 0xffffffffULL) return WTC_ERR_FORMAT;
+    // the writer zeroes the reserved words; anything else is a damaged header
+    if (std::memcmp(h.magic, kMagic, 8) != 0 || h.n == 0 || h.n > 0xffffffffULL ||
+        (h.reserved[0] | h.reserved[1] | h.reserved[2]) != 0)
+        return WTC_ERR_FORMAT;
     if (h.m > (f.size - sizeof h) / 4 || (f.size - sizeof h - 4 * h.m) / 8 != h.n + 1 ||
         sizeof h + 8 * (h.n + 1) + 4 * h.m != f.size)
         return WTC_ERR_FORMAT;
