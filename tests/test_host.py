@@ -61,6 +61,41 @@ def test_reader_cases_match_oracle(tmp_path, case):
     assert same_csr(wtc.read_graph(path, mm, zero), og)
 
 
+def _junk_text(rng):
+    """Edge-list-like text with stray characters, signs, blank and comment
+    lines, and header lines longer than read_graph's 256-byte line buffer."""
+    parts = []
+    if rng.random() < 0.3:
+        parts.append("%%MatrixMarket matrix coordinate pattern\n")
+    if rng.random() < 0.3:
+        parts.append("#" + "x" * int(rng.integers(200, 300)) + str(int(rng.integers(0, 9))) + " 7\n")
+    for _ in range(int(rng.integers(0, 120))):
+        r = rng.random()
+        if r < 0.08:
+            parts.append(str(rng.choice(list("%#x.e\t\r+\n "))))
+        else:
+            parts.append(("+" if rng.random() < 0.05 else "") + str(int(rng.integers(0, 400))))
+            parts.append(" " if rng.random() < 0.6 else "\n")
+    return "".join(parts)
+
+
+def test_reader_junk_text_matches_oracle(tmp_path):
+    """Seeded malformed inputs: the parallel reader must stop, fail or succeed
+    exactly where the sequential restatement of tc.cu:561-685 does."""
+    rng = np.random.default_rng(11)
+    for i in range(150):
+        path = _write(tmp_path, f"junk{i}.txt", _junk_text(rng))
+        for mm in (False, True):
+            for zero in (False, True):
+                og = O.read_graph(path, mm, zero)
+                if isinstance(og, int):
+                    with pytest.raises(wtc.WtcError) as e:
+                        wtc.read_graph(path, mm, zero)
+                    assert e.value.code == og, (i, mm, zero)
+                else:
+                    assert same_csr(wtc.read_graph(path, mm, zero), og), (i, mm, zero)
+
+
 def test_preprocess_style_checks():
     g = wtc.gen_rmat(11, 8, seed=3)
     ref = wtc.preprocess(g, 0)
