@@ -159,6 +159,13 @@ int wtc_tc_gpu(const wtc_graph *graph, uint32_t spread, uint32_t adjacency_matri
 /* tc_GPU of experiments/tc_big_graphs.cu:315-447 (64-bit offsets). */
 int wtc_tc_gpu_big(const wtc_big_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len,
                    wtc_gpu_time *t, uint64_t *triangles);
+/* tc_arrow_GPU, experiments/tc_arrow.cu:310-441 (64-bit offsets:
+ * experiments/tc_big_graphs_arrow.cu): the arrow-wedge count, per call
+ * allocating, copying, counting and freeing like wtc_tc_gpu.  Arrow wedges are
+ * the paths v->w->u; u is probed in adj+(v) (bitmap over the top-a ranks,
+ * else the sorted list).  Same triangles as wtc_tc_gpu_big. */
+int wtc_tc_arrow_gpu(const wtc_big_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len,
+                     wtc_gpu_time *t, uint64_t *triangles);
 
 /* --------------------------------------------- GPU: device-resident graphs */
 
@@ -225,6 +232,13 @@ int wtc_dgraph_download(const wtc_dgraph *dg, wtc_big_graph **out);
  * CSR to the count on the host.  `st` may be NULL. */
 int wtc_dgraph_count(wtc_ctx *ctx, wtc_dgraph *dg, uint32_t spread, uint32_t adjacency_matrix_len,
                      uint64_t w_begin, uint64_t w_end, uint64_t *triangles, wtc_count_stats *st);
+
+/* The arrow-wedge count of a resident graph (the device part of tc_arrow_GPU,
+ * experiments/tc_arrow.cu:173-289 and 345-368): arrow-wedge prefix over the
+ * edges (rows with d+ < 2 carry none), bitmap, count.  *arrow_wedges (may be
+ * NULL) receives the arrow-wedge total; st->wedges holds it too. */
+int wtc_dgraph_count_arrow(wtc_ctx *ctx, wtc_dgraph *dg, uint32_t spread, uint32_t adjacency_matrix_len,
+                           uint64_t *triangles, uint64_t *arrow_wedges, wtc_count_stats *st);
 
 /* Diagnostics for parity tests: the device wedge prefix (n+1 entries, the
  * h_wedgeSum of tc.cu:348-361), the 4095-entry search tree (the

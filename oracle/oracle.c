@@ -474,6 +474,65 @@ int orc_tc_wedge(const orc_graph *g, uint32_t spread, uint32_t a, uint64_t w_beg
     return ORC_OK;
 }
 
+/* ------------------------------------------------------- arrow-wedge TC */
+
+/* Arrow-wedge prefix, experiments/tc_arrow.cu:345-368: vi[0] = 0 and, for
+ * each edge i of row v, vi[i+1] = vi[i] + d+(colInd[i]) when d+(v) >= 2,
+ * else vi[i+1] = vi[i].  Returns the arrow-wedge total vi[m]. */
+uint64_t orc_arrow_prefix(const orc_graph *g, uint64_t *vi) {
+    vi[0] = 0;
+    for (uint32_t v = 0; v < g->n; v++) {
+        uint64_t vb = g->rowPtr[v], ve = g->rowPtr[v + 1];
+        for (uint64_t i = vb; i < ve; i++) {
+            uint32_t w = g->colInd[i];
+            uint64_t dw = ve - vb >= 2 ? g->rowPtr[w + 1] - g->rowPtr[w] : 0;
+            vi[i + 1] = vi[i] + dw;
+        }
+    }
+    return vi[g->m];
+}
+
+/* tc_arrow_GPU_kernel (experiments/tc_arrow.cu:173-289) executed
+ * sequentially: arrow wedge i of edge e = (v -> w) names u =
+ * colInd[rowPtr[w] + i - vi[e]], and counts when u is in adj+(v): a bitmap
+ * test for v in the top-a ranks (tc_arrow.cu:256-266, the same index as
+ * tc.cu:263 with v in place of w), else a list search (tc_arrow.cu:271-284).
+ * Every wedge is visited once, in index order; the block/lane split of the
+ * kernel only changes which thread visits it. */
+int orc_tc_arrow(const orc_graph *g, uint32_t a, uint64_t *arrow_wedges, uint64_t *count) {
+    const uint32_t n = g->n;
+    uint64_t *vi = (uint64_t *)malloc((g->m + 1) * sizeof(uint64_t));
+    uint64_t bm_words = orc_bitmap_words(a);
+    uint32_t *bm = (uint32_t *)calloc(bm_words ? bm_words : 1, sizeof(uint32_t));
+    if (!vi || !bm) { free(vi); free(bm); return ORC_ERR_NOMEM; }
+    uint64_t A = orc_arrow_prefix(g, vi);
+    if (a) orc_bitmap_build(g, a, bm);
+    const uint32_t thr = a ? (n > a ? n - a : 0) : n;
+    const uint64_t bm_size = orc_bitmap_bits(a);
+    const uint64_t *Ap = g->rowPtr;
+    const uint32_t *Ai = g->colInd;
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        for (uint64_t e = Ap[v]; e < Ap[v + 1]; e++) {
+            uint32_t w = Ai[e];
+            for (uint64_t k = 0; k < vi[e + 1] - vi[e]; k++) {
+                uint32_t u = Ai[Ap[w] + k];
+                if (v >= thr) {
+                    uint64_t idx = (bm_size - binom2((uint64_t)n - v)) + u - v - 1;
+                    total += (bm[idx >> 5] >> (idx & 31u)) & 1u;
+                } else {
+                    total += (uint64_t)list_contains(Ai, Ap[v], Ap[v + 1], u);
+                }
+            }
+        }
+    }
+    free(vi);
+    free(bm);
+    if (arrow_wedges) *arrow_wedges = A;
+    *count = total;
+    return ORC_OK;
+}
+
 /* ------------------------------------------------------- merge CPU baseline */
 
 typedef struct {

@@ -111,6 +111,15 @@ void orc_bitmap_build(const orc_graph *g, uint32_t a, uint32_t *words);
 int orc_tc_wedge(const orc_graph *g, uint32_t spread, uint32_t a,
                  uint64_t w_begin, uint64_t w_end, uint64_t *count);
 
+/* Arrow-wedge prefix of experiments/tc_arrow.cu:345-368 (vi: m+1 entries;
+ * edges of rows with d+ < 2 carry no wedges).  Returns the arrow-wedge total. */
+uint64_t orc_arrow_prefix(const orc_graph *g, uint64_t *vi);
+
+/* The arrow-wedge count of tc_arrow_GPU_kernel (experiments/tc_arrow.cu:
+ * 173-289): wedges v->w->u, u probed in adj+(v) (bitmap over the top-a
+ * ranks, else list search).  Writes the arrow-wedge total and the count. */
+int orc_tc_arrow(const orc_graph *g, uint32_t a, uint64_t *arrow_wedges, uint64_t *count);
+
 /* Merge-intersection count over the same oriented CSR, with `threads` POSIX
  * threads (<=0: all online cores).  This is the CPU baseline: for each v, for
  * each w in adj+(v), add |adj+(v) after w  intersect  adj+(w)|. */

@@ -47,6 +47,8 @@ _sig = {
     "orc_bitmap_index": (C.c_uint64, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]),
     "orc_bitmap_build": (None, [C.POINTER(_G), C.c_uint32, _P]),
     "orc_tc_wedge": (C.c_int, [C.POINTER(_G), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, _u64p]),
+    "orc_arrow_prefix": (C.c_uint64, [C.POINTER(_G), _P]),
+    "orc_tc_arrow": (C.c_int, [C.POINTER(_G), C.c_uint32, _u64p, _u64p]),
     "orc_tc_merge": (C.c_uint64, [C.POINTER(_G), C.c_int]),
     "orc_tc_merge_range": (C.c_uint64, [C.POINTER(_G), C.c_uint32, C.c_uint32, C.c_int]),
     "orc_decode_check": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_int, _u64p]),
@@ -168,6 +170,21 @@ def tc_wedge(g, spread, a, w_begin=0, w_end=2 ** 64 - 1) -> int:
     out = C.c_uint64()
     _rc(_lib.orc_tc_wedge(C.byref(s), spread, a, w_begin, w_end, C.byref(out)), "tc_wedge")
     return int(out.value)
+
+
+def arrow_prefix(g) -> np.ndarray:
+    s = _struct(g)
+    vi = np.zeros(g.m + 1, dtype=np.uint64)
+    _lib.orc_arrow_prefix(C.byref(s), vi.ctypes.data)
+    return vi
+
+
+def tc_arrow(g, a=0):
+    """(triangles, arrow wedges) of the arrow-wedge count."""
+    s = _struct(g)
+    A, out = C.c_uint64(), C.c_uint64()
+    _rc(_lib.orc_tc_arrow(C.byref(s), a, C.byref(A), C.byref(out)), "tc_arrow")
+    return int(out.value), int(A.value)
 
 
 def tc_merge(g, threads=0) -> int:

@@ -57,7 +57,8 @@ def oriented(pairs):
 
 
 def counts_all(g, combos=((1, 0), (3, 32), (5, 64), (8, 8192), (17, 0), (2, 4096))):
-    return {(s, a): O.tc_wedge(g, s, a) for s, a in combos} | {"merge": O.tc_merge(g, 2)}
+    return {(s, a): O.tc_wedge(g, s, a) for s, a in combos} | {"merge": O.tc_merge(g, 2)} | \
+        {("arrow", a): O.tc_arrow(g, a)[0] for a in (0, 32, 4096)}
 
 
 # ----------------------------------------------------------------- families
@@ -126,6 +127,35 @@ def test_golden_fixtures(golden):
         assert O.tc_merge(g) == fx["triangles"]
         assert O.tc_wedge(g, 5, 64) == fx["triangles"]
         assert O.tc_wedge(g, 1, 0) == fx["triangles"]
+        assert O.tc_arrow(g, 64)[0] == fx["triangles"]
+
+
+def _arrow_total_brute(g):
+    """Sum over edges (v,w) with d+(v) >= 2 of d+(w), vectorised."""
+    rp = np.asarray(g.row_ptr, np.int64)
+    d = np.diff(rp)
+    src = np.repeat(np.arange(g.n), d)
+    dst = np.asarray(g.col_ind, np.int64)
+    return int(np.sum(np.where(d[src] >= 2, d[dst], 0)))
+
+
+def test_arrow_prefix_and_total():
+    # K_n in any orientation is a transitive tournament: every path v->w->u
+    # closes, so arrow wedges = triangles = C(n,3)
+    for n in (3, 4, 9, 30):
+        g = oriented([(i, j) for i in range(n) for j in range(i + 1, n)])
+        assert O.tc_arrow(g, 0) == (n * (n - 1) * (n - 2) // 6,) * 2
+    # a path a-b-c oriented by degree has one edge per row: no arrow wedges
+    assert O.tc_arrow(oriented([(0, 1), (1, 2)]), 0) == (0, 0)
+    rnd = np.random.default_rng(4)
+    for trial in range(6):
+        n = int(rnd.integers(20, 300))
+        pairs = rnd.integers(0, n, size=(int(rnd.integers(n, 6 * n)), 2))
+        g = oriented(pairs)
+        vi = O.arrow_prefix(g)
+        assert vi[0] == 0 and np.all(np.diff(vi.astype(np.int64)) >= 0)
+        assert int(vi[-1]) == _arrow_total_brute(g) == O.tc_arrow(g, 0)[1]
+        assert O.tc_arrow(g, 0)[0] == O.tc_merge(g, 1)
 
 
 # ------------------------------------------------------------------- decode

@@ -1,0 +1,291 @@
+// wtc_arrow.hip -- the arrow-wedge count of experiments/tc_arrow.cu (and its
+// 64-bit twin experiments/tc_big_graphs_arrow.cu): every directed path
+// v -> w -> u of the oriented CSR is a wedge, closed when u is in adj+(v).
+// Each triangle v < w < u (in rank order) is one closed arrow wedge, so the
+// count equals the wedge-parallel count; the wedge set (and so the work) is
+// different: sum over edges (v,w) of d+(w) instead of sum over v of C(d+(v),2).
+//
+// Device pipeline on the context's stream:
+//   k_arrow_degrees / k_arrow_single_rows
+//       x[e] = d+(colInd[e]), then 0 for the edge of every row with d+ = 1
+//       (the host prefix of tc_arrow.cu:345-368 gives such rows no wedges)
+//   k_arrow_partials / k_scan_partials / k_arrow_scan
+//       vi = exclusive prefix of x over the m edges, in place (vi[m] = total)
+//   k_bitmap_clear / k_bitmap_build
+//       the top-a bitmap, shared with the wedge-parallel count
+//   k_count_arrow
+//       persistent grid; lane-contiguous chunks of `spread` wedges, located
+//       by two searches (edge over vi, source over rowPtr) and then stepped;
+//       u probed in adj+(v) by the bitmap for v in the top-a ranks, else by a
+//       binary search of the row.
+#include <chrono>
+#include <cstring>
+#include <mutex>
+
+#include "wtc_dev.h"
+#include "wtc_internal.h"
+#include "wtc_launch.h"
+
+namespace wtc {
+namespace {
+
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_arrow_degrees(const void *rp, const uint32_t *__restrict__ colInd,
+                                                          uint64_t m, uint64_t *__restrict__ x) {
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(rp);
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+        const uint32_t w = colInd[e];
+        x[e] = (uint64_t)(rowPtr[w + 1] - rowPtr[w]);
+    }
+}
+
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_arrow_single_rows(const void *rp, uint32_t n, uint64_t *__restrict__ x) {
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(rp);
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+        const uint64_t vb = rowPtr[v];
+        if ((uint64_t)rowPtr[v + 1] - vb == 1) x[vb] = 0;
+    }
+}
+
+// Block b sums x over [b*kScanChunk, (b+1)*kScanChunk) of [0, m).
+__global__ __launch_bounds__(kBlock) void k_arrow_partials(const uint64_t *__restrict__ x, uint64_t m,
+                                                           uint64_t *__restrict__ partials) {
+    __shared__ uint64_t s_tot[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    uint64_t acc = 0;
+#pragma unroll 4
+    for (int k = 0; k < kScanItems; k++) {
+        const uint64_t e = base + (uint64_t)k * kBlock + threadIdx.x;
+        if (e < m) acc += x[e];
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_tot[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partials[blockIdx.x] = s_tot[0] + s_tot[1] + s_tot[2] + s_tot[3];
+}
+
+// In place: each thread reads its x[e] before any thread of the block writes
+// (block_excl_scan synchronises), and blocks own disjoint ranges, so
+// vi[e] = partials[b] + (exclusive scan inside the block); the thread of the
+// last edge also writes vi[m].
+__global__ __launch_bounds__(kBlock) void k_arrow_scan(uint64_t *__restrict__ xv, uint64_t m,
+                                                       const uint64_t *__restrict__ partials) {
+    __shared__ uint64_t s_tot[kWavesPerBlock];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
+    uint64_t carry = partials[blockIdx.x];
+    for (int k = 0; k < kScanItems; k++) {
+        const uint64_t e = base + (uint64_t)k * kBlock + threadIdx.x;
+        const uint64_t x = e < m ? xv[e] : 0;
+        uint64_t tot;
+        const uint64_t ex = block_excl_scan(x, s_tot, tot);
+        if (e < m) xv[e] = carry + ex;
+        if (e + 1 == m) xv[m] = carry + ex + x;
+        carry += tot;
+    }
+}
+
+// Rightmost index i in [0, hi] with a[i] <= t, given a[0] <= t.
+template <class T>
+__device__ __forceinline__ uint64_t rightmost_le64(const T *__restrict__ a, uint64_t hi, uint64_t t) {
+    uint64_t s = 1, e = hi + 1;
+    while (s < e) {
+        const uint64_t mid = s + ((e - s) >> 1);
+        if ((uint64_t)a[mid] <= t) s = mid + 1;
+        else e = mid;
+    }
+    return s - 1;
+}
+
+// Whether u is in the ascending list[lo, hi).
+__device__ __forceinline__ bool list_has(const uint32_t *__restrict__ list, uint64_t lo, uint64_t end, uint32_t u) {
+    uint64_t hi = end;
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if (list[mid] < u) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo < end && list[lo] == u;
+}
+
+// Chunk c covers arrow wedges [c*spread, min((c+1)*spread, A)); thread t of
+// the grid takes chunks t, t + T, ...  Adjacent lanes take adjacent chunks,
+// so a wave mostly works inside one row.
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const uint32_t *__restrict__ colInd,
+                                                        uint32_t n, uint64_t m,
+                                                        const uint64_t *__restrict__ vi,
+                                                        const uint32_t *__restrict__ bitmap, uint64_t bm_size,
+                                                        uint32_t thr, uint32_t spread,
+                                                        unsigned long long *__restrict__ count) {
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(rp);
+    const uint64_t A = vi[m];
+    const uint64_t nchunks = (A + spread - 1) / spread;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    uint64_t total = 0;
+    for (uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c < nchunks; c += stride) {
+        uint64_t i = c * spread;
+        const uint64_t iend = i + spread < A ? i + spread : A;
+        // i < A = vi[m], so the edge found has vi[e] <= i < vi[e+1] and e < m;
+        // rowPtr[n] = m > e, so its source v < n has rowPtr[v] <= e < rowPtr[v+1].
+        uint64_t e = rightmost_le64(vi, m, i);
+        uint32_t v = (uint32_t)rightmost_le64(rowPtr, n, e);
+        uint64_t vb = rowPtr[v], ve = rowPtr[v + 1];
+        uint64_t eb = vi[e], ee = vi[e + 1];
+        uint64_t wb = rowPtr[colInd[e]];
+        for (; i < iend; i++) {
+            if (i >= ee) {  // next edge with wedges (i < A keeps e + 1 <= m)
+                do {
+                    e++;
+                    eb = ee;
+                    ee = vi[e + 1];
+                } while (i >= ee);
+                while (e >= ve) {  // next source row holding edge e
+                    v++;
+                    vb = ve;
+                    ve = rowPtr[v + 1];
+                }
+                wb = rowPtr[colInd[e]];
+            }
+            const uint32_t u = colInd[wb + (i - eb)];
+            if (v >= thr) {
+                const uint64_t idx = bm_size - binom2_u32(n - v) + (u - v - 1);
+                total += (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
+            } else {
+                total += list_has(colInd, vb, ve, u) ? 1u : 0u;
+            }
+        }
+    }
+    total = wave_sum(total);
+    if ((threadIdx.x & (kWave - 1)) == 0 && total) atomicAdd(count, (unsigned long long)total);
+}
+
+#define WTC_ARROW_LAUNCH(ob, KERNEL, GRID, STREAM, ...)                        \
+    [&]() -> hipError_t {                                                      \
+        if ((ob) == 8)                                                         \
+            KERNEL<uint64_t><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__);  \
+        else                                                                   \
+            KERNEL<uint32_t><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__);  \
+        return hipGetLastError();                                              \
+    }()
+
+uint32_t stride_grid(uint64_t items, uint32_t cap) {
+    const uint64_t b = (items + kBlock - 1) / kBlock;
+    return (uint32_t)(b < 1 ? 1 : b < cap ? b : cap);
+}
+
+}  // namespace
+}  // namespace wtc
+
+using namespace wtc;
+
+extern "C" {
+
+int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t a, uint64_t *triangles,
+                           uint64_t *arrow_wedges, wtc_count_stats *st) {
+    if (!c || !dg || !triangles) return WTC_ERR_ARG;
+    int rc = check_args(spread, a);
+    if (rc) return rc;
+    if (dg->m > 0xffffffffULL * kScanChunk) return WTC_ERR_RANGE;
+    WTC_HIPC(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    Slot &s = c->slot;
+    hipStream_t stream = s.stream;
+    const int ob = dg->ob;
+    const uint32_t n = dg->n;
+    const uint64_t m = dg->m;
+    const uint32_t nb = (uint32_t)((m + kScanChunk - 1) / kScanChunk);
+    WTC_HIPC(s.ws.ensure((m + 1) * sizeof(uint64_t)));
+    WTC_HIPC(s.partials.ensure(std::max<uint32_t>(nb, 1) * sizeof(uint64_t)));
+    WTC_HIPC(s.out.ensure((size_t)kMetaSlots * sizeof(uint64_t)));
+    WTC_HIPC(s.h_out.ensure((size_t)kMetaSlots * sizeof(uint64_t)));
+    GraphDesc d = make_desc(dg->rowPtr.p, dg->colInd.as<uint32_t>(), n, spread, a, 0, UINT64_MAX);
+    const uint64_t bm_words = a ? bm_words_for(a) : 0;
+    if (bm_words) {
+        WTC_HIPC(s.bm.ensure(bm_words * sizeof(uint32_t)));
+        d.bitmap = s.bm.as<uint32_t>();
+        WTC_HIPC(s.desc.ensure(sizeof(GraphDesc)));
+        WTC_HIPC(s.h_desc.ensure(sizeof(GraphDesc)));
+        std::memcpy(s.h_desc.p, &d, sizeof d);
+    }
+    uint64_t *vi = s.ws.as<uint64_t>();
+    uint64_t *out = s.out.as<uint64_t>();
+    unsigned long long *dcount = reinterpret_cast<unsigned long long *>(out);
+    const uint32_t cap = (uint32_t)std::max(c->tune.num_cus, 1) * 16u;
+
+    WTC_HIPC(hipEventRecord(s.ev[0], stream));
+    WTC_HIPC(launch_fill(reinterpret_cast<uint32_t *>(out), 2, 0u, stream));
+    if (m) {
+        WTC_HIPC(WTC_ARROW_LAUNCH(ob, k_arrow_degrees, dim3(stride_grid(m, cap)), stream, dg->rowPtr.p,
+                                  dg->colInd.as<uint32_t>(), m, vi));
+        WTC_HIPC(WTC_ARROW_LAUNCH(ob, k_arrow_single_rows, dim3(stride_grid(n, cap)), stream, dg->rowPtr.p, n, vi));
+        k_arrow_partials<<<dim3(nb), dim3(kBlock), 0, stream>>>(vi, m, s.partials.as<uint64_t>());
+        WTC_HIPC(hipGetLastError());
+        WTC_HIPC(launch_scan_partials(s.partials.as<uint64_t>(), nb, stream));
+        k_arrow_scan<<<dim3(nb), dim3(kBlock), 0, stream>>>(vi, m, s.partials.as<uint64_t>());
+        WTC_HIPC(hipGetLastError());
+    } else {
+        WTC_HIPC(launch_fill(reinterpret_cast<uint32_t *>(vi), 2, 0u, stream));  // vi[0] = A = 0
+    }
+    WTC_HIPC(hipEventRecord(s.ev[1], stream));
+    if (bm_words) {
+        WTC_HIPC(hipMemcpyAsync(s.desc.p, s.h_desc.p, sizeof(GraphDesc), hipMemcpyHostToDevice, stream));
+        WTC_HIPC(launch_bitmap(s.desc.as<GraphDesc>(), 1, ob, d.bm_rows, bm_words, stream));
+    }
+    WTC_HIPC(hipEventRecord(s.ev[2], stream));
+    // chunks of `spread` wedges, one per lane of a persistent grid
+    const uint32_t grid = cap * 2u;
+    if (m)
+        WTC_HIPC(WTC_ARROW_LAUNCH(ob, k_count_arrow, dim3(grid), stream, dg->rowPtr.p, dg->colInd.as<uint32_t>(), n,
+                                  m, vi, d.bitmap, d.bm_size, d.thr, spread, dcount));
+    WTC_HIPC(hipEventRecord(s.ev[3], stream));
+    WTC_HIPC(hipMemcpyAsync(s.h_out.p, out, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    WTC_HIPC(hipMemcpyAsync(s.h_out.as<uint64_t>() + 1, vi + m, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
+    WTC_HIPC(hipStreamSynchronize(stream));
+    auto t1 = std::chrono::steady_clock::now();
+    const uint64_t *h = s.h_out.as<uint64_t>();
+    *triangles = h[0];
+    if (arrow_wedges) *arrow_wedges = h[1];
+    if (st) {
+        float p = 0, b = 0, k = 0;
+        WTC_HIPC(hipEventElapsedTime(&p, s.ev[0], s.ev[1]));
+        WTC_HIPC(hipEventElapsedTime(&b, s.ev[1], s.ev[2]));
+        WTC_HIPC(hipEventElapsedTime(&k, s.ev[2], s.ev[3]));
+        *st = wtc_count_stats{};
+        st->prefix_ms = p;
+        st->bitmap_ms = b;
+        st->count_ms = k;
+        st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->wedges = h[1];
+        st->blocks = m ? grid : 0;
+        st->launches = m ? 1 : 0;
+    }
+    return WTC_OK;
+}
+
+int wtc_tc_arrow_gpu(const wtc_big_graph *g, uint32_t spread, uint32_t a, wtc_gpu_time *t, uint64_t *triangles) {
+    if (!g || !triangles) return WTC_ERR_ARG;
+    int rc = check_args(spread, a);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(default_ctx_mutex());
+    wtc_ctx *c;
+    if ((rc = default_context(&c))) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    wtc_dgraph *dg = nullptr;
+    if ((rc = wtc_dgraph_upload(c, g, 0, &dg))) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    wtc_count_stats st{};
+    rc = wtc_dgraph_count_arrow(c, dg, spread, a, triangles, nullptr, &st);
+    wtc_dgraph_free(dg);
+    c->slot.release_device();  // per-call buffers, like the reference
+    if (rc) return rc;
+    if (t) {
+        t->copy += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        t->exec += st.prefix_ms + st.bitmap_ms + st.count_ms;
+    }
+    return WTC_OK;
+}
+
+}  // extern "C"

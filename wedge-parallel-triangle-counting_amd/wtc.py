@@ -116,6 +116,7 @@ _sig = {
     "wtc_tc_gpu": (C.c_int, [C.POINTER(_Graph32), C.c_uint32, C.c_uint32, C.POINTER(GpuTime),
                             C.POINTER(C.c_uint64)]),
     "wtc_tc_gpu_big": (C.c_int, [_BGP, C.c_uint32, C.c_uint32, C.POINTER(GpuTime), C.POINTER(C.c_uint64)]),
+    "wtc_tc_arrow_gpu": (C.c_int, [_BGP, C.c_uint32, C.c_uint32, C.POINTER(GpuTime), C.POINTER(C.c_uint64)]),
     "wtc_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "wtc_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "wtc_ctx_destroy": (None, [_P]),
@@ -129,6 +130,8 @@ _sig = {
     "wtc_dgraph_download": (C.c_int, [_P, C.POINTER(_BGP)]),
     "wtc_dgraph_count": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64,
                                   C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
+    "wtc_dgraph_count_arrow": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64),
+                                        C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
     "wtc_ctx_debug_prefix": (C.c_int, [_P, C.c_uint64, _P, _P]),
     "wtc_ctx_debug_bitmap": (C.c_int, [_P, C.c_uint64, _P]),
     "wtc_debug_decode": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _P]),
@@ -342,6 +345,15 @@ def tc_gpu_big(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -> int:
     return int(cnt.value)
 
 
+def tc_arrow_gpu(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -> int:
+    """tc_arrow_GPU of experiments/tc_arrow.cu:310 (arrow wedges v->w->u)."""
+    s = g._struct()
+    cnt = C.c_uint64()
+    _check(_lib.wtc_tc_arrow_gpu(C.byref(s), spread, a, C.byref(t) if t is not None else None, C.byref(cnt)),
+           "tc_arrow_gpu")
+    return int(cnt.value)
+
+
 def device_count() -> int:
     c = C.c_int()
     _lib.wtc_device_count(C.byref(c))
@@ -461,6 +473,14 @@ class DeviceGraph:
         _check(_lib.wtc_dgraph_count(self.ctx._p, self._p, spread, a, w_begin, w_end, C.byref(cnt), C.byref(st)),
                "dgraph_count")
         return (int(cnt.value), st) if stats else int(cnt.value)
+
+    def count_arrow(self, spread: int, a: int, stats: bool = False):
+        """Arrow-wedge count: (triangles, arrow wedges[, stats])."""
+        cnt, aw = C.c_uint64(), C.c_uint64()
+        st = CountStats()
+        _check(_lib.wtc_dgraph_count_arrow(self.ctx._p, self._p, spread, a, C.byref(cnt), C.byref(aw),
+                                           C.byref(st)), "dgraph_count_arrow")
+        return (int(cnt.value), int(aw.value), st) if stats else (int(cnt.value), int(aw.value))
 
     def free(self):
         if self._p:
