@@ -200,6 +200,40 @@ def small_kernel_bench(ctx, k, base_seed, reps=7):
     return res
 
 
+def arrow_bench(scale, seed, spreads=(8, 16, 32), a=131072, iters=3):
+    """The arrow-wedge count (experiments/tc_arrow.cu) against the
+    wedge-parallel count on the same resident R-MAT graph: per spread, the
+    median count-kernel time over `iters` counts after a warm-up, the arrow
+    wedges checked and the rate; both counts must agree."""
+    ctx = wtc.Context(0)
+    dg = wtc.DeviceGraph.gen_rmat(ctx, scale, 16, seed=seed)
+    n, m, a_w = dg.n, dg.m, a
+    dg.count(12, a_w)
+    times, tri, W = [], 0, 0
+    for _ in range(iters):
+        tri, st = dg.count(12, a_w, stats=True)
+        times.append(st.count_ms)
+        W = st.wedges
+    res = {"workload": f"rmat{scale}", "n": n, "m": m, "triangles": tri,
+           "wedge_parallel": {"spread": 12, "a": a_w, "wedges": W, "count_ms": statistics.median(times),
+                              "checks_per_s": W / statistics.median(times) * 1e3}, "arrow": []}
+    for s in spreads:
+        dg.count_arrow(s, a)
+        times = []
+        for _ in range(iters):
+            t, A, st = dg.count_arrow(s, a, stats=True)
+            if t != tri:
+                raise SystemExit(f"arrow count {t} != wedge-parallel count {tri} (s={s}, a={a})")
+            times.append(st.count_ms)
+        ms = statistics.median(times)
+        res["arrow"].append({"spread": s, "a": a, "arrow_wedges": A, "count_ms": ms, "prefix_ms": st.prefix_ms,
+                             "checks_per_s": A / ms * 1e3})
+        log(f"arrow s={s} a={a}: {A} arrow wedges, count {ms:.1f} ms")
+    dg.free()
+    ctx.close()
+    return res
+
+
 def reference_configs(iters=10):
     """The reference's own measurement configurations through the tc_GPU
     drop-in (per call: allocate, copy, count, copy back, free): config 1
@@ -475,6 +509,8 @@ def main():
     ap.add_argument("--batch-only", action="store_true", help="run only the batched vs plain comparison")
     ap.add_argument("--reference-configs", action="store_true",
                     help="run only the reference's config 1 and config-2 sweep through the tc_GPU drop-in")
+    ap.add_argument("--arrow-only", action="store_true",
+                    help="run only the arrow-wedge count against the wedge-parallel count (R-MAT --scale)")
     ap.add_argument("--small-batches-only", action="store_true",
                     help="run only the small-graph kernel vs wedge kernel comparison on batches of small graphs")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving (Poisson arrivals) workload")
@@ -511,6 +547,9 @@ def main():
         return
     if args.reference_configs:
         print(json.dumps(reference_configs()), flush=True)
+        return
+    if args.arrow_only:
+        print(json.dumps(arrow_bench(args.scale, args.seed)), flush=True)
         return
     if args.small_batches_only:
         ctx = wtc.Context(0)

@@ -99,15 +99,17 @@ __device__ __forceinline__ uint64_t rightmost_le64(const T *__restrict__ a, uint
     return s - 1;
 }
 
-// Whether u is in the ascending list[lo, hi).
-__device__ __forceinline__ bool list_has(const uint32_t *__restrict__ list, uint64_t lo, uint64_t end, uint32_t u) {
+// Lower bound of u in the ascending list[lo, end): first position whose id
+// is >= u, or end.
+__device__ __forceinline__ uint64_t lower_bound(const uint32_t *__restrict__ list, uint64_t lo, uint64_t end,
+                                                uint32_t u) {
     uint64_t hi = end;
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
         if (list[mid] < u) lo = mid + 1;
         else hi = mid;
     }
-    return lo < end && list[lo] == u;
+    return lo;
 }
 
 // Chunk c covers arrow wedges [c*spread, min((c+1)*spread, A)); thread t of
@@ -135,6 +137,9 @@ __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const ui
         uint64_t vb = rowPtr[v], ve = rowPtr[v + 1];
         uint64_t eb = vi[e], ee = vi[e + 1];
         uint64_t wb = rowPtr[colInd[e]];
+        // the u of one edge ascend, so each search starts at the previous
+        // lower bound in adj+(v) (reset when the edge changes)
+        uint64_t lo = vb;
         for (; i < iend; i++) {
             if (i >= ee) {  // next edge with wedges (i < A keeps e + 1 <= m)
                 do {
@@ -148,13 +153,15 @@ __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const ui
                     ve = rowPtr[v + 1];
                 }
                 wb = rowPtr[colInd[e]];
+                lo = vb;
             }
             const uint32_t u = colInd[wb + (i - eb)];
             if (v >= thr) {
                 const uint64_t idx = bm_size - binom2_u32(n - v) + (u - v - 1);
                 total += (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
             } else {
-                total += list_has(colInd, vb, ve, u) ? 1u : 0u;
+                lo = lower_bound(colInd, lo, ve, u);
+                total += (lo < ve && colInd[lo] == u) ? 1u : 0u;
             }
         }
     }
