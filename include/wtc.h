@@ -159,13 +159,15 @@ int wtc_tc_gpu(const wtc_graph *graph, uint32_t spread, uint32_t adjacency_matri
 /* tc_GPU of experiments/tc_big_graphs.cu:315-447 (64-bit offsets). */
 int wtc_tc_gpu_big(const wtc_big_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len,
                    wtc_gpu_time *t, uint64_t *triangles);
-/* tc_arrow_GPU, experiments/tc_arrow.cu:310-441 (64-bit offsets:
- * experiments/tc_big_graphs_arrow.cu): the arrow-wedge count, per call
- * allocating, copying, counting and freeing like wtc_tc_gpu.  Arrow wedges are
- * the paths v->w->u; u is probed in adj+(v) (bitmap over the top-a ranks,
- * else the sorted list).  Same triangles as wtc_tc_gpu_big. */
-int wtc_tc_arrow_gpu(const wtc_big_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len,
-                     wtc_gpu_time *t, uint64_t *triangles);
+/* tc_arrow_GPU, experiments/tc_arrow.cu:311-441: the arrow-wedge count, per
+ * call allocating, copying, counting and freeing like wtc_tc_gpu.  Arrow
+ * wedges are the paths v->w->u; u is probed in adj+(v) (bitmap over the
+ * top-a ranks, else the sorted list).  Same triangles as wtc_tc_gpu. */
+int wtc_tc_arrow_gpu(const wtc_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len, wtc_gpu_time *t,
+                     uint64_t *triangles);
+/* tc_arrow_GPU of experiments/tc_big_graphs_arrow.cu:334 (64-bit offsets). */
+int wtc_tc_arrow_gpu_big(const wtc_big_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len,
+                         wtc_gpu_time *t, uint64_t *triangles);
 
 /* --------------------------------------------- GPU: device-resident graphs */
 

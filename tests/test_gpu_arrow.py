@@ -67,9 +67,9 @@ def test_arrow_reference_style_entry_point():
     assert wtc.tc_arrow_gpu(g, 5, 8192, t) == expect
     c1, e1 = t.copy, t.exec
     assert c1 > 0 and e1 > 0
-    assert wtc.tc_arrow_gpu(g, 1, 0, t) == expect
+    assert wtc.tc_arrow_gpu_big(g, 1, 0, t) == expect
     assert t.copy > c1 and t.exec > e1
     with pytest.raises(wtc.WtcError):
         wtc.tc_arrow_gpu(g, 0, 0)
     with pytest.raises(wtc.WtcError):
-        wtc.tc_arrow_gpu(g, 5, 33)
+        wtc.tc_arrow_gpu_big(g, 5, 33)

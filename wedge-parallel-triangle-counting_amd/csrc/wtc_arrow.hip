@@ -21,6 +21,8 @@
 #include <chrono>
 #include <cstring>
 #include <mutex>
+#include <new>
+#include <vector>
 
 #include "wtc_dev.h"
 #include "wtc_internal.h"
@@ -272,7 +274,8 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
     return WTC_OK;
 }
 
-int wtc_tc_arrow_gpu(const wtc_big_graph *g, uint32_t spread, uint32_t a, wtc_gpu_time *t, uint64_t *triangles) {
+int wtc_tc_arrow_gpu_big(const wtc_big_graph *g, uint32_t spread, uint32_t a, wtc_gpu_time *t,
+                         uint64_t *triangles) {
     if (!g || !triangles) return WTC_ERR_ARG;
     int rc = check_args(spread, a);
     if (rc) return rc;
@@ -293,6 +296,24 @@ int wtc_tc_arrow_gpu(const wtc_big_graph *g, uint32_t spread, uint32_t a, wtc_gp
         t->exec += st.prefix_ms + st.bitmap_ms + st.count_ms;
     }
     return WTC_OK;
+}
+
+// 32-bit offsets (tc_arrow.cu's GRAPH_TYPE): the offsets are widened on the
+// host and the upload narrows them again on the device side.
+int wtc_tc_arrow_gpu(const wtc_graph *g, uint32_t spread, uint32_t a, wtc_gpu_time *t, uint64_t *triangles) {
+    if (!g || !triangles || !g->rowPtr || g->numVertices == 0) return WTC_ERR_ARG;
+    std::vector<uint64_t> rp;
+    try {
+        rp.assign(g->rowPtr, g->rowPtr + (size_t)g->numVertices + 1);
+    } catch (const std::bad_alloc &) {
+        return WTC_ERR_NOMEM;
+    }
+    wtc_big_graph b{};
+    b.numVertices = g->numVertices;
+    b.numEdges = g->numEdges;
+    b.rowPtr = rp.data();
+    b.colInd = g->colInd;
+    return wtc_tc_arrow_gpu_big(&b, spread, a, t, triangles);
 }
 
 }  // extern "C"

@@ -116,7 +116,10 @@ _sig = {
     "wtc_tc_gpu": (C.c_int, [C.POINTER(_Graph32), C.c_uint32, C.c_uint32, C.POINTER(GpuTime),
                             C.POINTER(C.c_uint64)]),
     "wtc_tc_gpu_big": (C.c_int, [_BGP, C.c_uint32, C.c_uint32, C.POINTER(GpuTime), C.POINTER(C.c_uint64)]),
-    "wtc_tc_arrow_gpu": (C.c_int, [_BGP, C.c_uint32, C.c_uint32, C.POINTER(GpuTime), C.POINTER(C.c_uint64)]),
+    "wtc_tc_arrow_gpu": (C.c_int, [C.POINTER(_Graph32), C.c_uint32, C.c_uint32, C.POINTER(GpuTime),
+                                  C.POINTER(C.c_uint64)]),
+    "wtc_tc_arrow_gpu_big": (C.c_int, [_BGP, C.c_uint32, C.c_uint32, C.POINTER(GpuTime),
+                                      C.POINTER(C.c_uint64)]),
     "wtc_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "wtc_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "wtc_ctx_destroy": (None, [_P]),
@@ -346,11 +349,25 @@ def tc_gpu_big(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -> int:
 
 
 def tc_arrow_gpu(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -> int:
-    """tc_arrow_GPU of experiments/tc_arrow.cu:310 (arrow wedges v->w->u)."""
+    """tc_arrow_GPU of experiments/tc_arrow.cu:311 (32-bit offsets; arrow wedges v->w->u)."""
+    s = g._struct()
+    g32 = C.POINTER(_Graph32)()
+    _check(_lib.wtc_narrow_graph(C.byref(s), C.byref(g32)), "narrow_graph")
+    try:
+        cnt = C.c_uint64()
+        _check(_lib.wtc_tc_arrow_gpu(g32, spread, a, C.byref(t) if t is not None else None, C.byref(cnt)),
+               "tc_arrow_gpu")
+        return int(cnt.value)
+    finally:
+        _lib.wtc_free_graph32(g32)
+
+
+def tc_arrow_gpu_big(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -> int:
+    """tc_arrow_GPU of experiments/tc_big_graphs_arrow.cu:334 (64-bit offsets)."""
     s = g._struct()
     cnt = C.c_uint64()
-    _check(_lib.wtc_tc_arrow_gpu(C.byref(s), spread, a, C.byref(t) if t is not None else None, C.byref(cnt)),
-           "tc_arrow_gpu")
+    _check(_lib.wtc_tc_arrow_gpu_big(C.byref(s), spread, a, C.byref(t) if t is not None else None,
+                                     C.byref(cnt)), "tc_arrow_gpu_big")
     return int(cnt.value)
 
 
