@@ -200,11 +200,13 @@ def small_kernel_bench(ctx, k, base_seed, reps=7):
     return res
 
 
-def arrow_bench(scale, seed, spreads=(8, 16, 32), a=131072, iters=3):
+def arrow_bench(scale, seed, spreads=(8, 16, 32), a=131072, iters=3, baselines=False):
     """The arrow-wedge count (experiments/tc_arrow.cu) against the
     wedge-parallel count on the same resident R-MAT graph: per spread, the
     median count-kernel time over `iters` counts after a warm-up, the arrow
-    wedges checked and the rate; both counts must agree."""
+    wedges checked and the rate; all counts must agree.  With `baselines`,
+    also one timed count of each vertex-/edge-parallel kernel and wedge style
+    (experiments/tc_vertex.cu, tc_edge.cu)."""
     ctx = wtc.Context(0)
     dg = wtc.DeviceGraph.gen_rmat(ctx, scale, 16, seed=seed)
     n, m, a_w = dg.n, dg.m, a
@@ -229,6 +231,16 @@ def arrow_bench(scale, seed, spreads=(8, 16, 32), a=131072, iters=3):
         res["arrow"].append({"spread": s, "a": a, "arrow_wedges": A, "count_ms": ms, "prefix_ms": st.prefix_ms,
                              "checks_per_s": A / ms * 1e3})
         log(f"arrow s={s} a={a}: {A} arrow wedges, count {ms:.1f} ms")
+    if baselines:  # experiments/tc_vertex.cu and tc_edge.cu, every wedge style
+        res["baselines"] = []
+        for par, pname in ((wtc.PARALLEL_VERTEX, "vertex"), (wtc.PARALLEL_EDGE, "edge")):
+            for style, sname in ((wtc.WEDGE_OUTGOING, "outgoing"), (wtc.WEDGE_ARROW, "arrow"),
+                                 (wtc.WEDGE_MIXED, "mixed")):
+                t, st = dg.count_baseline(par, style, a, stats=True)
+                if t != tri:
+                    raise SystemExit(f"{pname}/{sname} count {t} != wedge-parallel count {tri}")
+                res["baselines"].append({"parallelism": pname, "style": sname, "a": a, "count_ms": st.count_ms})
+                log(f"{pname}-parallel {sname}: count {st.count_ms:.1f} ms")
     dg.free()
     ctx.close()
     return res
@@ -511,6 +523,8 @@ def main():
                     help="run only the reference's config 1 and config-2 sweep through the tc_GPU drop-in")
     ap.add_argument("--arrow-only", action="store_true",
                     help="run only the arrow-wedge count against the wedge-parallel count (R-MAT --scale)")
+    ap.add_argument("--baselines", action="store_true",
+                    help="with --arrow-only: also time the vertex-/edge-parallel baselines")
     ap.add_argument("--small-batches-only", action="store_true",
                     help="run only the small-graph kernel vs wedge kernel comparison on batches of small graphs")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving (Poisson arrivals) workload")
@@ -549,7 +563,7 @@ def main():
         print(json.dumps(reference_configs()), flush=True)
         return
     if args.arrow_only:
-        print(json.dumps(arrow_bench(args.scale, args.seed)), flush=True)
+        print(json.dumps(arrow_bench(args.scale, args.seed, baselines=args.baselines)), flush=True)
         return
     if args.small_batches_only:
         ctx = wtc.Context(0)

@@ -169,6 +169,25 @@ int wtc_tc_arrow_gpu(const wtc_graph *graph, uint32_t spread, uint32_t adjacency
 int wtc_tc_arrow_gpu_big(const wtc_big_graph *graph, uint32_t spread, uint32_t adjacency_matrix_len,
                          wtc_gpu_time *t, uint64_t *triangles);
 
+/* wedge_style_t of experiments/tc_vertex.cu:57 (the -w flag). */
+#define WTC_WEDGE_OUTGOING 0 /* u after w in adj+(v), probed in adj+(w) */
+#define WTC_WEDGE_ARROW 1    /* u in adj+(w), probed in adj+(v)         */
+#define WTC_WEDGE_MIXED 2    /* the shorter list walked, the other probed */
+#define WTC_PARALLEL_VERTEX 0
+#define WTC_PARALLEL_EDGE 1
+/* tc_vertex_GPU, experiments/tc_vertex.cu:287 (one lane per vertex), and its
+ * 64-bit twin experiments/tc_big_graphs_vertex.cu; per call like wtc_tc_gpu. */
+int wtc_tc_vertex_gpu(const wtc_graph *graph, uint32_t adjacency_matrix_len, int style, wtc_gpu_time *t,
+                      uint64_t *triangles);
+int wtc_tc_vertex_gpu_big(const wtc_big_graph *graph, uint32_t adjacency_matrix_len, int style,
+                          wtc_gpu_time *t, uint64_t *triangles);
+/* tc_edge_GPU, experiments/tc_edge.cu:374 (one lane per oriented edge), and
+ * its 64-bit twin experiments/tc_big_graphs_edge.cu. */
+int wtc_tc_edge_gpu(const wtc_graph *graph, uint32_t adjacency_matrix_len, int style, wtc_gpu_time *t,
+                    uint64_t *triangles);
+int wtc_tc_edge_gpu_big(const wtc_big_graph *graph, uint32_t adjacency_matrix_len, int style,
+                        wtc_gpu_time *t, uint64_t *triangles);
+
 /* --------------------------------------------- GPU: device-resident graphs */
 
 typedef struct wtc_ctx wtc_ctx;       /* HIP stream + reusable workspace */
@@ -241,6 +260,13 @@ int wtc_dgraph_count(wtc_ctx *ctx, wtc_dgraph *dg, uint32_t spread, uint32_t adj
  * NULL) receives the arrow-wedge total; st->wedges holds it too. */
 int wtc_dgraph_count_arrow(wtc_ctx *ctx, wtc_dgraph *dg, uint32_t spread, uint32_t adjacency_matrix_len,
                            uint64_t *triangles, uint64_t *arrow_wedges, wtc_count_stats *st);
+
+/* The vertex- or edge-parallel baseline count of a resident graph (the device
+ * part of tc_vertex_GPU / tc_edge_GPU, experiments/tc_vertex.cu:113-274,
+ * experiments/tc_edge.cu:179-352): parallelism WTC_PARALLEL_*, style
+ * WTC_WEDGE_*.  st->bitmap_ms and st->count_ms are filled; st->wedges is 0. */
+int wtc_dgraph_count_baseline(wtc_ctx *ctx, wtc_dgraph *dg, int parallelism, int style,
+                              uint32_t adjacency_matrix_len, uint64_t *triangles, wtc_count_stats *st);
 
 /* Diagnostics for parity tests: the device wedge prefix (n+1 entries, the
  * h_wedgeSum of tc.cu:348-361), the 4095-entry search tree (the

@@ -533,6 +533,55 @@ int orc_tc_arrow(const orc_graph *g, uint32_t a, uint64_t *arrow_wedges, uint64_
     return ORC_OK;
 }
 
+/* ------------------------------------------- vertex/edge-parallel styles */
+
+static uint32_t probe_has(const orc_graph *g, const uint32_t *bm, uint32_t thr, uint64_t bm_size, uint32_t x,
+                          uint64_t lo, uint64_t hi, uint32_t u) {
+    if (x >= thr) {
+        uint64_t idx = (bm_size - binom2((uint64_t)g->n - x)) + u - x - 1;
+        return (bm[idx >> 5] >> (idx & 31u)) & 1u;
+    }
+    return (uint32_t)list_contains(g->colInd, lo, hi, u);
+}
+
+/* The per-edge work of tc_vertex_{outgoing,arrow,mixed}_GPU_kernel
+ * (experiments/tc_vertex.cu:113-274) and tc_edge_*_GPU_kernel
+ * (experiments/tc_edge.cu:179-352), done for every edge in order: the vertex-
+ * and edge-parallel kernels only differ in which thread does an edge.
+ * style 0 outgoing, 1 arrow, 2 mixed (tc_vertex.cu:57). */
+int orc_tc_style(const orc_graph *g, int style, uint32_t a, uint64_t *count) {
+    if (style < 0 || style > 2) return ORC_ERR_ARG;
+    const uint32_t n = g->n;
+    uint64_t bm_words = orc_bitmap_words(a);
+    uint32_t *bm = (uint32_t *)calloc(bm_words ? bm_words : 1, sizeof(uint32_t));
+    if (!bm) return ORC_ERR_NOMEM;
+    if (a) orc_bitmap_build(g, a, bm);
+    const uint32_t thr = a ? (n > a ? n - a : 0) : n;
+    const uint64_t bm_size = orc_bitmap_bits(a);
+    const uint64_t *Ap = g->rowPtr;
+    const uint32_t *Ai = g->colInd;
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < n; v++) {
+        uint64_t vb = Ap[v], ve = Ap[v + 1];
+        for (uint64_t i = vb; i < ve; i++) {
+            uint32_t w = Ai[i];
+            uint64_t wb = Ap[w], we = Ap[w + 1];
+            int walk_v = style == 0 || (style == 2 && ve - (i + 1) <= we - wb);
+            if (walk_v) {
+                for (uint64_t j = i + 1; j < ve; j++) total += probe_has(g, bm, thr, bm_size, w, wb, we, Ai[j]);
+            } else {
+                /* mixed searches after w (tc_vertex.cu:258; tc_edge.cu:337 searches the whole row,
+                 * the same membership since u > w), arrow the whole row (tc_vertex.cu:187) */
+                uint64_t lo = style == 2 ? i + 1 : vb;
+                for (uint64_t j = wb; j < we; j++) total += probe_has(g, bm, thr, bm_size, v, lo, ve, Ai[j]);
+            }
+        }
+    }
+    free(bm);
+    *count = total;
+    return ORC_OK;
+}
+
 /* ------------------------------------------------------- merge CPU baseline */
 
 typedef struct {

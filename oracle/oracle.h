@@ -120,6 +120,10 @@ uint64_t orc_arrow_prefix(const orc_graph *g, uint64_t *vi);
  * ranks, else list search).  Writes the arrow-wedge total and the count. */
 int orc_tc_arrow(const orc_graph *g, uint32_t a, uint64_t *arrow_wedges, uint64_t *count);
 
+/* The vertex-/edge-parallel baselines' count in wedge style 0 outgoing,
+ * 1 arrow, 2 mixed (experiments/tc_vertex.cu:113-274, tc_edge.cu:179-352). */
+int orc_tc_style(const orc_graph *g, int style, uint32_t a, uint64_t *count);
+
 /* Merge-intersection count over the same oriented CSR, with `threads` POSIX
  * threads (<=0: all online cores).  This is the CPU baseline: for each v, for
  * each w in adj+(v), add |adj+(v) after w  intersect  adj+(w)|. */

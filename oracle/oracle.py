@@ -49,6 +49,7 @@ _sig = {
     "orc_tc_wedge": (C.c_int, [C.POINTER(_G), C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, _u64p]),
     "orc_arrow_prefix": (C.c_uint64, [C.POINTER(_G), _P]),
     "orc_tc_arrow": (C.c_int, [C.POINTER(_G), C.c_uint32, _u64p, _u64p]),
+    "orc_tc_style": (C.c_int, [C.POINTER(_G), C.c_int, C.c_uint32, _u64p]),
     "orc_tc_merge": (C.c_uint64, [C.POINTER(_G), C.c_int]),
     "orc_tc_merge_range": (C.c_uint64, [C.POINTER(_G), C.c_uint32, C.c_uint32, C.c_int]),
     "orc_decode_check": (C.c_uint64, [C.c_uint64, C.c_uint64, C.c_int, _u64p]),
@@ -185,6 +186,14 @@ def tc_arrow(g, a=0):
     A, out = C.c_uint64(), C.c_uint64()
     _rc(_lib.orc_tc_arrow(C.byref(s), a, C.byref(A), C.byref(out)), "tc_arrow")
     return int(out.value), int(A.value)
+
+
+def tc_style(g, style, a=0) -> int:
+    """Count of the vertex-/edge-parallel baselines in wedge style 0/1/2."""
+    s = _struct(g)
+    out = C.c_uint64()
+    _rc(_lib.orc_tc_style(C.byref(s), style, a, C.byref(out)), "tc_style")
+    return int(out.value)
 
 
 def tc_merge(g, threads=0) -> int:

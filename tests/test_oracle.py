@@ -58,7 +58,8 @@ def oriented(pairs):
 
 def counts_all(g, combos=((1, 0), (3, 32), (5, 64), (8, 8192), (17, 0), (2, 4096))):
     return {(s, a): O.tc_wedge(g, s, a) for s, a in combos} | {"merge": O.tc_merge(g, 2)} | \
-        {("arrow", a): O.tc_arrow(g, a)[0] for a in (0, 32, 4096)}
+        {("arrow", a): O.tc_arrow(g, a)[0] for a in (0, 32, 4096)} | \
+        {("style", k, a): O.tc_style(g, k, a) for k in (0, 1, 2) for a in (0, 64)}
 
 
 # ----------------------------------------------------------------- families
@@ -128,6 +129,7 @@ def test_golden_fixtures(golden):
         assert O.tc_wedge(g, 5, 64) == fx["triangles"]
         assert O.tc_wedge(g, 1, 0) == fx["triangles"]
         assert O.tc_arrow(g, 64)[0] == fx["triangles"]
+        assert [O.tc_style(g, k, 64) for k in (0, 1, 2)] == [fx["triangles"]] * 3
 
 
 def _arrow_total_brute(g):

@@ -1,5 +1,8 @@
-// wtc_arrow.hip -- the arrow-wedge count of experiments/tc_arrow.cu (and its
-// 64-bit twin experiments/tc_big_graphs_arrow.cu): every directed path
+// wtc_alt.hip -- the reference's alternative counts from experiments/, on the
+// same resident CSR and top-a bitmap as the wedge-parallel count.
+//
+// 1. The arrow-wedge count of experiments/tc_arrow.cu (and its 64-bit twin
+// experiments/tc_big_graphs_arrow.cu): every directed path
 // v -> w -> u of the oriented CSR is a wedge, closed when u is in adj+(v).
 // Each triangle v < w < u (in rank order) is one closed arrow wedge, so the
 // count equals the wedge-parallel count; the wedge set (and so the work) is
@@ -18,6 +21,16 @@
 //       by two searches (edge over vi, source over rowPtr) and then stepped;
 //       u probed in adj+(v) by the bitmap for v in the top-a ranks, else by a
 //       binary search of the row.
+//
+// 2. The vertex- and edge-parallel baselines of experiments/tc_vertex.cu and
+// experiments/tc_edge.cu (k_count_vertex / k_count_edge): one lane per vertex,
+// or per oriented edge (its source found by a search of rowPtr), with the
+// reference's three wedge styles per edge (v,w):
+//   outgoing  u over adj+(v) after w, probed in adj+(w)
+//   arrow     u over adj+(w), probed in adj+(v)
+//   mixed     whichever of the two lists is shorter is walked, the other probed
+// Their work per lane is a whole vertex or edge, so they are load-imbalanced by
+// design; they are kept as the baselines the wedge-parallel count replaces.
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -171,6 +184,87 @@ __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const ui
     if ((threadIdx.x & (kWave - 1)) == 0 && total) atomicAdd(count, (unsigned long long)total);
 }
 
+// Triangle-closing probe of u: the bitmap row of x when x is in the top-a
+// ranks, else a binary search of list[lo, hi).
+struct Probe {
+    const uint32_t *bitmap;
+    uint64_t bm_size;
+    uint32_t thr, n;
+    __device__ __forceinline__ uint32_t has(const uint32_t *__restrict__ colInd, uint32_t x, uint64_t lo,
+                                            uint64_t hi, uint32_t u) const {
+        if (x >= thr) {
+            const uint64_t idx = bm_size - binom2_u32(n - x) + (u - x - 1);
+            return (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
+        }
+        const uint64_t p = lower_bound(colInd, lo, hi, u);
+        return (p < hi && colInd[p] == u) ? 1u : 0u;
+    }
+};
+
+// Closed wedges of edge i = (v, w) of row [vb, ve) in wedge style `Style`
+// (tc_vertex.cu:113-274 / tc_edge.cu:179-352, per edge).
+template <class OffT, int Style>
+__device__ __forceinline__ uint64_t edge_closed(const OffT *__restrict__ rowPtr, const uint32_t *__restrict__ colInd,
+                                                const Probe &pr, uint32_t v, uint64_t vb, uint64_t ve, uint64_t i) {
+    const uint32_t w = colInd[i];
+    const uint64_t wb = rowPtr[w], we = rowPtr[w + 1];
+    bool walk_v = Style == 0;
+    if (Style == 2) walk_v = ve - (i + 1) <= we - wb;
+    uint64_t c = 0;
+    if (walk_v) {  // u after w in adj+(v), probed in adj+(w)
+        for (uint64_t j = i + 1; j < ve; j++) c += pr.has(colInd, w, wb, we, colInd[j]);
+    } else {  // u in adj+(w), probed in adj+(v) (after w for the mixed style)
+        const uint64_t lo = Style == 2 ? i + 1 : vb;
+        for (uint64_t j = wb; j < we; j++) c += pr.has(colInd, v, lo, ve, colInd[j]);
+    }
+    return c;
+}
+
+template <class OffT, int Style>
+__global__ __launch_bounds__(kBlock) void k_count_vertex(const void *rp, const uint32_t *__restrict__ colInd,
+                                                         uint32_t n, Probe pr,
+                                                         unsigned long long *__restrict__ count) {
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(rp);
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    uint64_t total = 0;
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += stride) {
+        const uint64_t vb = rowPtr[v], ve = rowPtr[v + 1];
+        for (uint64_t i = vb; i < ve; i++) total += edge_closed<OffT, Style>(rowPtr, colInd, pr, (uint32_t)v, vb, ve, i);
+    }
+    total = wave_sum(total);
+    if ((threadIdx.x & (kWave - 1)) == 0 && total) atomicAdd(count, (unsigned long long)total);
+}
+
+template <class OffT, int Style>
+__global__ __launch_bounds__(kBlock) void k_count_edge(const void *rp, const uint32_t *__restrict__ colInd,
+                                                       uint32_t n, uint64_t m, Probe pr,
+                                                       unsigned long long *__restrict__ count) {
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(rp);
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    uint64_t total = 0;
+    for (uint64_t e = (uint64_t)blockIdx.x * kBlock + threadIdx.x; e < m; e += stride) {
+        // rowPtr[n] = m > e: the source v < n has rowPtr[v] <= e < rowPtr[v+1]
+        const uint32_t v = (uint32_t)rightmost_le64(rowPtr, n, e);
+        total += edge_closed<OffT, Style>(rowPtr, colInd, pr, v, rowPtr[v], rowPtr[v + 1], e);
+    }
+    total = wave_sum(total);
+    if ((threadIdx.x & (kWave - 1)) == 0 && total) atomicAdd(count, (unsigned long long)total);
+}
+
+template <int Style>
+hipError_t launch_baseline(int ob, int edge_parallel, dim3 grid, hipStream_t st, const void *rp,
+                           const uint32_t *colInd, uint32_t n, uint64_t m, const Probe &pr,
+                           unsigned long long *count) {
+    if (edge_parallel) {
+        if (ob == 8) k_count_edge<uint64_t, Style><<<grid, dim3(kBlock), 0, st>>>(rp, colInd, n, m, pr, count);
+        else k_count_edge<uint32_t, Style><<<grid, dim3(kBlock), 0, st>>>(rp, colInd, n, m, pr, count);
+    } else {
+        if (ob == 8) k_count_vertex<uint64_t, Style><<<grid, dim3(kBlock), 0, st>>>(rp, colInd, n, pr, count);
+        else k_count_vertex<uint32_t, Style><<<grid, dim3(kBlock), 0, st>>>(rp, colInd, n, pr, count);
+    }
+    return hipGetLastError();
+}
+
 #define WTC_ARROW_LAUNCH(ob, KERNEL, GRID, STREAM, ...)                        \
     [&]() -> hipError_t {                                                      \
         if ((ob) == 8)                                                         \
@@ -179,6 +273,29 @@ __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const ui
             KERNEL<uint32_t><<<GRID, dim3(kBlock), 0, STREAM>>>(__VA_ARGS__);  \
         return hipGetLastError();                                              \
     }()
+
+// Descriptor of a resident graph with its top-a bitmap in the slot's
+// workspace; the descriptor is staged in the slot's pinned buffer for
+// enqueue_bitmap.
+int prepare_bitmap(Slot &s, const wtc_dgraph *dg, uint32_t a, GraphDesc &d, uint64_t &bm_words) {
+    d = make_desc(dg->rowPtr.p, dg->colInd.as<uint32_t>(), dg->n, 1, a, 0, UINT64_MAX);
+    bm_words = a ? bm_words_for(a) : 0;
+    if (!bm_words) return WTC_OK;
+    WTC_HIPC(s.bm.ensure(bm_words * sizeof(uint32_t)));
+    d.bitmap = s.bm.as<uint32_t>();
+    WTC_HIPC(s.desc.ensure(sizeof(GraphDesc)));
+    WTC_HIPC(s.h_desc.ensure(sizeof(GraphDesc)));
+    std::memcpy(s.h_desc.p, &d, sizeof d);
+    return WTC_OK;
+}
+
+// Clears and builds the bitmap prepared by prepare_bitmap on the slot's stream.
+int enqueue_bitmap(Slot &s, const GraphDesc &d, int ob, uint64_t bm_words) {
+    if (!bm_words) return WTC_OK;
+    WTC_HIPC(hipMemcpyAsync(s.desc.p, s.h_desc.p, sizeof(GraphDesc), hipMemcpyHostToDevice, s.stream));
+    WTC_HIPC(launch_bitmap(s.desc.as<GraphDesc>(), 1, ob, d.bm_rows, bm_words, s.stream));
+    return WTC_OK;
+}
 
 uint32_t stride_grid(uint64_t items, uint32_t cap) {
     const uint64_t b = (items + kBlock - 1) / kBlock;
@@ -210,15 +327,9 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
     WTC_HIPC(s.partials.ensure(std::max<uint32_t>(nb, 1) * sizeof(uint64_t)));
     WTC_HIPC(s.out.ensure((size_t)kMetaSlots * sizeof(uint64_t)));
     WTC_HIPC(s.h_out.ensure((size_t)kMetaSlots * sizeof(uint64_t)));
-    GraphDesc d = make_desc(dg->rowPtr.p, dg->colInd.as<uint32_t>(), n, spread, a, 0, UINT64_MAX);
-    const uint64_t bm_words = a ? bm_words_for(a) : 0;
-    if (bm_words) {
-        WTC_HIPC(s.bm.ensure(bm_words * sizeof(uint32_t)));
-        d.bitmap = s.bm.as<uint32_t>();
-        WTC_HIPC(s.desc.ensure(sizeof(GraphDesc)));
-        WTC_HIPC(s.h_desc.ensure(sizeof(GraphDesc)));
-        std::memcpy(s.h_desc.p, &d, sizeof d);
-    }
+    GraphDesc d;
+    uint64_t bm_words = 0;
+    if ((rc = prepare_bitmap(s, dg, a, d, bm_words))) return rc;
     uint64_t *vi = s.ws.as<uint64_t>();
     uint64_t *out = s.out.as<uint64_t>();
     unsigned long long *dcount = reinterpret_cast<unsigned long long *>(out);
@@ -239,10 +350,7 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
         WTC_HIPC(launch_fill(reinterpret_cast<uint32_t *>(vi), 2, 0u, stream));  // vi[0] = A = 0
     }
     WTC_HIPC(hipEventRecord(s.ev[1], stream));
-    if (bm_words) {
-        WTC_HIPC(hipMemcpyAsync(s.desc.p, s.h_desc.p, sizeof(GraphDesc), hipMemcpyHostToDevice, stream));
-        WTC_HIPC(launch_bitmap(s.desc.as<GraphDesc>(), 1, ob, d.bm_rows, bm_words, stream));
-    }
+    if ((rc = enqueue_bitmap(s, d, ob, bm_words))) return rc;
     WTC_HIPC(hipEventRecord(s.ev[2], stream));
     // chunks of `spread` wedges, one per lane of a persistent grid
     const uint32_t grid = cap * 2u;
@@ -274,6 +382,60 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
     return WTC_OK;
 }
 
+int wtc_dgraph_count_baseline(wtc_ctx *c, wtc_dgraph *dg, int parallelism, int style, uint32_t a,
+                              uint64_t *triangles, wtc_count_stats *st) {
+    if (!c || !dg || !triangles) return WTC_ERR_ARG;
+    if (parallelism != WTC_PARALLEL_VERTEX && parallelism != WTC_PARALLEL_EDGE) return WTC_ERR_ARG;
+    if (style < WTC_WEDGE_OUTGOING || style > WTC_WEDGE_MIXED) return WTC_ERR_ARG;
+    int rc = check_args(1, a);
+    if (rc) return rc;
+    WTC_HIPC(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    Slot &s = c->slot;
+    WTC_HIPC(s.out.ensure((size_t)kMetaSlots * sizeof(uint64_t)));
+    WTC_HIPC(s.h_out.ensure((size_t)kMetaSlots * sizeof(uint64_t)));
+    GraphDesc d;
+    uint64_t bm_words = 0;
+    if ((rc = prepare_bitmap(s, dg, a, d, bm_words))) return rc;
+    unsigned long long *dcount = reinterpret_cast<unsigned long long *>(s.out.p);
+    const uint64_t items = parallelism == WTC_PARALLEL_EDGE ? dg->m : dg->n;
+    const dim3 grid(stride_grid(items, (uint32_t)std::max(c->tune.num_cus, 1) * 32u));
+    const Probe pr{d.bitmap, d.bm_size, d.thr, dg->n};
+    WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
+    WTC_HIPC(launch_fill(reinterpret_cast<uint32_t *>(s.out.p), 2, 0u, s.stream));
+    WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
+    if ((rc = enqueue_bitmap(s, d, dg->ob, bm_words))) return rc;
+    WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
+    if (dg->m) {
+        const int ep = parallelism == WTC_PARALLEL_EDGE;
+        const void *rp = dg->rowPtr.p;
+        const uint32_t *ci = dg->colInd.as<uint32_t>();
+        if (style == WTC_WEDGE_OUTGOING)
+            WTC_HIPC(launch_baseline<0>(dg->ob, ep, grid, s.stream, rp, ci, dg->n, dg->m, pr, dcount));
+        else if (style == WTC_WEDGE_ARROW)
+            WTC_HIPC(launch_baseline<1>(dg->ob, ep, grid, s.stream, rp, ci, dg->n, dg->m, pr, dcount));
+        else
+            WTC_HIPC(launch_baseline<2>(dg->ob, ep, grid, s.stream, rp, ci, dg->n, dg->m, pr, dcount));
+    }
+    WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
+    WTC_HIPC(hipMemcpyAsync(s.h_out.p, s.out.p, sizeof(uint64_t), hipMemcpyDeviceToHost, s.stream));
+    WTC_HIPC(hipStreamSynchronize(s.stream));
+    auto t1 = std::chrono::steady_clock::now();
+    *triangles = s.h_out.as<uint64_t>()[0];
+    if (st) {
+        float b = 0, k = 0;
+        WTC_HIPC(hipEventElapsedTime(&b, s.ev[1], s.ev[2]));
+        WTC_HIPC(hipEventElapsedTime(&k, s.ev[2], s.ev[3]));
+        *st = wtc_count_stats{};
+        st->bitmap_ms = b;
+        st->count_ms = k;
+        st->total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        st->blocks = dg->m ? grid.x : 0;
+        st->launches = dg->m ? 1 : 0;
+    }
+    return WTC_OK;
+}
+
 int wtc_tc_arrow_gpu_big(const wtc_big_graph *g, uint32_t spread, uint32_t a, wtc_gpu_time *t,
                          uint64_t *triangles) {
     if (!g || !triangles) return WTC_ERR_ARG;
@@ -298,22 +460,76 @@ int wtc_tc_arrow_gpu_big(const wtc_big_graph *g, uint32_t spread, uint32_t a, wt
     return WTC_OK;
 }
 
-// 32-bit offsets (tc_arrow.cu's GRAPH_TYPE): the offsets are widened on the
-// host and the upload narrows them again on the device side.
-int wtc_tc_arrow_gpu(const wtc_graph *g, uint32_t spread, uint32_t a, wtc_gpu_time *t, uint64_t *triangles) {
-    if (!g || !triangles || !g->rowPtr || g->numVertices == 0) return WTC_ERR_ARG;
-    std::vector<uint64_t> rp;
+
+// Per-call form shared by the tc_vertex_GPU / tc_edge_GPU drop-ins: upload,
+// count, free, with the copy and device times accumulated like GPU_time.
+static int baseline_call(const wtc_big_graph *g, int parallelism, uint32_t a, int style, wtc_gpu_time *t,
+                         uint64_t *triangles) {
+    if (!g || !triangles) return WTC_ERR_ARG;
+    std::lock_guard<std::mutex> lk(default_ctx_mutex());
+    wtc_ctx *c;
+    int rc = default_context(&c);
+    if (rc) return rc;
+    auto t0 = std::chrono::steady_clock::now();
+    wtc_dgraph *dg = nullptr;
+    if ((rc = wtc_dgraph_upload(c, g, 0, &dg))) return rc;
+    auto t1 = std::chrono::steady_clock::now();
+    wtc_count_stats st{};
+    rc = wtc_dgraph_count_baseline(c, dg, parallelism, style, a, triangles, &st);
+    wtc_dgraph_free(dg);
+    c->slot.release_device();
+    if (rc) return rc;
+    if (t) {
+        t->copy += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        t->exec += st.bitmap_ms + st.count_ms;
+    }
+    return WTC_OK;
+}
+
+// 32-bit offsets widened on the host (the upload narrows them again).
+static int widen(const wtc_graph *g, std::vector<uint64_t> &rp, wtc_big_graph &b) {
+    if (!g || !g->rowPtr || g->numVertices == 0) return WTC_ERR_ARG;
     try {
         rp.assign(g->rowPtr, g->rowPtr + (size_t)g->numVertices + 1);
     } catch (const std::bad_alloc &) {
         return WTC_ERR_NOMEM;
     }
-    wtc_big_graph b{};
+    b = wtc_big_graph{};
     b.numVertices = g->numVertices;
     b.numEdges = g->numEdges;
     b.rowPtr = rp.data();
     b.colInd = g->colInd;
-    return wtc_tc_arrow_gpu_big(&b, spread, a, t, triangles);
+    return WTC_OK;
+}
+
+int wtc_tc_arrow_gpu(const wtc_graph *g, uint32_t spread, uint32_t a, wtc_gpu_time *t, uint64_t *triangles) {
+    if (!triangles) return WTC_ERR_ARG;
+    std::vector<uint64_t> rp;
+    wtc_big_graph b;
+    const int rc = widen(g, rp, b);
+    return rc ? rc : wtc_tc_arrow_gpu_big(&b, spread, a, t, triangles);
+}
+
+int wtc_tc_vertex_gpu(const wtc_graph *g, uint32_t a, int style, wtc_gpu_time *t, uint64_t *triangles) {
+    std::vector<uint64_t> rp;
+    wtc_big_graph b;
+    const int rc = widen(g, rp, b);
+    return rc ? rc : baseline_call(&b, WTC_PARALLEL_VERTEX, a, style, t, triangles);
+}
+
+int wtc_tc_vertex_gpu_big(const wtc_big_graph *g, uint32_t a, int style, wtc_gpu_time *t, uint64_t *triangles) {
+    return baseline_call(g, WTC_PARALLEL_VERTEX, a, style, t, triangles);
+}
+
+int wtc_tc_edge_gpu(const wtc_graph *g, uint32_t a, int style, wtc_gpu_time *t, uint64_t *triangles) {
+    std::vector<uint64_t> rp;
+    wtc_big_graph b;
+    const int rc = widen(g, rp, b);
+    return rc ? rc : baseline_call(&b, WTC_PARALLEL_EDGE, a, style, t, triangles);
+}
+
+int wtc_tc_edge_gpu_big(const wtc_big_graph *g, uint32_t a, int style, wtc_gpu_time *t, uint64_t *triangles) {
+    return baseline_call(g, WTC_PARALLEL_EDGE, a, style, t, triangles);
 }
 
 }  // extern "C"

@@ -120,6 +120,12 @@ _sig = {
                                   C.POINTER(C.c_uint64)]),
     "wtc_tc_arrow_gpu_big": (C.c_int, [_BGP, C.c_uint32, C.c_uint32, C.POINTER(GpuTime),
                                       C.POINTER(C.c_uint64)]),
+    "wtc_tc_vertex_gpu": (C.c_int, [C.POINTER(_Graph32), C.c_uint32, C.c_int, C.POINTER(GpuTime),
+                                   C.POINTER(C.c_uint64)]),
+    "wtc_tc_vertex_gpu_big": (C.c_int, [_BGP, C.c_uint32, C.c_int, C.POINTER(GpuTime), C.POINTER(C.c_uint64)]),
+    "wtc_tc_edge_gpu": (C.c_int, [C.POINTER(_Graph32), C.c_uint32, C.c_int, C.POINTER(GpuTime),
+                                 C.POINTER(C.c_uint64)]),
+    "wtc_tc_edge_gpu_big": (C.c_int, [_BGP, C.c_uint32, C.c_int, C.POINTER(GpuTime), C.POINTER(C.c_uint64)]),
     "wtc_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "wtc_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
     "wtc_ctx_destroy": (None, [_P]),
@@ -135,6 +141,8 @@ _sig = {
                                   C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
     "wtc_dgraph_count_arrow": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64),
                                         C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
+    "wtc_dgraph_count_baseline": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_uint64),
+                                           C.POINTER(CountStats)]),
     "wtc_ctx_debug_prefix": (C.c_int, [_P, C.c_uint64, _P, _P]),
     "wtc_ctx_debug_bitmap": (C.c_int, [_P, C.c_uint64, _P]),
     "wtc_debug_decode": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _P]),
@@ -371,6 +379,38 @@ def tc_arrow_gpu_big(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -
     return int(cnt.value)
 
 
+WEDGE_OUTGOING, WEDGE_ARROW, WEDGE_MIXED = 0, 1, 2
+PARALLEL_VERTEX, PARALLEL_EDGE = 0, 1
+
+
+def _baseline_call(fn32, fn64, g: CSR, a: int, style: int, t: Optional[GpuTime], big: bool) -> int:
+    s = g._struct()
+    cnt = C.c_uint64()
+    tp = C.byref(t) if t is not None else None
+    if big:
+        _check(fn64(C.byref(s), a, style, tp, C.byref(cnt)), fn64.__name__)
+        return int(cnt.value)
+    g32 = C.POINTER(_Graph32)()
+    _check(_lib.wtc_narrow_graph(C.byref(s), C.byref(g32)), "narrow_graph")
+    try:
+        _check(fn32(g32, a, style, tp, C.byref(cnt)), fn32.__name__)
+        return int(cnt.value)
+    finally:
+        _lib.wtc_free_graph32(g32)
+
+
+def tc_vertex_gpu(g: CSR, a: int, style: int = WEDGE_OUTGOING, t: Optional[GpuTime] = None,
+                  big: bool = False) -> int:
+    """tc_vertex_GPU of experiments/tc_vertex.cu:287 (big: tc_big_graphs_vertex.cu)."""
+    return _baseline_call(_lib.wtc_tc_vertex_gpu, _lib.wtc_tc_vertex_gpu_big, g, a, style, t, big)
+
+
+def tc_edge_gpu(g: CSR, a: int, style: int = WEDGE_OUTGOING, t: Optional[GpuTime] = None,
+                big: bool = False) -> int:
+    """tc_edge_GPU of experiments/tc_edge.cu:374 (big: tc_big_graphs_edge.cu)."""
+    return _baseline_call(_lib.wtc_tc_edge_gpu, _lib.wtc_tc_edge_gpu_big, g, a, style, t, big)
+
+
 def device_count() -> int:
     c = C.c_int()
     _lib.wtc_device_count(C.byref(c))
@@ -489,6 +529,14 @@ class DeviceGraph:
         st = CountStats()
         _check(_lib.wtc_dgraph_count(self.ctx._p, self._p, spread, a, w_begin, w_end, C.byref(cnt), C.byref(st)),
                "dgraph_count")
+        return (int(cnt.value), st) if stats else int(cnt.value)
+
+    def count_baseline(self, parallelism: int, style: int, a: int, stats: bool = False):
+        """Vertex- (0) or edge-parallel (1) count in wedge style 0/1/2."""
+        cnt = C.c_uint64()
+        st = CountStats()
+        _check(_lib.wtc_dgraph_count_baseline(self.ctx._p, self._p, parallelism, style, a, C.byref(cnt),
+                                              C.byref(st)), "dgraph_count_baseline")
         return (int(cnt.value), st) if stats else int(cnt.value)
 
     def count_arrow(self, spread: int, a: int, stats: bool = False):
