@@ -50,3 +50,26 @@ def test_device_builders(gpu_ctx):
             dp = wtc.DeviceGraph.from_pairs(gpu_ctx, pairs, offset_bytes=ob)
             assert same_csr(dp.download(), wtc.preprocess(wtc.graph_from_pairs(pairs))), (scale, ob)
             dp.free()
+
+
+def test_long_runs_of_empty_rows(gpu_ctx, tmp_path):
+    """Runs of hundreds of isolated ids (empty rows) before, between and after
+    the used ids, in the symmetric CSR and, ranked first, in the oriented one:
+    the row-offset builder writes short runs from the entry after them and
+    searches the rest."""
+    k5 = [(i, j) for i in range(5) for j in range(i + 1, 5)]
+    k4 = [(2000 + i, 2000 + j) for i in range(4) for j in range(i + 1, 4)]
+    for extra in ([(1000, 1001)], [(1000, 1001), (5000, 5001)], []):
+        pairs = np.array(k5 + k4 + extra, np.uint32)
+        sym_host = wtc.graph_from_pairs(pairs)
+        path = tmp_path / "runs.txt"
+        np.savetxt(path, pairs, fmt="%d")
+        for style in (1, 2):  # the symmetric CSR built on the GPU
+            assert same_csr(wtc.read_graph(str(path), zero_indexed=True, preprocess_style=style), sym_host)
+        host = wtc.preprocess(sym_host, 0)
+        assert same_csr(wtc.preprocess(sym_host, 2), host)
+        for ob in (4, 8):
+            dp = wtc.DeviceGraph.from_pairs(gpu_ctx, pairs, offset_bytes=ob)
+            assert same_csr(dp.download(), host), (extra, ob)
+            assert dp.count(3, 0) == 10 + 4
+            dp.free()

@@ -121,8 +121,15 @@ __global__ __launch_bounds__(kBlock) void k_cmp_write(const uint64_t *__restrict
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *total = carry;
 }
 
+// Rows of a run of empty rows that the entry after the run writes itself;
+// rows further back in a longer run are left to k_row_fill.
+constexpr uint64_t kRunInline = 64;
+
 // rowPtr[r] = first entry of row r, from sorted keys; rows without entries
-// repeat the next start; rowPtr[n] = cnt.
+// repeat the next start; rowPtr[n] = cnt.  Entry i writes the rows that end
+// at its own row (the empty rows before it and its row), at most kRunInline
+// of them: a long run (the isolated vertices, which the orientation ranks
+// first) would otherwise be written by one lane.  rowPtr starts all ones.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void k_row_bounds(const uint64_t *__restrict__ keys, uint64_t cnt, int bits,
                                                        uint32_t n, OffT *__restrict__ rowPtr) {
@@ -130,7 +137,29 @@ __global__ __launch_bounds__(kBlock) void k_row_bounds(const uint64_t *__restric
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i <= cnt; i += stride) {
         const uint64_t cur = i < cnt ? (keys[i] >> bits) : n;
         const uint64_t r0 = i > 0 ? (keys[i - 1] >> bits) + 1 : 0;
-        for (uint64_t r = r0; r <= cur; r++) rowPtr[r] = (OffT)i;
+        if (r0 > cur) continue;  // entry i continues the row of entry i-1
+        const uint64_t lo = cur - r0 >= kRunInline ? cur + 1 - kRunInline : r0;
+        for (uint64_t r = lo; r <= cur; r++) rowPtr[r] = (OffT)i;
+    }
+}
+
+// The rows k_row_bounds left unset (inside long runs of empty rows, so r < n
+// and r < 2^bits): the first entry whose row is >= r, by a lower-bound search
+// of the sorted keys.
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void k_row_fill(const uint64_t *__restrict__ keys, uint64_t cnt, int bits,
+                                                     uint32_t n, OffT *__restrict__ rowPtr) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t r = (uint64_t)blockIdx.x * kBlock + threadIdx.x; r < n; r += stride) {
+        if (rowPtr[r] != (OffT)~(OffT)0) continue;
+        const uint64_t t = r << bits;
+        uint64_t lo = 0, hi = cnt;
+        while (lo < hi) {
+            const uint64_t mid = lo + ((hi - lo) >> 1);
+            if (keys[mid] < t) lo = mid + 1;
+            else hi = mid;
+        }
+        rowPtr[r] = (OffT)lo;
     }
 }
 
@@ -176,6 +205,17 @@ dim3 grid_for(uint64_t n) {
         hipError_t _e = (call);                                 \
         if (_e != hipSuccess) return hip_fail(#call, _e);       \
     } while (0)
+
+// rowPtr[0..n] of the sorted keys[0..cnt) (rows = keys >> bits).
+template <typename OffT>
+hipError_t row_bounds(hipStream_t st, const uint64_t *keys, uint64_t cnt, int bits, uint32_t n, OffT *rowPtr) {
+    hipError_t e = hipMemsetAsync(rowPtr, 0xFF, ((uint64_t)n + 1) * sizeof(OffT), st);
+    if (e != hipSuccess) return e;
+    k_row_bounds<OffT><<<grid_for(cnt + 1), dim3(kBlock), 0, st>>>(keys, cnt, bits, n, rowPtr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_row_fill<OffT><<<grid_for(n), dim3(kBlock), 0, st>>>(keys, cnt, bits, n, rowPtr);
+    return hipGetLastError();
+}
 
 // Stable LSD radix sort of 64-bit keys (bits [0, end_bit)) from `in` to `out`.
 int sort_keys(hipStream_t st, DevBuf &tmp, const uint64_t *in, uint64_t *out, uint64_t n, int end_bit) {
@@ -234,8 +274,7 @@ int orient_keys(hipStream_t st, const uint64_t *sym, uint64_t U, uint32_t n, int
     DevBuf srow, deg, ids, degs, order, rank, okeys, osorted, tmp, blk, total;
     const uint64_t mask = (1ull << bits) - 1;
     PREP_CHECK(srow.ensure(((uint64_t)n + 1) * 8));
-    k_row_bounds<uint64_t><<<grid_for(U + 1), dim3(kBlock), 0, st>>>(sym, U, bits, n, srow.as<uint64_t>());
-    PREP_CHECK(hipGetLastError());
+    PREP_CHECK(row_bounds<uint64_t>(st, sym, U, bits, n, srow.as<uint64_t>()));
     PREP_CHECK(deg.ensure((uint64_t)n * 4));
     PREP_CHECK(ids.ensure((uint64_t)n * 4));
     PREP_CHECK(degs.ensure((uint64_t)n * 4));
@@ -263,9 +302,7 @@ int orient_keys(hipStream_t st, const uint64_t *sym, uint64_t U, uint32_t n, int
     if ((rc = sort_keys(st, tmp, okeys.as<uint64_t>(), osorted.as<uint64_t>(), m, 2 * bits))) return rc;
     PREP_CHECK(rowPtr.ensure(((uint64_t)n + 1) * sizeof(OffT)));
     PREP_CHECK(colInd.ensure(std::max<uint64_t>(m, 1) * 4));
-    k_row_bounds<OffT><<<grid_for(m + 1), dim3(kBlock), 0, st>>>(osorted.as<uint64_t>(), m, bits, n,
-                                                                 rowPtr.as<OffT>());
-    PREP_CHECK(hipGetLastError());
+    PREP_CHECK(row_bounds<OffT>(st, osorted.as<uint64_t>(), m, bits, n, rowPtr.as<OffT>()));
     k_low_bits<<<grid_for(m), dim3(kBlock), 0, st>>>(osorted.as<uint64_t>(), m, mask, colInd.as<uint32_t>());
     PREP_CHECK(hipGetLastError());
     PREP_CHECK(hipStreamSynchronize(st));
@@ -375,9 +412,7 @@ int gpu_symmetric_from_pairs(wtc_ctx *c, const uint32_t *pairs, uint64_t np, uin
     hipStream_t st = c->slot.stream;
     PREP_CHECK(srow.ensure(((uint64_t)nn + 1) * 8));
     PREP_CHECK(scol.ensure(std::max<uint64_t>(U, 1) * 4));
-    k_row_bounds<uint64_t><<<grid_for(U + 1), dim3(kBlock), 0, st>>>(sym.as<uint64_t>(), U, bits, nn,
-                                                                     srow.as<uint64_t>());
-    PREP_CHECK(hipGetLastError());
+    PREP_CHECK(row_bounds<uint64_t>(st, sym.as<uint64_t>(), U, bits, nn, srow.as<uint64_t>()));
     k_low_bits<<<grid_for(U), dim3(kBlock), 0, st>>>(sym.as<uint64_t>(), U, (1ull << bits) - 1,
                                                        scol.as<uint32_t>());
     PREP_CHECK(hipGetLastError());
