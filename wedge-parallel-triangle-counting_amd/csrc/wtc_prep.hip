@@ -251,26 +251,35 @@ int compact(hipStream_t st, DevBuf &blk, DevBuf &total, const uint64_t *in, uint
 }
 
 // raw keys (2 per pair, self-loops as sentinels) -> sorted unique keys in `out`.
-int unique_sorted(hipStream_t st, DevBuf &raw, uint64_t nraw, int bits, DevBuf &out, uint64_t *ucount) {
+// The unique keys are compacted back into raw's memory, which then becomes
+// `out`; the sorted copy is handed to `spare` (when given) for the
+// orientation to reuse, instead of freeing one buffer of that size and
+// allocating another: at R-MAT 28 each such buffer is 69 GB, and mapping it
+// costs more than the kernels that fill it.
+int unique_sorted(hipStream_t st, DevBuf &raw, uint64_t nraw, int bits, DevBuf &out, uint64_t *ucount,
+                  DevBuf *spare = nullptr) {
     DevBuf tmp, blk, total, sorted;
     int rc;
+    PREP_CHECK(raw.ensure(std::max<uint64_t>(nraw, 1) * 8));
     PREP_CHECK(sorted.ensure(std::max<uint64_t>(nraw, 1) * 8));
     if ((rc = sort_keys(st, tmp, raw.as<uint64_t>(), sorted.as<uint64_t>(), nraw, 2 * bits))) return rc;
     PREP_CHECK(hipStreamSynchronize(st));
     tmp.release();
-    raw.release();
-    PREP_CHECK(out.ensure(std::max<uint64_t>(nraw, 1) * 8));
     // sentinel keys (all ones above 2*bits) sort last only if they compare
     // greater on the sorted bits: every valid key is < 2^(2*bits) <= sentinel
-    if ((rc = compact(st, blk, total, sorted.as<uint64_t>(), nraw, UniqueValid{}, out.as<uint64_t>(), ucount)))
+    if ((rc = compact(st, blk, total, sorted.as<uint64_t>(), nraw, UniqueValid{}, raw.as<uint64_t>(), ucount)))
         return rc;
+    out = std::move(raw);
+    if (spare) *spare = std::move(sorted);
     return WTC_OK;
 }
 
 // Sorted unique symmetric keys (n vertices) -> oriented CSR on the device.
+// `spare`, when it holds at least 2*(U/2) keys past a 256-byte boundary,
+// stores the oriented keys and their sorted copy instead of two new buffers.
 template <typename OffT>
 int orient_keys(hipStream_t st, const uint64_t *sym, uint64_t U, uint32_t n, int bits, DevBuf &rowPtr,
-                DevBuf &colInd) {
+                DevBuf &colInd, DevBuf *spare = nullptr) {
     DevBuf srow, deg, ids, degs, order, rank, okeys, osorted, tmp, blk, total;
     const uint64_t mask = (1ull << bits) - 1;
     PREP_CHECK(srow.ensure(((uint64_t)n + 1) * 8));
@@ -293,17 +302,27 @@ int orient_keys(hipStream_t st, const uint64_t *sym, uint64_t U, uint32_t n, int
     k_ranks<<<grid_for(n), dim3(kBlock), 0, st>>>(order.as<uint32_t>(), n, rank.as<uint32_t>());
     PREP_CHECK(hipGetLastError());
     // keep v->w iff rank(w) > rank(v), relabelled (tc.cu:738-758)
-    PREP_CHECK(okeys.ensure(std::max<uint64_t>(U / 2, 1) * 8));
+    const uint64_t half = std::max<uint64_t>(U / 2, 1);
+    const uint64_t second = (half * 8 + 255) & ~(uint64_t)255;  // byte offset of the sorted copy
+    uint64_t *ok = nullptr, *os = nullptr;
+    if (spare && spare->p && spare->cap >= second + half * 8) {
+        ok = spare->as<uint64_t>();
+        os = reinterpret_cast<uint64_t *>(spare->as<char>() + second);
+    } else {
+        PREP_CHECK(okeys.ensure(half * 8));
+        PREP_CHECK(osorted.ensure(half * 8));
+        ok = okeys.as<uint64_t>();
+        os = osorted.as<uint64_t>();
+    }
     uint64_t m = 0;
-    int rc = compact(st, blk, total, sym, U, Orient{rank.as<uint32_t>(), bits, mask}, okeys.as<uint64_t>(), &m);
+    int rc = compact(st, blk, total, sym, U, Orient{rank.as<uint32_t>(), bits, mask}, ok, &m);
     if (rc) return rc;
     if (m * 2 != U) return WTC_ERR_CSR;
-    PREP_CHECK(osorted.ensure(std::max<uint64_t>(m, 1) * 8));
-    if ((rc = sort_keys(st, tmp, okeys.as<uint64_t>(), osorted.as<uint64_t>(), m, 2 * bits))) return rc;
+    if ((rc = sort_keys(st, tmp, ok, os, m, 2 * bits))) return rc;
     PREP_CHECK(rowPtr.ensure(((uint64_t)n + 1) * sizeof(OffT)));
     PREP_CHECK(colInd.ensure(std::max<uint64_t>(m, 1) * 4));
-    PREP_CHECK(row_bounds<OffT>(st, osorted.as<uint64_t>(), m, bits, n, rowPtr.as<OffT>()));
-    k_low_bits<<<grid_for(m), dim3(kBlock), 0, st>>>(osorted.as<uint64_t>(), m, mask, colInd.as<uint32_t>());
+    PREP_CHECK(row_bounds<OffT>(st, os, m, bits, n, rowPtr.as<OffT>()));
+    k_low_bits<<<grid_for(m), dim3(kBlock), 0, st>>>(os, m, mask, colInd.as<uint32_t>());
     PREP_CHECK(hipGetLastError());
     PREP_CHECK(hipStreamSynchronize(st));
     return WTC_OK;
@@ -346,7 +365,8 @@ int download_csr(hipStream_t st, const DevBuf &rowPtr, int ob, const DevBuf &col
 }
 
 // Oriented device CSR from sorted unique symmetric keys, into a new dgraph.
-int make_dgraph(wtc_ctx *c, const uint64_t *sym, uint64_t U, uint32_t n, int bits, uint32_t ob, wtc_dgraph **out) {
+int make_dgraph(wtc_ctx *c, const uint64_t *sym, uint64_t U, uint32_t n, int bits, uint32_t ob, wtc_dgraph **out,
+                DevBuf *spare = nullptr) {
     wtc_dgraph *dg = new (std::nothrow) wtc_dgraph();
     if (!dg) return WTC_ERR_NOMEM;
     dg->ctx = c;
@@ -358,8 +378,8 @@ int make_dgraph(wtc_ctx *c, const uint64_t *sym, uint64_t U, uint32_t n, int bit
         return WTC_ERR_RANGE;
     }
     dg->ob = (int)ob;
-    const int rc = ob == 8 ? orient_keys<uint64_t>(c->slot.stream, sym, U, n, bits, dg->rowPtr, dg->colInd)
-                           : orient_keys<uint32_t>(c->slot.stream, sym, U, n, bits, dg->rowPtr, dg->colInd);
+    const int rc = ob == 8 ? orient_keys<uint64_t>(c->slot.stream, sym, U, n, bits, dg->rowPtr, dg->colInd, spare)
+                           : orient_keys<uint32_t>(c->slot.stream, sym, U, n, bits, dg->rowPtr, dg->colInd, spare);
     if (rc) {
         wtc_dgraph_free(dg);
         return rc;
@@ -383,7 +403,8 @@ int n_from_keys(hipStream_t st, const DevBuf &keys, uint64_t U, int bits, uint32
 }
 
 // Symmetric keys of host pairs (ids < n_cap) on the context's stream.
-int keys_from_pairs(wtc_ctx *c, const uint32_t *pairs, uint64_t np, int bits, DevBuf &sym, uint64_t *U) {
+int keys_from_pairs(wtc_ctx *c, const uint32_t *pairs, uint64_t np, int bits, DevBuf &sym, uint64_t *U,
+                    DevBuf *spare = nullptr) {
     hipStream_t st = c->slot.stream;
     DevBuf dpairs, raw;
     PREP_CHECK(dpairs.ensure(std::max<uint64_t>(np, 1) * 8));
@@ -393,7 +414,7 @@ int keys_from_pairs(wtc_ctx *c, const uint32_t *pairs, uint64_t np, int bits, De
     PREP_CHECK(hipGetLastError());
     PREP_CHECK(hipStreamSynchronize(st));
     dpairs.release();
-    return unique_sorted(st, raw, 2 * np, bits, sym, U);
+    return unique_sorted(st, raw, 2 * np, bits, sym, U, spare);
 }
 
 }  // namespace
@@ -471,13 +492,13 @@ int wtc_dgraph_from_pairs(wtc_ctx *c, const uint32_t *pairs, uint64_t npairs, ui
     if ((uint64_t)mx + 1 > 0xffffffffULL) return WTC_ERR_FORMAT;
     WTC_HIPC(hipSetDevice(c->device));
     const int bits = key_bits((uint64_t)mx + 1);
-    DevBuf sym;
+    DevBuf sym, spare;
     uint64_t U = 0;
-    int rc = keys_from_pairs(c, pairs, npairs, bits, sym, &U);
+    int rc = keys_from_pairs(c, pairs, npairs, bits, sym, &U, &spare);
     if (rc) return rc;
     uint32_t n = 1;
     if ((rc = n_from_keys(c->slot.stream, sym, U, bits, &n))) return rc;
-    return make_dgraph(c, sym.as<uint64_t>(), U, n, bits, offset_bytes, out);
+    return make_dgraph(c, sym.as<uint64_t>(), U, n, bits, offset_bytes, out, &spare);
 }
 
 int wtc_dgraph_gen_rmat(wtc_ctx *c, uint32_t scale, uint32_t ef, double a, double b, double cc, uint64_t seed,
@@ -499,11 +520,12 @@ int wtc_dgraph_gen_rmat(wtc_ctx *c, uint32_t scale, uint32_t ef, double a, doubl
                                                        raw.as<uint64_t>());
     PREP_CHECK(hipGetLastError());
     uint64_t U = 0;
-    int rc = unique_sorted(st, raw, 2 * np, bits, sym, &U);
+    DevBuf spare;
+    int rc = unique_sorted(st, raw, 2 * np, bits, sym, &U, &spare);
     if (rc) return rc;
     uint32_t n = 1;
     if ((rc = n_from_keys(st, sym, U, bits, &n))) return rc;
-    return make_dgraph(c, sym.as<uint64_t>(), U, n, bits, offset_bytes, out);
+    return make_dgraph(c, sym.as<uint64_t>(), U, n, bits, offset_bytes, out, &spare);
 }
 
 int wtc_dgraph_download(const wtc_dgraph *dg, wtc_big_graph **out) {
