@@ -179,13 +179,16 @@ def test_validate_rejects_bad_csr():
 
 def test_autotune_table():
     # the measured buckets: alone, R-MAT-like skew (W/m >= 24) gets a bitmap
-    # of about n/32 top ranks once W >= 2^26, capped at 32768 / 131072
+    # of about n/32 top ranks once W >= 2^26, capped at 32768 / 131072 /
+    # 262144 / 393216 as n grows (R-MAT 20-24, 26, 27-28)
     assert wtc.autotune(4083, 48254, 780919) == (2, 0)
     assert wtc.autotune(65482, 909788, 33312005) == (8, 2048)
     assert wtc.autotune(262089, 3805718, 204948349) == (16, 8192)
     assert wtc.autotune(1048465, 15701962, 1226291937) == (16, 32768)
     assert wtc.autotune(4193857, 64156969, 7204823400) == (12, 32768)
     assert wtc.autotune(16775427, 260385033, 41737375097) == (12, 131072)
+    assert wtc.autotune(67104851, 1051918711, 239067018554) == (12, 262144)
+    assert wtc.autotune(268428511, 4236159419, 1357709474934) == (12, 393216)
     # low skew (ER/BA): no bitmap; batch members use larger tiles
     assert wtc.autotune(40000, 600000, 4000000)[1] == 0
     assert wtc.autotune(100, 300, 900, batch=True) == (4, 0)
