@@ -61,3 +61,34 @@ def test_cli_csr_cache(golden, tmp_path):
     os.utime(src, ns=(stamp + 10 ** 9, stamp + 10 ** 9))
     assert run() == first
     assert cache.stat().st_mtime_ns != stamp  # key changed: rebuilt and rewritten
+
+
+STYLED_HEADER = ["graph", "n", "m", "a", "wedge style", "triangles", "prepro (s)", "GPU copy (s)",
+                 "GPU exec (s)", "GPU total (s)", "CPU+GPU (s)"]
+
+
+@pytest.mark.parametrize("binary", ["tc_arrow", "tc_vertex", "tc_edge"])
+def test_experiment_cli_tables(golden, binary):
+    """The experiments' command lines: tc_arrow takes tc's flags and table;
+    tc_vertex / tc_edge take -w <style> instead of -s and print the style."""
+    styled = binary != "tc_arrow"
+    for fx in golden[:5]:
+        flag = "-m" if fx["format"] == "mm" else "-e"
+        for style in ((0, 1, 2) if styled else (None,)):
+            args = [os.path.join(PKG, binary), flag, fx["path"], "-a", "64", "-l", "2"]
+            args += ["-w", str(style)] if styled else ["-s", "5"]
+            if fx["zero_indexed"]:
+                args.append("-z")
+            r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            lines = r.stdout.strip().splitlines()
+            assert len(lines) == 1 + 2
+            assert re.split(r"\s{2,}", lines[0].strip()) == (STYLED_HEADER if styled else HEADER)
+            for row in lines[1:]:
+                cols = row.split()
+                if styled:
+                    assert [int(x) for x in cols[1:4]] == [fx["n"], fx["m"], 64]
+                    assert cols[4] == ("outgoing", "arrow", "mixed")[style]
+                    assert int(cols[5]) == fx["triangles"], (binary, fx["file"], style)
+                else:
+                    assert [int(x) for x in cols[1:6]] == [fx["n"], fx["m"], 5, 64, fx["triangles"]]

@@ -222,6 +222,18 @@ def test_cli_usage_errors(tmp_path, binary):
     assert r.returncode == 1 and "zero vertex id detected but -z was not set" in r.stderr
 
 
+@pytest.mark.parametrize("binary", ["tc_vertex", "tc_edge"])
+def test_experiment_cli_usage_errors(tmp_path, binary):
+    """tc_vertex / tc_edge: -w <0..2> instead of -s (experiments/tc_vertex.cu:383-400)."""
+    good = _write(tmp_path, "g.txt", "1 2\n2 3\n3 1\n")
+    for args in ([], ["-e", good, "-w", "3"], ["-e", good, "-s", "5"], ["-e", good, "-a", "33"], ["-e"]):
+        r = _tc(*args, binary=binary)
+        assert r.returncode == 1, args
+        assert "Usage:" in r.stdout and "-w <num>" in r.stdout and "-s <num>" not in r.stdout
+    r = _tc("-e", "/nonexistent.txt", "-a", "0", binary=binary)
+    assert r.returncode == 1 and "unable to open graph file" in r.stderr
+
+
 # ------------------------------------------------------------- ingestion
 
 def _edge_text(rng, n_pairs, n_ids, one_indexed=True):

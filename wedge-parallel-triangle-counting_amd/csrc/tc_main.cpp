@@ -1,7 +1,11 @@
 // tc_main.cpp -- the `tc` command line (and `tc_big_graphs` when built with
 // WTC_BIG=1).  Same flags, usage exit and results table as the reference's
 // main (tc.cu:816-918; 64-bit table of experiments/tc_big_graphs.cu:874-891);
-// the counting itself goes through the library's C ABI.
+// the counting itself goes through the library's C ABI.  Built with
+// WTC_KERNEL=1/2/3 it is the experiments' `tc_arrow` (experiments/tc_arrow.cu,
+// same flags and table as tc), `tc_vertex` or `tc_edge` (experiments/
+// tc_vertex.cu:733-850, tc_edge.cu: no -s, -w <wedge style>, the style in
+// the table's fifth column).
 #include <sys/stat.h>
 #include <sys/time.h>
 
@@ -13,6 +17,31 @@
 #ifndef WTC_BIG
 #define WTC_BIG 0
 #endif
+#ifndef WTC_KERNEL
+#define WTC_KERNEL 0  // 0 wedge-parallel, 1 arrow wedges, 2 vertex-parallel, 3 edge-parallel
+#endif
+#if WTC_BIG && WTC_KERNEL
+#error "the experiment command lines are built with 32-bit offsets only"
+#endif
+constexpr bool kStyled = WTC_KERNEL >= 2;  // -w instead of -s
+
+static const char *title() {
+    switch (WTC_KERNEL) {
+        case 1: return "Arrow Wedge Parallel Triangle Counting";
+        case 2: return "Vertex-parallel Triangle Counting";
+        case 3: return "Edge-parallel Triangle Counting";
+        default: return WTC_BIG ? "Wedge Parallel Triangle Counting for big graphs" : "Wedge Parallel Triangle Counting";
+    }
+}
+
+static const char *binary() {
+    switch (WTC_KERNEL) {
+        case 1: return "tc_arrow";
+        case 2: return "tc_vertex";
+        case 3: return "tc_edge";
+        default: return WTC_BIG ? "tc_big_graphs" : "tc";
+    }
+}
 
 static double wall_seconds() {
     timeval tv;
@@ -38,22 +67,23 @@ static uint64_t source_tag(const char *file, int mm, int zero) {
 }
 
 [[noreturn]] static void usage() {
-    std::printf(WTC_BIG ? "Wedge Parallel Triangle Counting for big graphs\n\n" : "Wedge Parallel Triangle Counting\n\n");
+    std::printf("%s\n\n", title());
     std::printf("Usage:\n\n");
     std::printf("Either one of these must be selected:\n");
     std::printf(" -m <filename>        [Input graph in Matrix Market format]\n");
     std::printf(" -e <filename>        [Input graph in edge list format]\n");
     std::printf("Required arguments:\n");
-    std::printf(" -s <num>             [Spread, a.k.a. wedges/thread]\n");
+    if (!kStyled) std::printf(" -s <num>             [Spread, a.k.a. wedges/thread]\n");
     std::printf(" -a <num>             [Adjacency matrix length] (must be divisble by 32)\n");
     std::printf("Optional arguments:\n");
+    if (kStyled) std::printf(" -w <num>             [Wedge style, 0:outgoing (default), 1:arrow, 2:mixed]\n");
     std::printf(" -l <num>             [Loop count]\n");
     std::printf(" -z                   [Input graph is zero-indexed]\n");
     std::printf(" -p                   [Preprocessing style, 0:CPU, 1:GPU, 2:GPU low-memory (default)]\n");
     std::printf("\n");
     std::printf("Example:\n");
-    std::printf(WTC_BIG ? "./tc_big_graphs -m Amazon0302.mtx -s 5 -a 8192 -l 10\n"
-                        : "./tc -m Amazon0302.mtx -s 5 -a 8192 -l 10\n");
+    std::printf(kStyled ? "./%s -m Amazon0302.mtx -a 8192 -l 10\n" : "./%s -m Amazon0302.mtx -s 5 -a 8192 -l 10\n",
+                binary());
     std::exit(EXIT_FAILURE);
 }
 
@@ -68,6 +98,7 @@ int main(int argc, char **argv) {
     bool graph_mm = false, graph_zero_indexed = false;
     uint32_t loop_cnt = 1, spread = 0, adjacency_matrix_len = 0;
     int preprocess_style = WTC_PREPROCESS_GPU_CONSTRAINED;
+    int style = WTC_WEDGE_OUTGOING;
 
     while (argc > 1 && argv[1][0] == '-') {
         switch (argv[1][1]) {
@@ -92,8 +123,15 @@ int main(int argc, char **argv) {
                 argv += 2;
                 argc -= 2;
                 break;
+            case 'w':
+                if (!kStyled || argc < 3) usage();
+                if (std::atoi(argv[2]) < WTC_WEDGE_OUTGOING || std::atoi(argv[2]) > WTC_WEDGE_MIXED) usage();
+                style = std::atoi(argv[2]);
+                argv += 2;
+                argc -= 2;
+                break;
             case 's':
-                if (argc < 3) usage();
+                if (kStyled || argc < 3) usage();
                 spread = (uint32_t)std::atoi(argv[2]);
                 if (spread == 0) usage();
                 argv += 2;
@@ -117,7 +155,8 @@ int main(int argc, char **argv) {
                 usage();
         }
     }
-    if (graph_filename == nullptr || spread == 0) usage();
+    if (graph_filename == nullptr || (!kStyled && spread == 0)) usage();
+    (void)style;  // read only by the vertex-/edge-parallel command lines
 
     // WTC_CSR_CACHE=<file>: the oriented CSR is loaded from that binary file
     // when its key matches the input (the prepro column is then the load
@@ -162,8 +201,12 @@ int main(int argc, char **argv) {
 #if !WTC_BIG
     wtc_graph *graph32 = nullptr;
     if ((rc = wtc_narrow_graph(graph, &graph32))) die(rc);
-    std::printf("%-60s %16s %16s %16s %16s %16s %16s %16s %16s %16s %16s\n", "graph", "n", "m", "s", "a", "triangles",
-                "prepro (s)", "GPU copy (s)", "GPU exec (s)", "GPU total (s)", "CPU+GPU (s)");
+    if (kStyled)
+        std::printf("%-60s %16s %16s %16s %16s %16s %16s %16s %16s %16s %16s\n", "graph", "n", "m", "a", "wedge style",
+                    "triangles", "prepro (s)", "GPU copy (s)", "GPU exec (s)", "GPU total (s)", "CPU+GPU (s)");
+    else
+        std::printf("%-60s %16s %16s %16s %16s %16s %16s %16s %16s %16s %16s\n", "graph", "n", "m", "s", "a",
+                    "triangles", "prepro (s)", "GPU copy (s)", "GPU exec (s)", "GPU total (s)", "CPU+GPU (s)");
 #else
     std::printf("%-60s %16s %22s %16s %16s %22s %16s %16s %16s %16s %16s\n", "graph", "n", "m", "s", "a", "triangles",
                 "prepro (s)", "GPU copy (s)", "GPU exec (s)", "GPU total (s)", "CPU+GPU (s)");
@@ -175,7 +218,12 @@ int main(int argc, char **argv) {
         wtc_gpu_time t_gpu = {0.0, 0.0};
         uint64_t triangles = 0;
 #if !WTC_BIG
-        rc = wtc_tc_gpu(graph32, spread, adjacency_matrix_len, &t_gpu, &triangles);
+        switch (WTC_KERNEL) {
+            case 1: rc = wtc_tc_arrow_gpu(graph32, spread, adjacency_matrix_len, &t_gpu, &triangles); break;
+            case 2: rc = wtc_tc_vertex_gpu(graph32, adjacency_matrix_len, style, &t_gpu, &triangles); break;
+            case 3: rc = wtc_tc_edge_gpu(graph32, adjacency_matrix_len, style, &t_gpu, &triangles); break;
+            default: rc = wtc_tc_gpu(graph32, spread, adjacency_matrix_len, &t_gpu, &triangles);
+        }
 #else
         rc = wtc_tc_gpu_big(graph, spread, adjacency_matrix_len, &t_gpu, &triangles);
 #endif
@@ -185,10 +233,18 @@ int main(int argc, char **argv) {
         t_gpu.exec /= 1000.0;
         if (warmed_up) {
 #if !WTC_BIG
-            std::printf("%-60s %16u %16u %16u %16u %16lu %16.6f %16.6f %16.6f %16.6f %16.6f\n", graph_filename,
-                        graph32->numVertices, graph32->numEdges, spread, adjacency_matrix_len,
-                        (unsigned long)triangles, t_preprocessing, t_gpu.copy, t_gpu.exec, t_gpu.copy + t_gpu.exec,
-                        t_cpu);
+            if (kStyled) {
+                const char *style_str = style == WTC_WEDGE_ARROW ? "arrow" : style == WTC_WEDGE_MIXED ? "mixed" : "outgoing";
+                std::printf("%-60s %16u %16u %16u %16s %16lu %16.6f %16.6f %16.6f %16.6f %16.6f\n", graph_filename,
+                            graph32->numVertices, graph32->numEdges, adjacency_matrix_len, style_str,
+                            (unsigned long)triangles, t_preprocessing, t_gpu.copy, t_gpu.exec,
+                            t_gpu.copy + t_gpu.exec, t_cpu);
+            } else {
+                std::printf("%-60s %16u %16u %16u %16u %16lu %16.6f %16.6f %16.6f %16.6f %16.6f\n", graph_filename,
+                            graph32->numVertices, graph32->numEdges, spread, adjacency_matrix_len,
+                            (unsigned long)triangles, t_preprocessing, t_gpu.copy, t_gpu.exec,
+                            t_gpu.copy + t_gpu.exec, t_cpu);
+            }
 #else
             std::printf("%-60s %16u %22lu %16u %16u %22lu %16.6f %16.6f %16.6f %16.6f %16.6f\n", graph_filename,
                         graph->numVertices, (unsigned long)graph->numEdges, spread, adjacency_matrix_len,
