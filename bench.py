@@ -247,9 +247,9 @@ def arrow_bench(scale, seed, spreads=(8, 16, 32), a=131072, iters=3, baselines=F
 
 
 def reference_configs(iters=10):
-    """The reference's own measurement configurations through the tc_GPU
-    drop-in (per call: allocate, copy, count, copy back, free): config 1
-    (Chung-Lu n = 2^18, ~900k edges, s = 5, a = 8192) and the config-2 sweep
+    """The reference's own measurement cases through the tc_GPU drop-in (per
+    call: allocate, copy, count, copy back, free): the size of its usage example
+    (Chung-Lu n = 2^18, ~900k edges, s = 5, a = 8192) and its (s, a) sweep
     (R-MAT-20, s in {1,3,5,8} x a in {0,4096,8192,16384}).  Per (s, a): the
     median over `iters` calls of GPU_time's copy and exec (ms, tc.cu:76-79)
     and of the call's wall time, and the device time of the same count on a
@@ -294,8 +294,8 @@ def reference_configs(iters=10):
 
     g1, p1 = prep(wtc.gen_chung_lu())
     g2, p2 = prep(wtc.gen_rmat(20, 16, seed=1))
-    out = {"config1": dict(run(g1, [(5, 8192)]), **p1),
-           "config2": dict(run(g2, [(s_, a_) for s_ in (1, 3, 5, 8) for a_ in (0, 4096, 8192, 16384)]), **p2)}
+    out = {"chung_lu_2e18": dict(run(g1, [(5, 8192)]), **p1),
+           "rmat20_sweep": dict(run(g2, [(s_, a_) for s_ in (1, 3, 5, 8) for a_ in (0, 4096, 8192, 16384)]), **p2)}
     ctx.close()
     return out
 
@@ -520,7 +520,8 @@ def main():
                     help="only the RMAT count: no batched-vs-plain and no serving legs (profiler runs)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched vs plain comparison")
     ap.add_argument("--reference-configs", action="store_true",
-                    help="run only the reference's config 1 and config-2 sweep through the tc_GPU drop-in")
+                    help="run only the reference's Chung-Lu 2^18 case and R-MAT-20 (s, a) sweep through the "
+                         "tc_GPU drop-in")
     ap.add_argument("--arrow-only", action="store_true",
                     help="run only the arrow-wedge count against the wedge-parallel count (R-MAT --scale)")
     ap.add_argument("--baselines", action="store_true",

@@ -200,8 +200,8 @@ def test_batch_graph_replays(gpu_ctx):
         b.free()
 
 
-def test_config1_chung_lu_full_size(gpu_ctx):
-    """Config 1 at its real size: Chung-Lu, n = 262,144, ~900k unique edges,
+def test_chung_lu_2e18_full_size(gpu_ctx):
+    """The size of the usage example of tc.cu (Amazon0302) as a Chung-Lu graph: n = 262,144, ~900k edges,
     power-law expected degrees (exponent 2.7 on [1, 420]); host preprocessing,
     the oracle's count, then s=5, a=8192 on the GPU (and the tc_GPU drop-ins)."""
     g = wtc.preprocess(wtc.gen_chung_lu(262144, 2.7, 1, 420, 900000, seed=1))
@@ -215,8 +215,8 @@ def test_config1_chung_lu_full_size(gpu_ctx):
     assert wtc.tc_gpu_big(g, 5, 8192) == expect
 
 
-def test_config2_rmat20_sweep(gpu_ctx):
-    """Config 2 at its real size: R-MAT scale 20, edge factor 16, u32 ids and
+def test_rmat20_sa_sweep_full_size(gpu_ctx):
+    """The reference (s, a) sweep at full size: R-MAT scale 20, edge factor 16, u32 ids and
     offsets, every (s, a) of the sweep exact against the oracle."""
     g = wtc.preprocess(wtc.gen_rmat(20, 16, seed=1))
     expect = O.tc_merge(g)
