@@ -16,9 +16,12 @@
 //       vi = exclusive prefix of x over the m edges, in place (vi[m] = total)
 //   k_bitmap_clear / k_bitmap_build
 //       the top-a bitmap, shared with the wedge-parallel count
+//   k_arrow_vertex_prefix
+//       vp[v] = vi[rowPtr[v]], the first arrow wedge of every source
 //   k_count_arrow
 //       persistent grid; lane-contiguous chunks of `spread` wedges, located
-//       by two searches (edge over vi, source over rowPtr) and then stepped;
+//       by two searches (source over vp, edge over vi within the source's
+//       row) and then stepped;
 //       u probed in adj+(v) by the bitmap for v in the top-a ranks, else by a
 //       binary search of the row.
 //
@@ -102,6 +105,16 @@ __global__ __launch_bounds__(kBlock) void k_arrow_scan(uint64_t *__restrict__ xv
     }
 }
 
+// vp[v] = vi[rowPtr[v]] for v in [0, n]: the first arrow wedge of each
+// source vertex (the reference's wedge_vp, tc_arrow.cu:362-366).
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_arrow_vertex_prefix(const void *rp, const uint64_t *__restrict__ vi,
+                                                                uint32_t n, uint64_t *__restrict__ vp) {
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(rp);
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x; v <= n; v += stride) vp[v] = vi[rowPtr[v]];
+}
+
 // Rightmost index i in [0, hi] with a[i] <= t, given a[0] <= t.
 template <class T>
 __device__ __forceinline__ uint64_t rightmost_le64(const T *__restrict__ a, uint64_t hi, uint64_t t) {
@@ -134,6 +147,7 @@ template <class OffT>
 __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const uint32_t *__restrict__ colInd,
                                                         uint32_t n, uint64_t m,
                                                         const uint64_t *__restrict__ vi,
+                                                        const uint64_t *__restrict__ vp,
                                                         const uint32_t *__restrict__ bitmap, uint64_t bm_size,
                                                         uint32_t thr, uint32_t spread,
                                                         unsigned long long *__restrict__ count) {
@@ -145,11 +159,12 @@ __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const ui
     for (uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x; c < nchunks; c += stride) {
         uint64_t i = c * spread;
         const uint64_t iend = i + spread < A ? i + spread : A;
-        // i < A = vi[m], so the edge found has vi[e] <= i < vi[e+1] and e < m;
-        // rowPtr[n] = m > e, so its source v < n has rowPtr[v] <= e < rowPtr[v+1].
-        uint64_t e = rightmost_le64(vi, m, i);
-        uint32_t v = (uint32_t)rightmost_le64(rowPtr, n, e);
+        // the source first (vp[0] = 0 <= i < vp[n] = A, so vp[v] <= i < vp[v+1]
+        // and row v holds at least two edges), then its edge e in [vb, ve)
+        // with vi[e] <= i < vi[e+1] (vi[vb] = vp[v] <= i < vi[ve] = vp[v+1])
+        uint32_t v = (uint32_t)rightmost_le64(vp, n, i);
         uint64_t vb = rowPtr[v], ve = rowPtr[v + 1];
+        uint64_t e = vb + rightmost_le64(vi + vb, ve - 1 - vb, i);
         uint64_t eb = vi[e], ee = vi[e + 1];
         uint64_t wb = rowPtr[colInd[e]];
         // the u of one edge ascend, so each search starts at the previous
@@ -323,7 +338,7 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
     const uint32_t n = dg->n;
     const uint64_t m = dg->m;
     const uint32_t nb = (uint32_t)((m + kScanChunk - 1) / kScanChunk);
-    WTC_HIPC(s.ws.ensure((m + 1) * sizeof(uint64_t)));
+    WTC_HIPC(s.ws.ensure((m + 1 + (uint64_t)n + 1) * sizeof(uint64_t)));  // vi, then vp
     WTC_HIPC(s.partials.ensure(std::max<uint32_t>(nb, 1) * sizeof(uint64_t)));
     WTC_HIPC(s.out.ensure((size_t)kMetaSlots * sizeof(uint64_t)));
     WTC_HIPC(s.h_out.ensure((size_t)kMetaSlots * sizeof(uint64_t)));
@@ -331,6 +346,7 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
     uint64_t bm_words = 0;
     if ((rc = prepare_bitmap(s, dg, a, d, bm_words))) return rc;
     uint64_t *vi = s.ws.as<uint64_t>();
+    uint64_t *vp = vi + m + 1;
     uint64_t *out = s.out.as<uint64_t>();
     unsigned long long *dcount = reinterpret_cast<unsigned long long *>(out);
     const uint32_t cap = (uint32_t)std::max(c->tune.num_cus, 1) * 16u;
@@ -346,6 +362,8 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
         WTC_HIPC(launch_scan_partials(s.partials.as<uint64_t>(), nb, stream));
         k_arrow_scan<<<dim3(nb), dim3(kBlock), 0, stream>>>(vi, m, s.partials.as<uint64_t>());
         WTC_HIPC(hipGetLastError());
+        WTC_HIPC(WTC_ARROW_LAUNCH(ob, k_arrow_vertex_prefix, dim3(stride_grid((uint64_t)n + 1, cap)), stream,
+                                  dg->rowPtr.p, vi, n, vp));
     } else {
         WTC_HIPC(launch_fill(reinterpret_cast<uint32_t *>(vi), 2, 0u, stream));  // vi[0] = A = 0
     }
@@ -356,7 +374,7 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
     const uint32_t grid = cap * 2u;
     if (m)
         WTC_HIPC(WTC_ARROW_LAUNCH(ob, k_count_arrow, dim3(grid), stream, dg->rowPtr.p, dg->colInd.as<uint32_t>(), n,
-                                  m, vi, d.bitmap, d.bm_size, d.thr, spread, dcount));
+                                  m, vi, vp, d.bitmap, d.bm_size, d.thr, spread, dcount));
     WTC_HIPC(hipEventRecord(s.ev[3], stream));
     WTC_HIPC(hipMemcpyAsync(s.h_out.p, out, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
     WTC_HIPC(hipMemcpyAsync(s.h_out.as<uint64_t>() + 1, vi + m, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
