@@ -177,6 +177,37 @@ def test_validate_rejects_bad_csr():
         assert not b.validate_oriented()
 
 
+def test_tc_gpu_drop_ins_reject_bad_csr_on_host():
+    """Both tc_GPU drop-ins check the whole CSR on the host before any device
+    call: a row out of order, an id not above its row, an id out of range and
+    a decreasing row offset all return WTC_ERR_CSR (-10), with or without a
+    GPU, and nothing reaches a kernel."""
+    good = wtc.preprocess(wtc.gen_barabasi_albert(500, 3, seed=903))
+    assert good.validate_oriented()
+    d = np.diff(good.row_ptr.astype(np.int64))
+    r = int(np.argmax(d >= 2))
+    r0 = int(good.row_ptr[r])
+    cases = []
+    ci = good.col_ind.copy()
+    ci[r0], ci[r0 + 1] = ci[r0 + 1], ci[r0]  # a row out of order
+    cases.append(wtc.CSR(good.n, good.m, good.row_ptr.copy(), ci))
+    ci = good.col_ind.copy()
+    ci[r0] = r  # an id not above its row
+    cases.append(wtc.CSR(good.n, good.m, good.row_ptr.copy(), ci))
+    ci = good.col_ind.copy()
+    ci[-1] = good.n  # an id out of range
+    cases.append(wtc.CSR(good.n, good.m, good.row_ptr.copy(), ci))
+    rp = good.row_ptr.copy()
+    rp[r] = rp[r + 1] + 1  # a decreasing offset: rowPtr[r] > rowPtr[r+1]
+    cases.append(wtc.CSR(good.n, good.m, rp, good.col_ind.copy()))
+    for bad in cases:
+        assert not bad.validate_oriented()
+        for fn in (wtc.tc_gpu, wtc.tc_gpu_big):
+            with pytest.raises(wtc.WtcError) as ei:
+                fn(bad, 5, 64)
+            assert ei.value.code == -10, fn.__name__
+
+
 def test_autotune_table():
     # the measured buckets: alone, R-MAT-like skew (W/m >= 24) gets a bitmap
     # of about n/32 top ranks once W >= 2^26, capped at 32768 / 131072 /

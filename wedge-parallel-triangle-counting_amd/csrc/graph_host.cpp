@@ -29,18 +29,7 @@ int host_threads() {
 
 int validate_oriented_rows(uint64_t n, uint64_t m, const uint64_t *rowPtr, const uint32_t *colInd, uint64_t v0,
                            uint64_t v1) {
-    if (rowPtr[0] != 0 || rowPtr[n] != m) return WTC_ERR_CSR;
-    for (uint64_t v = v0; v < v1; v++) {
-        const uint64_t s = rowPtr[v], t = rowPtr[v + 1];
-        if (t < s || t > m) return WTC_ERR_CSR;
-        uint64_t prev = v;  // every id must exceed its row and the previous id
-        for (uint64_t j = s; j < t; j++) {
-            const uint64_t x = colInd[j];
-            if (x <= prev || x >= n) return WTC_ERR_CSR;
-            prev = x;
-        }
-    }
-    return WTC_OK;
+    return validate_oriented_rows_t(n, m, rowPtr, colInd, v0, v1);
 }
 
 // Runs f(t) on T threads (t = 0..T-1) and joins.
@@ -325,15 +314,7 @@ int wtc_narrow_graph(const wtc_big_graph *g, wtc_graph **out) {
 
 int wtc_validate_oriented(const wtc_big_graph *g) {
     if (!g || !g->rowPtr || (g->numEdges && !g->colInd)) return WTC_ERR_ARG;
-    const uint64_t n = g->numVertices;
-    if (g->rowPtr[0] != 0 || g->rowPtr[n] != g->numEdges) return WTC_ERR_CSR;
-    std::atomic<int> bad{0};
-    par_chunks(n, 1 << 14, [&](uint64_t b, uint64_t e, int) {
-        if (!bad.load(std::memory_order_relaxed) &&
-            validate_oriented_rows(n, g->numEdges, g->rowPtr, g->colInd, b, e) != WTC_OK)
-            bad = 1;
-    });
-    return bad ? WTC_ERR_CSR : WTC_OK;
+    return validate_oriented_par(g->numVertices, g->numEdges, g->rowPtr, g->colInd);
 }
 
 uint64_t wtc_wedge_total(const wtc_big_graph *g) {

@@ -66,11 +66,41 @@ T *alloc_n(uint64_t n) {
     return static_cast<T *>(std::malloc((n ? n : 1) * sizeof(T)));
 }
 
-// Checks rows [v0, v1) of an oriented CSR (offsets in range and monotone,
-// ids strictly ascending, above their row and below n).  Single-threaded, so
-// it can run inside par_chunks.  Returns WTC_OK or WTC_ERR_CSR.
+// Checks rows [v0, v1) of an oriented CSR with row offsets of type OffT
+// (offsets in range and monotone, ids strictly ascending, above their row and
+// below n).  Single-threaded, so it can run inside par_chunks.  Returns
+// WTC_OK or WTC_ERR_CSR.
+template <class OffT>
+int validate_oriented_rows_t(uint64_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colInd, uint64_t v0,
+                             uint64_t v1) {
+    if (rowPtr[0] != 0 || (uint64_t)rowPtr[n] != m) return WTC_ERR_CSR;
+    for (uint64_t v = v0; v < v1; v++) {
+        const uint64_t s = rowPtr[v], t = rowPtr[v + 1];
+        if (t < s || t > m) return WTC_ERR_CSR;
+        uint64_t prev = v;  // every id must exceed its row and the previous id
+        for (uint64_t j = s; j < t; j++) {
+            const uint64_t x = colInd[j];
+            if (x <= prev || x >= n) return WTC_ERR_CSR;
+            prev = x;
+        }
+    }
+    return WTC_OK;
+}
 int validate_oriented_rows(uint64_t n, uint64_t m, const uint64_t *rowPtr, const uint32_t *colInd, uint64_t v0,
                            uint64_t v1);
+
+// The same check over all rows, on the host threads.
+template <class OffT>
+int validate_oriented_par(uint64_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colInd) {
+    if (!rowPtr || (m && !colInd)) return WTC_ERR_ARG;
+    if (rowPtr[0] != 0 || (uint64_t)rowPtr[n] != m) return WTC_ERR_CSR;
+    std::atomic<int> bad{0};
+    par_chunks(n, 1 << 14, [&](uint64_t b, uint64_t e, int) {
+        if (!bad.load(std::memory_order_relaxed) && validate_oriented_rows_t(n, m, rowPtr, colInd, b, e) != WTC_OK)
+            bad = 1;
+    });
+    return bad ? WTC_ERR_CSR : WTC_OK;
+}
 
 }  // namespace wtc
 

@@ -401,7 +401,10 @@ int tc_gpu_impl(uint32_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colI
     int rc = check_args(spread, a);
     if (rc) return rc;
     if (!rowPtr || (m && !colInd) || !triangles || n == 0) return WTC_ERR_ARG;
-    if ((uint64_t)rowPtr[n] != m || rowPtr[0] != 0) return WTC_ERR_CSR;
+    // the whole CSR is checked on the host before anything reaches the
+    // device: an id out of range or not above its row, an unsorted row or a
+    // decreasing offset would otherwise become out-of-bounds device accesses
+    if ((rc = validate_oriented_par<OffT>(n, m, rowPtr, colInd))) return rc;
     std::lock_guard<std::mutex> lk(default_ctx_mutex());
     wtc_ctx *c;
     if ((rc = default_context(&c))) return rc;
