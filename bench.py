@@ -300,7 +300,7 @@ def reference_configs(iters=10):
     return out
 
 
-def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, window):
+def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, waves):
     """Serving workload: Poisson arrivals of count requests on seeded R-MAT
     graphs (scale uniform in [10, 22], edge factor 16), each with the
     autotuner's spread and bitmap length.  Graphs are registered once and stay
@@ -329,7 +329,7 @@ def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, win
     plain_gps = 1e3 / float(np.mean(plain_ms))
     log(f"serve: plain path {plain_gps:.1f} graphs/s on the uniform mix")
     srv = wtc.Server(0, hbm_budget_bytes=32 << 30, max_batch=256, slots=2, queue_capacity=queue_capacity,
-                     probe_ways=ways, lds_chunk=chunk, probe_window=window if window else -1)
+                     probe_ways=ways, lds_chunk=chunk, waves_per_block=waves)
     gids = [srv.register(g) for g in graphs]
     for gid, (s_g, a_g) in zip(gids, tune):  # each graph alone once: resident, server warmed up
         srv.wait(srv.submit(gid, s_g, a_g))
@@ -502,7 +502,7 @@ def main():
     ap.add_argument("--a", type=int, default=-1, help="-1: the autotuner's choice")
     ap.add_argument("--ways", type=int, default=4, help="probes advanced together per lane")
     ap.add_argument("--chunk", type=int, default=12, help="wedges per lane staged in LDS per phase")
-    ap.add_argument("--window", type=int, default=0, help="probe window in elements (0, 8 or 16)")
+    ap.add_argument("--waves", type=int, default=4, help="waves per counting block (1, 2, 4 or 8)")
     ap.add_argument("--builder", choices=("auto", "host", "device"), default="auto",
                     help="build the oriented CSR on the host or on the GPU (auto: GPU from scale 26)")
     ap.add_argument("--offset-bytes", type=int, default=0)
@@ -557,7 +557,7 @@ def main():
 
     if args.serve_only:
         res = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 2, 512,
-                          args.ways, args.chunk, args.window)
+                          args.ways, args.chunk, args.waves)
         print(json.dumps(res), flush=True)
         return
     if args.reference_configs:
@@ -568,14 +568,14 @@ def main():
         return
     if args.small_batches_only:
         ctx = wtc.Context(0)
-        ctx.set_tuning(args.ways, args.chunk, args.window)
+        ctx.set_tuning(args.ways, args.chunk, args.waves)
         res = small_kernel_bench(ctx, args.batch_graphs or 4096, 7000)
         ctx.close()
         print(json.dumps(res), flush=True)
         return
     if args.batch_only:
         ctx = wtc.Context(0)
-        ctx.set_tuning(args.ways, args.chunk, args.window)
+        ctx.set_tuning(args.ways, args.chunk, args.waves)
         res = batch_bench(ctx, args.batch_graphs or 4096, 1000)
         res["small_graph_batches"] = small_kernel_bench(ctx, args.batch_graphs or 4096, 7000)
         ctx.close()
@@ -584,7 +584,7 @@ def main():
 
     workload = f"rmat{args.scale}"
     ctx = wtc.Context(0)
-    ctx.set_tuning(args.ways, args.chunk, args.window)
+    ctx.set_tuning(args.ways, args.chunk, args.waves)
     device_build = args.builder == "device" or (args.builder == "auto" and args.scale >= 26)
     t0 = time.perf_counter()
     if device_build:  # generate, sort, deduplicate and orient on the GPU; keep it resident
@@ -609,7 +609,7 @@ def main():
     auto_s, auto_a = wtc.autotune(g.n, g.m, W)
     args.spread = args.spread or auto_s
     args.a = auto_a if args.a < 0 else args.a
-    log(f"spread={args.spread} a={args.a} ways={args.ways} chunk={args.chunk} window={args.window}")
+    log(f"spread={args.spread} a={args.a} ways={args.ways} chunk={args.chunk} waves={args.waves}")
     for _ in range(args.warmup):
         dg.count(args.spread, args.a)
     counts, stats, wall = [], [], []
@@ -660,7 +660,7 @@ def main():
                      "traffic_frac": traffic["bytes_per_launch"] / (count_ms / 1e3) / HBM_PEAK if traffic else None},
         "prepro_s": t2 - t1,
         "builder": "device" if device_build else "host",
-        "tuning": {"probe_ways": args.ways, "lds_chunk": args.chunk, "probe_window": args.window},
+        "tuning": {"probe_ways": args.ways, "lds_chunk": args.chunk, "waves_per_block": args.waves},
     }
     # size-independent check at full size: four wedge ranges sum to the whole count
     if not args.no_range_check:
@@ -682,7 +682,7 @@ def main():
     ctx.close()
     if args.serve_rates and not args.no_extras:
         out["serving"] = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 2, 512,
-                                     args.ways, args.chunk, args.window)
+                                     args.ways, args.chunk, args.waves)
         log(f"served/plain = {out['serving']['served_over_plain']:.2f}x")
     print(json.dumps(out), flush=True)
 

@@ -211,10 +211,29 @@ void wtc_ctx_destroy(wtc_ctx *ctx);
 /* Tuning of the counting kernel: probe_ways = independent binary searches
  * each lane advances together (1, 2, 4 or 8; default 4 or $WTC_PROBE_WAYS);
  * lds_chunk = wedges per lane staged in LDS per phase (1..16, default 16);
- * probe_window = 0 for a plain binary search over adj+(w), or 8 / 16 to end
- * the search with one aligned window of that many elements tested by vector
- * loads (default 0 or $WTC_PROBE_WINDOW).  Counts do not depend on these. */
-int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int probe_window);
+ * waves_per_block = the counting block's shape, 1, 2, 4 or 8 waves of 64
+ * lanes (default 4 or $WTC_COUNT_WAVES).  Counts do not depend on these. */
+int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int waves_per_block);
+/* Counting kernel of a context (both count the same tiles of the same wedge
+ * order, tc.cu:173-293, with the same probes):
+ *   WTC_KERNEL_WALK   each wave walks its tiles' wedges in steps of 64
+ *                     consecutive wedges (long rows: u loaded straight from
+ *                     the row; each vertex's last 64 entries: pairs from a
+ *                     register window); probe_ways = steps in flight.
+ *   WTC_KERNEL_STAGED each lane decodes its spread of s wedges and stages
+ *                     them through LDS, as the reference does
+ *                     (tc.cu:174-244); lds_chunk applies.
+ * Default WTC_KERNEL_WALK or $WTC_COUNT_KERNEL. */
+#define WTC_KERNEL_WALK 0
+#define WTC_KERNEL_STAGED 1
+int wtc_ctx_set_count_kernel(wtc_ctx *ctx, int kernel);
+/* How the counting kernels hand out work: the wedge order is cut into tiles
+ * of 64*s wedges, and a wave takes up to tiles_per_grab consecutive tiles
+ * per atomic grab (1..65535; 0 = 32).  deal_to_xcds != 0 deals super-chunks
+ * of grabs to the 8 XCDs in turn, each XCD taking from its own counter;
+ * 0 gives one stream of grabs for the whole grid.  Counts do not depend on
+ * these. */
+int wtc_ctx_set_work_split(wtc_ctx *ctx, uint32_t tiles_per_grab, int deal_to_xcds);
 /* Kernel choice for sets of small graphs (the role of the vertex- and
  * edge-parallel kernels, experiments/tc_vertex.cu:113-274 and
  * experiments/tc_edge.cu:179-352, for small graphs in a batch).  A count over
@@ -328,7 +347,7 @@ typedef struct {
     uint32_t queue_capacity;   /* queued requests before WTC_ERR_BUSY (65536)     */
     int32_t probe_ways;        /* counting-kernel tuning, 0 = default             */
     uint32_t lds_chunk;        /* counting-kernel tuning, 0 = default             */
-    int32_t probe_window;      /* 0 = default, -1 = none, else 8 or 16            */
+    int32_t waves_per_block;   /* counting block: 1, 2, 4 or 8 waves, 0 = 4       */
     int32_t small_graph_words; /* small-graph kernel limit (wtc_ctx_set_small_graphs):
                                   0 = default, -1 = off, else the limit in words */
 } wtc_server_config;

@@ -30,7 +30,19 @@ def test_golden_fixtures(gpu_ctx, golden):
         dg.free()
 
 
+def _square_bits(words, a):
+    """Device bitmap words (a x a bits, row-major, 32-bit words) as a
+    boolean a x a matrix."""
+    bits = np.unpackbits(np.asarray(words, np.uint32).view(np.uint8), bitorder="little")
+    return bits.reshape(a, a).astype(bool)
+
+
 def test_device_prefix_tree_bitmap_match_oracle(gpu_ctx):
+    """Device wedge prefix and search tree equal the oracle's word for word.
+    The device keeps the top-a bitmap (tc.cu:329-346) as a square a x a bit
+    matrix over the ranks [thr, thr + a): its bits must be exactly the
+    upper-triangular adjacency of those ranks, and agree bit for bit with
+    the oracle's packed bitmap through its index (tc.cu:263)."""
     for scale, seed in ((8, 1), (11, 2), (13, 3)):
         g = _rmat(scale, seed=seed)
         dg = gpu_ctx.upload(g)
@@ -40,8 +52,22 @@ def test_device_prefix_tree_bitmap_match_oracle(gpu_ctx):
             ows = O.wedge_prefix(g)
             assert np.array_equal(ws, ows)
             assert np.array_equal(tree, O.search_tree(ows, g.n))
-            bm = gpu_ctx.debug_bitmap(O.bitmap_words(a))
-            assert np.array_equal(bm, O.bitmap_build(g, a)), (scale, a)
+            sq = _square_bits(gpu_ctx.debug_bitmap(a * a // 32), a)
+            thr = max(g.n, a) - a
+            expect = np.zeros((a, a), bool)
+            rp = g.row_ptr.astype(np.int64)
+            for w in range(thr, g.n):
+                expect[w - thr, g.col_ind[rp[w]:rp[w + 1]].astype(np.int64) - thr] = True
+            assert np.array_equal(sq, expect), (scale, a)
+            packed = np.unpackbits(O.bitmap_build(g, a).view(np.uint8), bitorder="little")
+            rng = np.random.default_rng(scale)
+            rows = min(g.n, a)
+            for _ in range(400):
+                w = thr + int(rng.integers(0, max(rows - 1, 1)))
+                u = w + 1 + int(rng.integers(0, max(g.n - w - 1, 1)))
+                if u >= g.n:
+                    continue
+                assert bool(packed[O.bitmap_index(g.n, a, w, u)]) == sq[w - thr, u - thr], (scale, a, w, u)
         dg.free()
 
 
@@ -134,12 +160,16 @@ def test_batch_exact(gpu_ctx):
     assert got.tolist() == expect
 
 
-@pytest.mark.parametrize("win", [0, 8, 16])
+@pytest.mark.parametrize("wpb", [1, 2, 4, 8])
 @pytest.mark.parametrize("ways", [1, 2, 4, 8])
-def test_probe_variants(gpu_ctx, ways, win):
-    """Every (probe_ways, probe_window) kernel variant gives the same counts,
-    on single graphs (u32 and u64 offsets) and on batches, whose per-graph
-    colInd starts are not 16-byte aligned (the window's skew path)."""
+def test_probe_variants(gpu_ctx, ways, wpb):
+    """Both counting kernels (the row walk and the LDS-staged kernel), every
+    variant (probe_ways: steps or probes in flight) at every block shape
+    (waves per block) and, for the staged kernel, every LDS chunk give the
+    same counts, on single graphs (u32 and u64 offsets) and on batches.
+    Chunks 8 and 16 (and the 4-wedge tail of s = 12 at chunk 8) divide 64,
+    which takes the staged kernel's linear LDS read path; 3, 7 and 12 take
+    the general one."""
     g = _rmat(13, seed=21)
     expect = O.tc_merge(g)
     k300 = wtc.preprocess(wtc.graph_from_pairs(
@@ -147,12 +177,13 @@ def test_probe_variants(gpu_ctx, ways, win):
     graphs = [wtc.preprocess(wtc.gen_barabasi_albert(70 + 13 * i, 1 + i % 6, seed=40 + i)) for i in range(17)]
     bexp = [O.tc_merge(x) for x in graphs]
     try:
-        for chunk in (3, 16):
-            gpu_ctx.set_tuning(ways, chunk, win)
+        for kernel, chunk in [(wtc.KERNEL_WALK, 16)] + [(wtc.KERNEL_STAGED, c) for c in (3, 8, 12, 16)]:
+            gpu_ctx.set_count_kernel(kernel)
+            gpu_ctx.set_tuning(ways, chunk, wpb)
             for ob in (4, 8):
                 dg = gpu_ctx.upload(g, offset_bytes=ob)
-                for s, a in ((1, 0), (4, 0), (7, 64), (16, 4096)):
-                    assert dg.count(s, a) == expect, (ways, win, chunk, ob, s, a)
+                for s, a in ((1, 0), (4, 0), (7, 64), (12, 4096), (16, 4096)):
+                    assert dg.count(s, a) == expect, (ways, wpb, chunk, ob, s, a)
                 dg.free()
             dk = gpu_ctx.upload(k300)
             assert dk.count(5, 0) == 300 * 299 * 298 // 6
@@ -163,6 +194,7 @@ def test_probe_variants(gpu_ctx, ways, win):
             b.free()
     finally:
         gpu_ctx.set_tuning()
+        gpu_ctx.set_count_kernel(wtc.KERNEL_WALK)
 
 
 def test_batch_graph_replays(gpu_ctx):
@@ -192,7 +224,7 @@ def test_batch_graph_replays(gpu_ctx):
         for spread, a in ((2, 32), ([1 + i % 7 for i in range(24)], [32 * (i % 3) for i in range(24)])):
             for rep in range(3):
                 assert b.count(spread, a).tolist() == expect, (spread, a, rep)
-        gpu_ctx.set_tuning(2, 8, 8)
+        gpu_ctx.set_tuning(2, 8, 2)
         for rep in range(3):
             assert b.count(4, 64).tolist() == expect, rep
     finally:

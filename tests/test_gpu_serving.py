@@ -63,7 +63,7 @@ def test_alone_equals_mix(small_graph_words):
 
 def _footprint(g, a):
     up = lambda x: (x + 255) // 256 * 256  # noqa: E731  (the pool's 256-byte granule)
-    bitmap_words = (a * (a - 1) // 2 + 31) // 32
+    bitmap_words = a * (a // 32)  # the square a x a bitmap
     return up((g.n + 1) * 4 + g.m * 4) + up((g.n + 1) * 8) + (up(bitmap_words * 4) if a else 0)
 
 

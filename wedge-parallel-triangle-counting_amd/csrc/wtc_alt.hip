@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const ui
                                                         uint32_t n, uint64_t m,
                                                         const uint64_t *__restrict__ vi,
                                                         const uint64_t *__restrict__ vp,
-                                                        const uint32_t *__restrict__ bitmap, uint64_t bm_size,
+                                                        const uint32_t *__restrict__ bitmap, uint32_t a32,
                                                         uint32_t thr, uint32_t spread,
                                                         unsigned long long *__restrict__ count) {
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(rp);
@@ -187,8 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const ui
             }
             const uint32_t u = colInd[wb + (i - eb)];
             if (v >= thr) {
-                const uint64_t idx = bm_size - binom2_u32(n - v) + (u - v - 1);
-                total += (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
+                total += bm_test(bitmap, v, u, thr, a32);
             } else {
                 lo = lower_bound(colInd, lo, ve, u);
                 total += (lo < ve && colInd[lo] == u) ? 1u : 0u;
@@ -203,14 +202,11 @@ __global__ __launch_bounds__(kBlock) void k_count_arrow(const void *rp, const ui
 // ranks, else a binary search of list[lo, hi).
 struct Probe {
     const uint32_t *bitmap;
-    uint64_t bm_size;
+    uint32_t a32;
     uint32_t thr, n;
     __device__ __forceinline__ uint32_t has(const uint32_t *__restrict__ colInd, uint32_t x, uint64_t lo,
                                             uint64_t hi, uint32_t u) const {
-        if (x >= thr) {
-            const uint64_t idx = bm_size - binom2_u32(n - x) + (u - x - 1);
-            return (bitmap[idx >> 5] >> (idx & 31u)) & 1u;
-        }
+        if (x >= thr) return bm_test(bitmap, x, u, thr, a32);
         const uint64_t p = lower_bound(colInd, lo, hi, u);
         return (p < hi && colInd[p] == u) ? 1u : 0u;
     }
@@ -374,7 +370,7 @@ int wtc_dgraph_count_arrow(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t
     const uint32_t grid = cap * 2u;
     if (m)
         WTC_HIPC(WTC_ARROW_LAUNCH(ob, k_count_arrow, dim3(grid), stream, dg->rowPtr.p, dg->colInd.as<uint32_t>(), n,
-                                  m, vi, vp, d.bitmap, d.bm_size, d.thr, spread, dcount));
+                                  m, vi, vp, d.bitmap, d.a >> 5, d.thr, spread, dcount));
     WTC_HIPC(hipEventRecord(s.ev[3], stream));
     WTC_HIPC(hipMemcpyAsync(s.h_out.p, out, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
     WTC_HIPC(hipMemcpyAsync(s.h_out.as<uint64_t>() + 1, vi + m, sizeof(uint64_t), hipMemcpyDeviceToHost, stream));
@@ -418,7 +414,7 @@ int wtc_dgraph_count_baseline(wtc_ctx *c, wtc_dgraph *dg, int parallelism, int s
     unsigned long long *dcount = reinterpret_cast<unsigned long long *>(s.out.p);
     const uint64_t items = parallelism == WTC_PARALLEL_EDGE ? dg->m : dg->n;
     const dim3 grid(stride_grid(items, (uint32_t)std::max(c->tune.num_cus, 1) * 32u));
-    const Probe pr{d.bitmap, d.bm_size, d.thr, dg->n};
+    const Probe pr{d.bitmap, d.a >> 5, d.thr, dg->n};
     WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
     WTC_HIPC(launch_fill(reinterpret_cast<uint32_t *>(s.out.p), 2, 0u, s.stream));
     WTC_HIPC(hipEventRecord(s.ev[1], s.stream));

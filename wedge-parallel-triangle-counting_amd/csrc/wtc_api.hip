@@ -30,7 +30,8 @@ struct wtc_batch {
     hipGraphExec_t exec = nullptr;
     wtc::SetPlan plan;
     std::vector<uint32_t> key_s, key_a;
-    int key_ways = 0, key_win = 0;
+    int key_ways = 0, key_wpb = 0, key_kernel = 0;
+    uint32_t key_work = 0;
     uint32_t key_chunk = 0, key_small = 0;
 };
 
@@ -108,7 +109,7 @@ void Slot::destroy() {
     for (auto &e : ev) e = nullptr;
 }
 
-uint64_t bm_words_for(uint32_t a) { return (binom2(a) + 31u) / 32u; }
+uint64_t bm_words_for(uint32_t a) { return (uint64_t)a * (a / 32u); }
 
 int validate_oriented_serial(const wtc_big_graph *g) {
     if (!g || !g->rowPtr || (g->numEdges && !g->colInd)) return WTC_ERR_ARG;
@@ -131,7 +132,7 @@ GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint
     d.a = a;
     d.bm_rows = a ? std::min(n, a) : 0u;
     d.thr = a ? std::max(n, a) - a : n;
-    d.bm_size = binom2(a);
+    d.bm_words = bm_words_for(a);
     d.w_begin = w_begin;
     d.w_end = w_end;
     return d;
@@ -144,7 +145,7 @@ static uint32_t small_parts_of(uint64_t W) {
 }
 
 static int ways_index(int w) { return w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3; }
-static int win_index(int v) { return v == 8 ? 1 : v == 16 ? 2 : 0; }
+static int wpb_index(int v) { return v == 1 ? 0 : v == 2 ? 1 : v == 4 ? 2 : 3; }
 
 int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan, const uint64_t *m,
               const uint64_t *W) {
@@ -170,7 +171,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
         d.a = 0;
         d.bm_rows = 0;
         d.thr = d.n;
-        d.bm_size = 0;
+        d.bm_words = 0;
         d.bitmap = nullptr;
         small_n++;
         small_parts += small_parts_of(W[g]);
@@ -226,8 +227,8 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
 
     const int oi = ob == 8 ? 1 : 0;
     const uint32_t staged = std::min<uint32_t>(max_spread, t.chunk);
-    uint32_t &gc = t.grid_cache[oi][ways_index(t.ways)][win_index(t.win)][staged];
-    if (!gc) WTC_HIPC(count_grid(ob, staged, t.ways, t.win, t.num_cus, &gc));
+    uint32_t &gc = t.grid_cache[t.kernel == kKernelWalk ? 0 : 1][oi][ways_index(t.ways)][wpb_index(t.wpb)][staged];
+    if (!gc) WTC_HIPC(count_grid(t.kernel, ob, staged, t.ways, t.wpb, t.num_cus, &gc));
     s.G = G;
     s.grid = gc;
     s.small = small_n;
@@ -239,7 +240,9 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->max_spread = max_spread;
     plan->single = single;
     plan->ways = t.ways;
-    plan->win = t.win;
+    plan->wpb = t.wpb;
+    plan->kernel = t.kernel;
+    plan->work = count_work(t.grab, t.deal);
     plan->chunk = t.chunk;
     plan->desc_bytes = desc_bytes;
     plan->small_count = (uint32_t)small_parts;
@@ -262,7 +265,8 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
     WTC_HIPC(launch_count_small(ddesc, reinterpret_cast<const SmallPart *>(ddesc + p.G), p.small_count, p.ob,
                                 dcounts, p.small_lds, s.stream));
     if (p.tile_graphs)  // a set counted wholly by the small-graph kernel has no tiles
-        WTC_HIPC(launch_count(ddesc, p.G, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways, p.win, s.grid,
+        WTC_HIPC(launch_count(p.kernel, ddesc, p.G, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways, p.wpb,
+                              p.work, s.grid,
                               s.stream));
     if (timed) WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
     WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + p.G) * sizeof(uint64_t), hipMemcpyDeviceToHost,
@@ -313,7 +317,7 @@ int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt) {
 namespace {
 
 bool valid_ways(int w) { return w == 1 || w == 2 || w == 4 || w == 8; }
-bool valid_window(int v) { return v == 0 || v == 8 || v == 16; }
+bool valid_waves(int v) { return valid_count_waves(v); }
 
 // Default probes per lane for new contexts: WTC_PROBE_WAYS (1, 2, 4 or 8),
 // else 4.
@@ -323,11 +327,19 @@ int default_probe_ways() {
     return valid_ways(v) ? v : 4;
 }
 
-// Default probe window for new contexts: WTC_PROBE_WINDOW (0, 8 or 16), else 0.
-int default_probe_window() {
-    const char *e = std::getenv("WTC_PROBE_WINDOW");
-    const int v = e ? std::atoi(e) : 0;
-    return valid_window(v) ? v : 0;
+// Default waves per counting block for new contexts: WTC_COUNT_WAVES (1, 2,
+// 4 or 8), else 4.
+int default_count_waves() {
+    const char *e = std::getenv("WTC_COUNT_WAVES");
+    const int v = e ? std::atoi(e) : 4;
+    return valid_waves(v) ? v : 4;
+}
+
+// Default counting kernel for new contexts: WTC_COUNT_KERNEL (0 row walk,
+// 1 staged), else the row walk.
+int default_count_kernel() {
+    const char *e = std::getenv("WTC_COUNT_KERNEL");
+    return e && std::atoi(e) == kKernelStaged ? kKernelStaged : kKernelWalk;
 }
 
 int set_device(int dev) {
@@ -478,7 +490,8 @@ int wtc_ctx_create(int device, wtc_ctx **out) {
     }
     c->tune.num_cus = prop.multiProcessorCount;
     c->tune.ways = default_probe_ways();
-    c->tune.win = default_probe_window();
+    c->tune.wpb = default_count_waves();
+    c->tune.kernel = default_count_kernel();
     if ((rc = init_small_graphs(c->tune, prop, default_small_words()))) {
         delete c;
         return rc;
@@ -504,12 +517,25 @@ int wtc_ctx_get_small_graphs(const wtc_ctx *c, uint32_t *max_words, uint32_t *de
     return WTC_OK;
 }
 
-int wtc_ctx_set_tuning(wtc_ctx *c, int probe_ways, uint32_t lds_chunk, int probe_window) {
-    if (!c || !valid_ways(probe_ways) || lds_chunk < 1 || lds_chunk > kChunkMax || !valid_window(probe_window))
+int wtc_ctx_set_tuning(wtc_ctx *c, int probe_ways, uint32_t lds_chunk, int waves_per_block) {
+    if (!c || !valid_ways(probe_ways) || lds_chunk < 1 || lds_chunk > kChunkMax || !valid_waves(waves_per_block))
         return WTC_ERR_ARG;
     c->tune.ways = probe_ways;
     c->tune.chunk = lds_chunk;
-    c->tune.win = probe_window;
+    c->tune.wpb = waves_per_block;
+    return WTC_OK;
+}
+
+int wtc_ctx_set_work_split(wtc_ctx *c, uint32_t tiles_per_grab, int deal_to_xcds) {
+    if (!c || tiles_per_grab > 0xffffu) return WTC_ERR_ARG;
+    c->tune.grab = tiles_per_grab;
+    c->tune.deal = deal_to_xcds != 0;
+    return WTC_OK;
+}
+
+int wtc_ctx_set_count_kernel(wtc_ctx *c, int kernel) {
+    if (!c || (kernel != WTC_KERNEL_WALK && kernel != WTC_KERNEL_STAGED)) return WTC_ERR_ARG;
+    c->tune.kernel = kernel == WTC_KERNEL_WALK ? kKernelWalk : kKernelStaged;
     return WTC_OK;
 }
 
@@ -937,7 +963,9 @@ int batch_capture(wtc_ctx *c, wtc_batch *b, const std::vector<uint32_t> &s, cons
     b->key_s = s;
     b->key_a = a;
     b->key_ways = c->tune.ways;
-    b->key_win = c->tune.win;
+    b->key_wpb = c->tune.wpb;
+    b->key_kernel = c->tune.kernel;
+    b->key_work = count_work(c->tune.grab, c->tune.deal);
     b->key_chunk = c->tune.chunk;
     b->key_small = c->tune.small_words;
     return WTC_OK;
@@ -1069,7 +1097,8 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
     std::vector<uint32_t> s, aa;
     int rc = batch_params(b, spread, a, s, aa);
     if (rc) return rc;
-    if (!b->exec || s != b->key_s || aa != b->key_a || c->tune.ways != b->key_ways || c->tune.win != b->key_win ||
+    if (!b->exec || s != b->key_s || aa != b->key_a || c->tune.ways != b->key_ways || c->tune.wpb != b->key_wpb || c->tune.kernel != b->key_kernel ||
+        count_work(c->tune.grab, c->tune.deal) != b->key_work ||
         c->tune.chunk != b->key_chunk || c->tune.small_words != b->key_small) {
         if ((rc = batch_capture(c, b, s, aa))) return rc;
     }

@@ -12,11 +12,37 @@ __device__ __forceinline__ uint64_t binom2_d(uint64_t x) {
 // C(x,2) of a 32-bit x: x(x-1) < 2^64, so one 32x32->64 multiply and a shift.
 __device__ __forceinline__ uint64_t binom2_u32(uint32_t x) { return ((uint64_t)x * (uint64_t)(x - 1u)) >> 1; }
 
+// The top-a bitmap (tc.cu:262-272, 329-346) as a square a x a bit matrix over
+// the ranks [thr, thr + a), row-major in 32-bit words (a is a multiple of 32,
+// so every row starts on a word boundary): the bit of (w, u) is bit
+// (u - thr) % 32 of word (w - thr) * a/32 + (u - thr) / 32.  Only the strict
+// upper triangle u > w is ever set or read.  a32 = a/32.
+__device__ __forceinline__ uint64_t bm_word(uint32_t w, uint32_t u, uint32_t thr, uint32_t a32) {
+    return (uint64_t)(w - thr) * a32 + ((u - thr) >> 5);
+}
+__device__ __forceinline__ uint32_t bm_test(const uint32_t *__restrict__ bitmap, uint32_t w, uint32_t u,
+                                            uint32_t thr, uint32_t a32) {
+    return (bitmap[bm_word(w, u, thr, a32)] >> ((u - thr) & 31u)) & 1u;
+}
+
 __device__ __forceinline__ uint32_t uni32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// The lane-read builtins return int: each 32-bit half goes back through
+// uint32_t before it is widened, so the low half is never sign-extended.
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(x >> 32)) << 32) |
-           __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return uni32(x); }
+__device__ __forceinline__ uint64_t uni(uint64_t x) { return uni64(x); }
+
+// Value of x in lane `src`, which must be the same in every lane.
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t x, uint32_t src) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), (int)src);
+    return ((uint64_t)hi << 32) | lo;
 }
 
 // Value of x in lane src (ds_bpermute), for 64- and 32-bit values; every

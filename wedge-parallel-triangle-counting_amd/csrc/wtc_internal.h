@@ -97,7 +97,10 @@ struct Slot {
 struct Tuning {
     int num_cus = 0;
     int ways = 4;         // probes advanced together per lane
-    int win = 0;          // probe window in elements: 0, 8 or 16
+    int wpb = 4;          // waves per counting block: 1, 2, 4 or 8
+    int kernel = 0;       // kKernelWalk or kKernelStaged
+    uint32_t grab = 0;    // most tiles per grab (0: the kernel's default, 32)
+    bool deal = true;     // deal super-chunks of grabs to the XCDs in turn
     uint32_t chunk = 16;  // wedges per lane staged in LDS per phase
     // In sets of two or more graphs, a graph with 2(n+1) + m <= small_words
     // is counted by the small-graph kernel from LDS (0: never);
@@ -105,7 +108,7 @@ struct Tuning {
     // kSmallWordsMax).
     uint32_t small_words = 0;
     uint32_t small_words_max = 0;
-    uint32_t grid_cache[2][4][3][kChunkMax + 2] = {};
+    uint32_t grid_cache[2][2][4][4][kChunkMax + 2] = {};  // by kernel, offset width, ways, waves, staged
 };
 
 // Default small-graph limit of new contexts and servers, in LDS words
@@ -133,7 +136,8 @@ GraphDesc make_desc(const void *rowPtr, const uint32_t *colInd, uint32_t n, uint
 // What launch_set needs to replay the launches of a staged set.
 struct SetPlan {
     uint32_t G = 0, blocks = 0, max_spread = 1, chunk = 16;
-    int ob = 4, ways = 4, win = 0;
+    int ob = 4, ways = 4, wpb = 4, kernel = 0;
+    uint32_t work = 0;  // count_work(grab, deal)
     uint64_t rows = 0, clear_words = 0;
     bool single = false;
     // small_count blocks of the small-graph kernel, whose SmallPart list

@@ -9,6 +9,7 @@ namespace wtc {
 constexpr int kWave = 64;               // CDNA wavefront
 constexpr int kWavesPerBlock = 4;       // counting-kernel block = 256 lanes
 constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kCountWavesMax = 8;       // counting-kernel block: 1..8 waves, chosen per launch
 constexpr uint32_t kChunkMax = 16;      // wedges per lane staged in LDS at once
 constexpr int kTreeLevels = 12;         // cached levels of the wedge-prefix search tree
 constexpr uint32_t kTreeSize = (1u << kTreeLevels) - 1u;
@@ -37,7 +38,12 @@ static_assert(sizeof(SmallPart) == 8, "SmallPart layout");
 enum : uint32_t {
     kModeTiles = 0,  // counted by k_count in tiles of 64*s wedges
     kModeSmall = 1,  // counted by k_count_small, one block with the CSR in LDS
+    kModeInvalid = 2,  // failed the device-side CSR check: nothing is read
+                       // beyond rowPtr, and its count is set to kCountInvalid
 };
+
+// Count of a graph whose CSR failed the device-side check.
+constexpr unsigned long long kCountInvalid = ~0ull;
 
 // One graph of a graph set.  A graph set is any number of oriented CSRs
 // resident in HBM (ids local to each graph, row offsets relative to the
@@ -50,7 +56,7 @@ struct GraphDesc {
     const uint32_t *colInd;  // ids in [0, n), every row strictly ascending
     uint64_t *ws;            // n+1 wedge prefix (workspace slice)
     uint32_t *bitmap;        // ceil(C(a,2)/32) words (workspace slice), or null
-    uint64_t bm_size;        // C(a,2): bits of the graph's top-a bitmap
+    uint64_t bm_words;       // a*a/32: words of the graph's top-a bitmap
     uint64_t row_base;       // first bitmap row of this graph in the set's row list
     uint64_t blk_base;       // first prefix-scan block of this graph
     uint64_t bm_clear_base;  // first bitmap word of this graph in the set's clear list
@@ -61,7 +67,7 @@ struct GraphDesc {
     uint32_t a;              // bitmap length (top-a ranks), multiple of 32
     uint32_t thr;            // bitmap threshold max(n,a) - a (a = 0: n)
     uint32_t bm_rows;        // min(n, a)
-    uint32_t mode;           // kModeTiles or kModeSmall
+    uint32_t mode;           // kModeTiles, kModeSmall or kModeInvalid
     // device-filled by the setup kernel
     uint64_t wbase;          // first counted wedge (graph-local index)
     uint64_t wcount;         // wedges counted

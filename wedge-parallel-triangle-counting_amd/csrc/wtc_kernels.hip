@@ -126,8 +126,9 @@ __global__ __launch_bounds__(kBlock) void k_setup(GraphDesc *__restrict__ desc, 
             D.wbase = b;
             D.wcount = wc;
             const uint64_t per = (uint64_t)kWave * D.spread;
-            tiles = D.mode == kModeSmall ? 0 : (wc + per - 1) / per;  // small graphs: k_count_small
-            counts[g] = 0;
+            // small graphs: k_count_small; invalid ones: not counted
+            tiles = D.mode == kModeTiles ? (wc + per - 1) / per : 0;
+            counts[g] = D.mode == kModeInvalid ? kCountInvalid : 0ull;
         }
         uint64_t tot, wtot;
         const uint64_t ex = block_excl_scan(tiles, s_tot, tot);
@@ -181,7 +182,9 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_clear(const GraphDesc *__rest
     }
 }
 
-// One wave per bitmap row (rows v in [n - min(n,a), n) of every graph).
+// One wave per bitmap row (rows v in [n - min(n,a), n) of every graph): the
+// device form of the host loop tc.cu:329-346, in the square layout of
+// bm_word (a row owns its words alone).
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void k_bitmap_build(const GraphDesc *__restrict__ desc, uint32_t G,
                                                          uint64_t rows) {
@@ -191,24 +194,21 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_build(const GraphDesc *__rest
     for (uint64_t r = gw; r < rows; r += nw) {
         const uint32_t g = G > 1 ? graph_of(desc, G, r, &GraphDesc::row_base) : 0u;
         const GraphDesc &D = desc[g];
+        if (D.mode == kModeInvalid) continue;  // wave-uniform: its rows are never read
         const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
         const uint32_t *__restrict__ colInd = D.colInd;
         uint32_t *__restrict__ bm = D.bitmap;
         const uint32_t v = D.n - D.bm_rows + (uint32_t)(r - D.row_base);
-        const uint64_t rowbits = D.bm_size - binom2_u32(D.n - v) - v - 1;
-        // the row's bits are [rowbits + v + 1, rowbits + n - 1]: its first and
-        // last words may hold bits of the neighbouring rows, every word in
-        // between belongs to this row (and so to this wave) alone
-        const uint64_t fw = (rowbits + v + 1) >> 5, lw = (rowbits + D.n - 1) >> 5;
+        const uint32_t a32 = D.a >> 5;
         const OffT b = rowPtr[v], e = rowPtr[v + 1];
         for (OffT j0 = b; j0 < e; j0 += kWave) {  // wave-uniform
             const OffT j = j0 + lane;
             uint64_t word = ~0ull;  // no lane's word: inactive lanes never match
             uint32_t acc = 0;
             if (j < e) {
-                const uint64_t idx = rowbits + colInd[j];
-                word = idx >> 5;
-                acc = 1u << (idx & 31u);
+                const uint32_t u = colInd[j];
+                word = bm_word(v, u, D.thr, a32);
+                acc = 1u << ((u - D.thr) & 31u);
             }
             // ids ascend along the lanes, so lanes sharing a word are
             // consecutive: OR each run into its last lane (segmented scan)
@@ -222,13 +222,11 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_build(const GraphDesc *__rest
             const uint64_t first_w = shfl_u64(word, 0);
             const bool run_end = j < e && (lane == kWave - 1 || next_w != word);
             if (run_end) {
-                // shared with a neighbouring row, or possibly continued in the
-                // next chunk (run reaching lane 63) or begun in the previous
-                // one (run holding lane 0): atomic; otherwise a plain store
-                if (word == fw || word == lw || lane == kWave - 1 || word == first_w)
-                    atomicOr(&bm[word], acc);
-                else
-                    bm[word] = acc;
+                // a run reaching lane 63 may continue in the next 64-edge
+                // chunk and one holding lane 0 may have begun in the previous
+                // one: atomic; every other word belongs to this chunk alone
+                if (lane == kWave - 1 || word == first_w) atomicOr(&bm[word], acc);
+                else bm[word] = acc;
             }
         }
     }
@@ -245,6 +243,15 @@ using gptr = const __attribute__((address_space(1))) T *;
 template <class T>
 __device__ __forceinline__ gptr<T> to_global(const T *p) {
     return (gptr<T>)p;
+}
+// Constant-address-space views of the same read-only arrays, for reads whose
+// address is wave-uniform: they become scalar loads into SGPRs, so the
+// arithmetic that follows stays on the scalar unit.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <class T>
+__device__ __forceinline__ cptr<T> to_const(const T *p) {
+    return (cptr<T>)p;
 }
 
 // Rightmost v in [lo, hi] with ws[v] <= t, given ws[lo] <= t.  Called with
@@ -263,8 +270,8 @@ __device__ __forceinline__ uint32_t rightmost_le(P ws, uint32_t lo, uint32_t hi,
 // Wave-uniform search for the tile's first vertex over [0, n) of a single
 // graph: 12 levels from the compact search tree, then wedgeSum
 // (tc.cu:139-170 semantics).
-template <class P>
-__device__ __forceinline__ uint32_t tree_search(const uint64_t *__restrict__ tree, P ws, uint32_t n, uint64_t t) {
+template <class TP, class P>
+__device__ __forceinline__ uint32_t tree_search(TP tree, P ws, uint32_t n, uint64_t t) {
     uint32_t gs = 0, ge = n, ci = 0;
 #pragma unroll
     for (int l = 0; l < kTreeLevels; l++) {
@@ -326,30 +333,26 @@ __global__ __launch_bounds__(kBlock) void k_debug_decode(const uint32_t *__restr
     u_out[i] = u;
 }
 
-// Four ids loaded by one 16-byte vector load.
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-// The same at any element (4-byte) alignment, which the global memory
-// instructions accept for multi-dword loads.
+// Four ids loaded by one 16-byte vector load, at any element (4-byte)
+// alignment, which the global memory instructions accept for multi-dword
+// loads.
 typedef uint32_t u32x4_e __attribute__((ext_vector_type(4), aligned(4)));
 
-// Elements between the 16-byte aligned address at or below p and p.
-__device__ __forceinline__ uint32_t win_skew(gptr<uint32_t> p) { return (uint32_t)(((uintptr_t)p >> 2) & 3u); }
-
-// kWays: independent probes each lane advances together.  kWin: 0 for a plain
-// binary search over adj+(w); 8 or 16 to stop the search once the remaining
-// range fits an aligned window of that many elements and test the window
-// with vector loads (fewer dependent round trips per probe).
-template <typename OffT, int kWays, int kWin>
-__global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ desc, uint32_t G,
-                                                  const uint64_t *__restrict__ tree, uint64_t *__restrict__ meta,
-                                                  unsigned long long *__restrict__ counts, uint32_t pad_max,
-                                                  uint32_t chunk) {
+// kWays: independent probes each lane advances together.  The block holds
+// blockDim.x / 64 waves (1..kCountWavesMax), each with its own LDS staging.
+template <typename OffT, int kWays>
+__global__ __launch_bounds__(kWave * kCountWavesMax) void k_count(const GraphDesc *__restrict__ desc, uint32_t G,
+                                                                  const uint64_t *__restrict__ tree,
+                                                                  uint64_t *__restrict__ meta,
+                                                                  unsigned long long *__restrict__ counts,
+                                                                  uint32_t pad_max, uint32_t chunk, uint32_t work) {
     extern __shared__ uint2 lds_wu[];
-    __shared__ uint32_t s_g[kWavesPerBlock];
-    __shared__ uint64_t s_c[kWavesPerBlock];
+    __shared__ uint32_t s_g[kCountWavesMax];
+    __shared__ uint64_t s_c[kCountWavesMax];
 
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wid = uni32(threadIdx.x >> 6);
+    const uint32_t wpb = blockDim.x >> 6;  // waves of this block
     // this wave's staging: (w, u) of a staged wedge as one 8-byte entry,
     // lane l's k-th wedge at l * pad + k
     uint2 *const lwu = lds_wu + wid * (kWave * pad_max);
@@ -367,13 +370,17 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
     // grab size: about two grabs per wave, at most kTilesPerGrabMax tiles;
     // super-chunks of 2^lg grabs, the largest power of two (at most 64) that
     // still gives every XCD at least 64 super-chunks
-    const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
+    const uint64_t waves = (uint64_t)gridDim.x * wpb;
     uint64_t tpg = ntiles / (2 * waves);
-    tpg = tpg < 1 ? 1 : tpg > kTilesPerGrabMax ? kTilesPerGrabMax : tpg;
+    const uint64_t gmax = (work & 0xffffu) ? (work & 0xffffu) : kTilesPerGrabMax;
+    tpg = tpg < 1 ? 1 : tpg > gmax ? gmax : tpg;
     const uint64_t ngrabs = (ntiles + tpg - 1) / tpg;
+    // dealing to the XCDs (work bit 16) or one stream for the whole grid
+    const bool deal = (work >> 16) & 1u;
+    const uint32_t streams = deal ? 8u : 1u;
     uint32_t lg = 0;
-    while (lg < 6 && (ngrabs >> (lg + 1)) >= 8ull * 64) lg++;
-    const uint32_t xcd = blockIdx.x & 7u;
+    while (deal && lg < 6 && (ngrabs >> (lg + 1)) >= 8ull * 64) lg++;
+    const uint32_t xcd = deal ? blockIdx.x & 7u : 0u;
     unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaNext + xcd);
 
     uint32_t cur_g = kSentinel;
@@ -383,7 +390,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
         unsigned long long grab = 0;
         if (lane == 0) grab = atomicAdd(ctr, 1ull);
         const uint64_t gi = uni64(grab);
-        const uint64_t chunk_id = xcd + 8 * (gi >> lg);  // this XCD's super-chunk
+        const uint64_t chunk_id = xcd + streams * (gi >> lg);  // this XCD's super-chunk
         const uint64_t tb = ((chunk_id << lg) + (gi & ((1ull << lg) - 1))) * tpg;
         if (tb >= ntiles) break;  // tb grows with gi, so every later grab is past the end too
         const uint64_t te = tb + tpg < ntiles ? tb + tpg : ntiles;
@@ -397,7 +404,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
         gptr<uint64_t> ws = nullptr;
         gptr<uint32_t> bitmap = nullptr;
         uint32_t s = 1, sc = 1, pad = 1, gn = 1, thr = 0;
-        uint64_t wbase = 0, wend = 0, tile_base = 0, bm_size = 0;
+        uint32_t a32 = 0, bm_rows = 0;
+        bool bm_narrow = true;
+        uint64_t wbase = 0, wend = 0, tile_base = 0;
 
         for (uint64_t t = tb; t < te; t++) {
             if (g == kSentinel || t >= g_tile_end) {
@@ -412,7 +421,9 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                 pad = sc | 1u;
                 gn = D.n;
                 thr = D.thr;
-                bm_size = D.bm_size;
+                a32 = D.a >> 5;
+                bm_rows = D.bm_rows;
+                bm_narrow = D.bm_words < (1ull << 32);
                 wbase = D.wbase;
                 wend = D.wbase + D.wcount;
                 tile_base = D.tile_base;
@@ -462,16 +473,39 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
             // wedges of this lane not yet staged (<= s, so 32 bits); the 64-bit
             // index of the next one is lend - left
             uint32_t left = first < lend ? (uint32_t)(lend - first) : 0u;
-            // rightmost window index in [lo_l, 63] with wedgeSum <= first
-            // (binary lifting; every lane takes part in the shuffles)
             uint32_t pos = lo_l;
+            uint64_t iv = 0;  // local index of the lane's first wedge in its vertex
+            if (s < (1u << 25)) {  // wave-uniform: every lane offset lane*s fits 31 bits
+                // The window's prefixes relative to the tile start, saturated
+                // to 32 bits: 0 for the vertices starting at or before it,
+                // ~0 for those 2^32 or more wedges on (and past the graph).
+                // Compared with the lane's offset lane*s < 2^31 they order
+                // exactly as the 64-bit prefixes do against its first wedge.
+                const uint64_t dw = wwin - tile_start;
+                const uint32_t rel = wwin <= tile_start ? 0u : dw > 0xffffffffull ? 0xffffffffu : (uint32_t)dw;
+                const uint32_t foff = lane * s;
+                // rightmost window index in [lo_l, 63] with prefix <= first
+                // (binary lifting; every lane takes part in the shuffles)
 #pragma unroll
-            for (uint32_t step = 32; step > 0; step >>= 1) {
-                const uint32_t cand = pos + step;
-                const uint64_t wc = shfl_u64(wwin, cand < (uint32_t)kWave ? cand : kWave - 1);
-                pos = cand < (uint32_t)kWave && wc <= first ? cand : pos;
+                for (uint32_t step = 32; step > 0; step >>= 1) {
+                    const uint32_t cand = pos + step;
+                    const uint32_t rc = shfl_off(rel, cand < (uint32_t)kWave ? cand : kWave - 1);
+                    pos = cand < (uint32_t)kWave && rc <= foff ? cand : pos;
+                }
+                const uint32_t rel_v = shfl_off(rel, pos);
+                // the tile's first vertex may start before the tile: its exact
+                // prefix is the wave-uniform window value at lo_l
+                const uint64_t ws_lo = readlane_u64(wwin, lo_l);
+                iv = pos == lo_l ? first - ws_lo : (uint64_t)(foff - rel_v);
+            } else {
+#pragma unroll
+                for (uint32_t step = 32; step > 0; step >>= 1) {
+                    const uint32_t cand = pos + step;
+                    const uint64_t wc = shfl_u64(wwin, cand < (uint32_t)kWave ? cand : kWave - 1);
+                    pos = cand < (uint32_t)kWave && wc <= first ? cand : pos;
+                }
+                iv = first - shfl_u64(wwin, pos);
             }
-            const uint64_t ws_v = shfl_u64(wwin, pos);
             const OffT rb = shfl_off(rwin, pos);
             const OffT re = shfl_off(rwin, pos + 1 < (uint32_t)kWave ? pos + 1 : kWave - 1);
             uint32_t v = 0, d = 0, w_i = 0, u_i = 0;
@@ -486,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                     v = base + pos;
                     vb = rb;
                     d = (uint32_t)((pos + 1 < (uint32_t)kWave ? re : rowPtr[v + 1]) - vb);
-                    decode_wedge(d, first - ws_v, w_i, u_i);
+                    decode_wedge(d, iv, w_i, u_i);
                 }
             }
             // hits of the wave in this tile: popcount(ballot(match)) as each
@@ -573,15 +607,15 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                         }
                     }
                     uint32_t wv[4], uv[4], wl[4];
-                    // Inside a row the group's u positions are consecutive:
+                    // Inside a row a full group's u positions are consecutive:
                     // then one 16-byte load per lane fetches all four (one
-                    // vL1D access instead of four; up to three elements past a
-                    // lane's last wedge are read, which stay inside colInd or
-                    // its allocation's tail slack).  Across a row end the ids
-                    // are loaded one by one.
+                    // vL1D access instead of four).  Across a row end, and for
+                    // a lane's last partial group, the ids are loaded one by
+                    // one, so no load reaches past the lane's last wedge (nor
+                    // past the end of colInd).
                     bool cont = ok[0];
 #pragma unroll
-                    for (int j = 1; j < 4; j++) cont = cont && (!ok[j] || upos[j] == upos[0] + j);
+                    for (int j = 1; j < 4; j++) cont = cont && ok[j] && upos[j] == upos[0] + j;
 #pragma unroll
                     for (int j = 0; j < 4; j++) {  // ok[] is a prefix of the group
                         const OffT prev = j == 0 ? last_wpos : wpos[j - 1];
@@ -617,128 +651,118 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
                 }
                 __syncwarp();
                 // phase 2: consecutive lanes probe consecutive wedges; each lane
-                // advances kWays independent searches in lockstep so that their
+                // advances kWays independent probes in lockstep so that their
                 // dependent loads overlap (memory-level parallelism per lane).
                 // Step j of lane l reads staged wedge p = 64j + l, which lane
-                // p / kn wrote as its (p % kn)-th: slot (p / kn) * pad + p % kn.
-                // The slot is advanced from step to step (p grows by 64 =
-                // sa*kn + sb) instead of dividing again.  Division of lane by
-                // kn once: lane < 64 and magic <= 2^16, so the 24-bit
-                // multiplies are exact.
+                // p / kn wrote as its (p % kn)-th, at slot (p / kn) * pad + p % kn
+                // = p + (p / kn)(pad - kn).  When kn divides 64 that is
+                // slot0 + j * (64 / kn) * pad with slot0 the slot of step 0: one
+                // add per read.  Otherwise p / kn = (p * magic) >> 16 with
+                // magic = ceil(2^16 / kn), exact for p < 64 * kChunkMax = 1024
+                // (the error p(magic kn - 2^16) / 2^16 < 1/kn), in 24-bit
+                // multiplies.
                 const uint32_t magic = (65536u + kn - 1) / kn;
-                const uint32_t sa = kWave / kn, sb = kWave % kn;
-                const uint32_t step0 = sa * pad + sb, step1 = step0 + pad - kn;
-                uint32_t q0 = __umul24(lane, magic) >> 16;
-                uint32_t rem = lane - __umul24(q0, kn);
-                uint32_t slot = __umul24(q0, pad) + rem;
+                const uint32_t dpad = pad - kn;
+                const bool lin = (kWave % kn) == 0;  // wave-uniform
+                const uint32_t lstep = (kWave / kn) * pad;
+                const uint2 *const lrow = lwu + (lane + __umul24(__umul24(lane, magic) >> 16, dpad));
                 for (uint32_t j0 = 0; j0 < kn; j0 += kWays) {
                     uint32_t pw[kWays], pu[kWays];
-                    OffT b[kWays], len[kWays];
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
                         pw[k] = kSentinel;
                         pu[k] = 0;
-                        if (j0 + k < kn) {
-                            const uint2 e = lwu[slot];
+                        if (j0 + k < kn) {  // wave-uniform
+                            uint2 e;
+                            if (lin) {
+                                e = lrow[(j0 + k) * lstep];
+                            } else {
+                                const uint32_t p = (j0 + k) * kWave + lane;
+                                const uint32_t src = __umul24(p, magic) >> 16;
+                                e = lwu[__umul24(src, dpad) + p];
+                            }
                             pw[k] = e.x;
                             pu[k] = e.y;
-                            rem += sb;
-                            const bool carry = rem >= kn;
-                            rem = carry ? rem - kn : rem;
-                            slot += carry ? step1 : step0;
                         }
                     }
                     // Every way's first load is issued before any is used, so
                     // the ways' round trips overlap: a bitmap word for w in the
                     // top-a ranks (tc.cu:262-272), else rowPtr[w], rowPtr[w+1]
                     // for the sorted-list probe of adj+(w) (tc.cu:273-286).
+                    // (tc.cu:262-272: w in the top-a ranks, r = w - thr < min(n, a);
+                    // empty slots hold w = kSentinel, above every rank)
                     bool isb[kWays], isl[kWays];
-                    uint64_t bidx[kWays];
-                    uint32_t bw[kWays];
+                    uint32_t bcol[kWays], bw[kWays];
+                    uint64_t bwd[kWays];
                     OffT r0[kWays], r1[kWays];
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
-                        isb[k] = pw[k] != kSentinel && pw[k] >= thr;
+                        const uint32_t r = pw[k] - thr;
+                        isb[k] = r < bm_rows;
                         isl[k] = pw[k] < thr;
-                        bidx[k] = isb[k] ? bm_size - binom2_u32(gn - pw[k]) + (pu[k] - pw[k] - 1) : 0;
+                        bcol[k] = pu[k] - thr;
+                        // word of (w, u) in the square layout: a 24-bit
+                        // multiply-add while the bitmap has < 2^32 words
+                        bwd[k] = bm_narrow ? (uint64_t)(__umul24(r, a32) + (bcol[k] >> 5))
+                                           : (uint64_t)r * a32 + (bcol[k] >> 5);
                     }
 #pragma unroll
-                    for (int k = 0; k < kWays; k++) bw[k] = isb[k] ? bitmap[bidx[k] >> 5] : 0u;
+                    for (int k = 0; k < kWays; k++) bw[k] = isb[k] ? bitmap[bwd[k]] : 0u;
 #pragma unroll
                     for (int k = 0; k < kWays; k++) {
-                        r0[k] = isl[k] ? rowPtr[pw[k]] : (OffT)0;
-                        r1[k] = isl[k] ? rowPtr[pw[k] + 1] : (OffT)0;
-                    }
-#pragma unroll
-                    for (int k = 0; k < kWays; k++) {
-                        tile_cnt += (uint64_t)__popcll(__ballot((bw[k] >> (bidx[k] & 31u)) & 1u));
-                        b[k] = r0[k];
-                        len[k] = r1[k] - r0[k];
-                    }
-                    // Binary search, updates by selects; with a window, only
-                    // until the rest of the range fits one aligned window.
-                    for (;;) {
-                        bool act[kWays], more = false;
-#pragma unroll
-                        for (int k = 0; k < kWays; k++) {
-                            act[k] = kWin == 0 ? len[k] > 0
-                                               : (uint64_t)len[k] + win_skew(colInd + b[k]) > (uint64_t)kWin;
-                            more |= act[k];
+                        r0[k] = 0;
+                        r1[k] = 0;
+                        if (isl[k]) {
+                            const gptr<OffT> rp = rowPtr + pw[k];
+                            r0[k] = rp[0];
+                            r1[k] = rp[1];
                         }
+                    }
+                    // Sorted-list probe of u in adj+(w) = colInd[r0, r1): a
+                    // halving search over [r0 - 1, r1) whose first position
+                    // stands for an element below u and is never read.  base
+                    // is a position below u's lower bound and base + len one
+                    // at or above it (r1: past the row); a step reads
+                    // base + len/2, moves base there if that element is below
+                    // u, and len loses len/2.  The lower bound, when it lies
+                    // in the row, is the last position read at or above u, so
+                    // a hit shows as an equal element read at some step; the
+                    // hits of a way are kept as a lane mask (one compare per
+                    // step, no per-step count).  At len = 1 a probe is done:
+                    // len/2 = 0 then moves nothing, so finished, bitmap and
+                    // empty slots need no masking beyond their loads, and a
+                    // stale element only repeats a hit already in the mask.
+                    OffT bs[kWays];
+                    uint32_t len[kWays], x[kWays];
+                    uint64_t hit[kWays];
+#pragma unroll
+                    for (int k = 0; k < kWays; k++) {
+                        tile_cnt += (uint64_t)__popcll(__ballot(__builtin_amdgcn_ubfe(bw[k], bcol[k], 1u)));
+                        bs[k] = r0[k] - 1;
+                        len[k] = isl[k] ? (uint32_t)(r1[k] - r0[k]) + 1u : 1u;
+                        x[k] = kSentinel;  // never a vertex id
+                        hit[k] = 0;
+                    }
+                    for (;;) {
+                        bool more = false;
+#pragma unroll
+                        for (int k = 0; k < kWays; k++) more |= len[k] > 1u;
                         // wave-uniform exit: every lane stays in the loop until
                         // all are done, so the ballots below see the whole wave
                         if (__ballot(more) == 0) break;
-                        uint32_t x[kWays];
 #pragma unroll
-                        for (int k = 0; k < kWays; k++) x[k] = act[k] ? colInd[b[k] + (len[k] >> 1)] : 0u;
+                        for (int k = 0; k < kWays; k++)
+                            if (len[k] > 1u) x[k] = colInd[bs[k] + (len[k] >> 1)];
 #pragma unroll
                         for (int k = 0; k < kWays; k++) {
-                            // go right past the probed element, or keep the left
-                            // half; no early exit on a match (the range left
-                            // holds only smaller ids, so nothing matches twice),
-                            // so the update is a few selects.  Inactive ways:
-                            // go = 0 keeps b, and len stays (0 without a window).
-                            const OffT half = len[k] >> 1;
-                            const bool lt = x[k] < pu[k];
-                            const OffT go = act[k] && lt ? half + 1 : (OffT)0;
-                            tile_cnt += (uint64_t)__popcll(__ballot(act[k] && x[k] == pu[k]));
-                            b[k] += go;
-                            const OffT nl = lt ? len[k] - go : half;
-                            len[k] = kWin == 0 ? nl : act[k] ? nl : len[k];
+                            const uint32_t half = len[k] >> 1;
+                            hit[k] |= __ballot(x[k] == pu[k]);
+                            bs[k] = x[k] < pu[k] ? bs[k] + half : bs[k];
+                            len[k] -= half;
                         }
                     }
-                    if constexpr (kWin > 0) {
-                        // [b, b+len) lies inside the 16-byte aligned window that
-                        // starts at or just below colInd + b: one vector load per
-                        // 4 elements, then compares in registers.  The window may
-                        // reach up to kWin-1 elements past the end of colInd;
-                        // every colInd allocation carries that much tail slack.
-                        u32x4 wv4[kWays][kWin / 4];
 #pragma unroll
-                        for (int k = 0; k < kWays; k++) {
-                            if (len[k] > 0) {
-                                const gptr<u32x4> p = (gptr<u32x4>)((uintptr_t)(colInd + b[k]) & ~(uintptr_t)15);
-#pragma unroll
-                                for (int q = 0; q < kWin / 4; q++) wv4[k][q] = p[q];
-                            }
-                        }
-#pragma unroll
-                        for (int k = 0; k < kWays; k++) {
-                            bool f = false;
-                            if (len[k] > 0) {
-                                const uint32_t lo = win_skew(colInd + b[k]), hi = lo + (uint32_t)len[k];
-#pragma unroll
-                                for (int q = 0; q < kWin / 4; q++) {
-                                    const uint32_t e0 = 4 * q;
-                                    f |= (wv4[k][q].x == pu[k]) & (e0 + 0 >= lo) & (e0 + 0 < hi);
-                                    f |= (wv4[k][q].y == pu[k]) & (e0 + 1 >= lo) & (e0 + 1 < hi);
-                                    f |= (wv4[k][q].z == pu[k]) & (e0 + 2 >= lo) & (e0 + 2 < hi);
-                                    f |= (wv4[k][q].w == pu[k]) & (e0 + 3 >= lo) & (e0 + 3 < hi);
-                                }
-                            }
-                            tile_cnt += (uint64_t)__popcll(__ballot(f));  // converged: outside the branch
-                        }
-                    }
+                    for (int k = 0; k < kWays; k++) tile_cnt += (uint64_t)__popcll(hit[k]);
                 }
                 __syncwarp();
             }
@@ -759,10 +783,308 @@ __global__ __launch_bounds__(kBlock) void k_count(const GraphDesc *__restrict__ 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int a = 0; a < kWavesPerBlock; a++) {
+        for (uint32_t a = 0; a < wpb; a++) {
             if (s_g[a] == kSentinel || s_c[a] == 0) continue;
             uint64_t sum = s_c[a];
-            for (int b = a + 1; b < kWavesPerBlock; b++)
+            for (uint32_t b = a + 1; b < wpb; b++)
+                if (s_g[b] == s_g[a]) {
+                    sum += s_c[b];
+                    s_c[b] = 0;
+                }
+            atomicAdd(&counts[s_g[a]], (unsigned long long)sum);
+        }
+    }
+}
+
+// ------------------------------------------------------------- row walk
+
+// The last kTail entries of every row form the row's short triangle; rows
+// with kTail or more u after w are its long rows.
+constexpr uint32_t kTail = 64;
+constexpr uint32_t kTailPairs = kTail * (kTail - 1) / 2;  // 2016
+
+// Wedges of a vertex with out-degree d in its long rows: the first
+// C(d,2) - C(kTail,2) wedges of its row-major order.
+__device__ __forceinline__ uint64_t long_wedges(uint32_t d) {
+    return d > kTail ? binom2_u32(d) - (uint64_t)kTailPairs : 0ull;
+}
+
+// k_walk counts the same wedges in the same order as k_count (the order of
+// tc.cu:203-240: vertices ascending, each vertex's wedges (w_i, u_i)
+// row-major), but assigns them to lanes differently.  A wave owns a
+// contiguous range of the order (the tiles of a grab) and walks it in steps
+// of up to 64 consecutive wedges, one per lane, so consecutive lanes always
+// probe consecutive wedges (the cooperative probe order of tc.cu:251-287)
+// without per-lane decoding or LDS staging:
+//  * a long step covers up to 64 wedges of the long rows of one vertex:
+//    lane l takes u at position ui + l of row wi, or of row wi + 1 when row
+//    wi ends inside the step (rows are at least 64 long, so a step spans at
+//    most two); the u are one coalesced load and w is one of two
+//    wave-uniform ids;
+//  * a tail step covers up to 64 wedges of one vertex's short triangle (the
+//    pairs of its last m = min(d, 64) entries): the m entries are loaded
+//    once into lanes 64 - m .. 63 of a register, and lane l takes pair
+//    p0 + l of the 64-entry triangle (the short triangle's pairs are its
+//    last C(m,2) pairs in row-major order) from an LDS table of (i, j), then
+//    w and u by two shuffles.
+// Only the walk state (vertex, row, position) is per wave; it advances on
+// the scalar unit.  kSteps steps are prepared before any of their loads is
+// used, so a lane has kSteps independent probes in flight.
+template <typename OffT, int kSteps>
+__global__ __launch_bounds__(kWave * kCountWavesMax) void k_walk(const GraphDesc *__restrict__ desc, uint32_t G,
+                                                                 const uint64_t *__restrict__ tree,
+                                                                 uint64_t *__restrict__ meta,
+                                                                 unsigned long long *__restrict__ counts,
+                                                                 uint32_t work) {
+    __shared__ uint16_t s_pair[kTailPairs];  // pair p of the 64-triangle: i | j << 8
+    __shared__ uint32_t s_g[kCountWavesMax];
+    __shared__ uint64_t s_c[kCountWavesMax];
+    for (uint32_t i = threadIdx.x; i < kTail - 1; i += blockDim.x) {
+        const uint32_t r0 = i * (2 * kTail - 1 - i) / 2;  // first pair of row i
+        for (uint32_t j = i + 1; j < kTail; j++) s_pair[r0 + j - i - 1] = (uint16_t)(i | (j << 8));
+    }
+    __syncthreads();
+
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = uni32(threadIdx.x >> 6);
+    const uint32_t wpb = blockDim.x >> 6;
+
+    // work distribution as in k_count: XCD-interleaved super-chunks of
+    // grabs of consecutive tiles from the XCD's own counter
+    const uint64_t ntiles = meta[kMetaTiles];
+    const uint64_t waves = (uint64_t)gridDim.x * wpb;
+    uint64_t tpg = ntiles / (2 * waves);
+    const uint64_t gmax = (work & 0xffffu) ? (work & 0xffffu) : kTilesPerGrabMax;
+    tpg = tpg < 1 ? 1 : tpg > gmax ? gmax : tpg;
+    const uint64_t ngrabs = (ntiles + tpg - 1) / tpg;
+    // dealing to the XCDs (work bit 16) or one stream for the whole grid
+    const bool deal = (work >> 16) & 1u;
+    const uint32_t streams = deal ? 8u : 1u;
+    uint32_t lg = 0;
+    while (deal && lg < 6 && (ngrabs >> (lg + 1)) >= 8ull * 64) lg++;
+    const uint32_t xcd = deal ? blockIdx.x & 7u : 0u;
+    unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaNext + xcd);
+
+    uint32_t cur_g = kSentinel;
+    uint64_t cur_cnt = 0;
+
+    for (;;) {
+        unsigned long long grab = 0;
+        if (lane == 0) grab = atomicAdd(ctr, 1ull);
+        const uint64_t gi = uni64(grab);
+        const uint64_t chunk_id = xcd + streams * (gi >> lg);
+        const uint64_t tb = ((chunk_id << lg) + (gi & ((1ull << lg) - 1))) * tpg;
+        if (tb >= ntiles) break;
+        const uint64_t te = tb + tpg < ntiles ? tb + tpg : ntiles;
+
+        for (uint64_t t = tb; t < te;) {
+            // the grab's tiles of one graph: wedges [pos, pend) of its order
+            const uint32_t g = G > 1 ? graph_of(desc, G, t, &GraphDesc::tile_base) : 0u;
+            const GraphDesc &D = desc[g];
+            const gptr<OffT> rowPtr = to_global(static_cast<const OffT *>(D.rowPtr));
+            const gptr<uint32_t> colInd = to_global(D.colInd);
+            const gptr<uint64_t> ws = to_global(static_cast<const uint64_t *>(D.ws));
+            const gptr<uint32_t> bitmap = to_global(static_cast<const uint32_t *>(D.bitmap));
+            // the same arrays for wave-uniform reads (scalar loads)
+            const cptr<OffT> rowPtrU = to_const(static_cast<const OffT *>(D.rowPtr));
+            const cptr<uint32_t> colIndU = to_const(D.colInd);
+            const cptr<uint64_t> wsU = to_const(static_cast<const uint64_t *>(D.ws));
+            const uint32_t gn = D.n, vmax = gn - 1, thr = D.thr, bm_rows = D.bm_rows, a32 = D.a >> 5;
+            const uint64_t T = (uint64_t)kWave * D.spread;
+            const uint64_t wend = D.wbase + D.wcount;
+            const uint64_t g_tile_end = D.tile_base + (D.wcount + T - 1) / T;
+            const uint64_t t1 = te < g_tile_end ? te : g_tile_end;
+            // (the walk state is wave-uniform: readfirstlane keeps it in
+            // SGPRs, so the walk runs on the scalar unit)
+            uint64_t pos = uni(D.wbase + (t - D.tile_base) * T);
+            const uint64_t pe = D.wbase + (t1 - D.tile_base) * T;
+            const uint64_t pend = uni(pe < wend ? pe : wend);
+            t = t1;
+            if (g != cur_g) {
+                if (cur_cnt != 0 && lane == 0) atomicAdd(&counts[cur_g], (unsigned long long)cur_cnt);
+                cur_g = g;
+                cur_cnt = 0;
+            }
+
+            // walk state (wave-uniform): vertex v with vb, d, its C(d,2)
+            // wedges cv of which lv in long rows, position iv in it; in the
+            // long rows the current row wi, position ui and w id wcur
+            uint32_t v = uni((tree != nullptr) ? tree_search(to_const(tree), wsU, gn, pos)
+                                               : rightmost_le(wsU, 0u, vmax, pos));
+            OffT vb = uni(rowPtrU[v]);
+            uint32_t d = uni((uint32_t)(rowPtrU[v + 1] - vb));
+            uint64_t cv = binom2_u32(d), lv = long_wedges(d), iv = uni(pos - wsU[v]);
+            uint32_t wi = 0, ui = 1, wcur = 0;
+            if (iv < lv) {
+                decode_wedge(d, iv, wi, ui);
+                wi = uni32(wi);
+                ui = uni32(ui);
+                wcur = uni(colIndU[vb + wi]);
+            }
+            uint64_t hits = 0;
+            while (pos < pend) {
+                // step parameters (wave-uniform): n wedges; long: u of lanes
+                // below `split` at pa + lane, the rest at pb + lane - split,
+                // w = wa / wb; tail: window from pa (lanes from `split` =
+                // 64 - m valid), pairs from pb
+                uint32_t n[kSteps], split[kSteps], wa[kSteps], wb[kSteps];
+                bool tail[kSteps];
+                OffT pa[kSteps], pb[kSteps];
+#pragma unroll
+                for (int k = 0; k < kSteps; k++) {
+                    n[k] = 0;
+                    split[k] = 0;
+                    wa[k] = wb[k] = 0;
+                    tail[k] = false;
+                    pa[k] = pb[k] = 0;
+                    if (pos >= pend) continue;
+                    if (iv == cv) {  // vertex done: next vertex with wedges (pos = ws[v + 1] < W)
+                        v = uni(wsU[v + 2] > pos ? v + 1 : gallop_le(wsU, v + 2, vmax, pos));
+                        vb = uni(rowPtrU[v]);
+                        d = uni((uint32_t)(rowPtrU[v + 1] - vb));
+                        cv = binom2_u32(d);
+                        lv = long_wedges(d);
+                        iv = 0;
+                        wi = 0;
+                        ui = 1;
+                        if (lv) wcur = uni(colIndU[vb]);
+                    }
+                    const uint64_t left = pend - pos;
+                    if (iv < lv) {
+                        uint64_t m = lv - iv;
+                        m = m < left ? m : left;
+                        const uint32_t na = m < kWave ? (uint32_t)m : kWave;
+                        const uint32_t inrow = d - ui;
+                        n[k] = na;
+                        split[k] = inrow;
+                        pa[k] = vb + ui;
+                        pb[k] = vb + wi + 2;
+                        wa[k] = wcur;
+                        wb[k] = uni(na > inrow ? colIndU[vb + wi + 1] : wcur);
+                        pos += na;
+                        iv += na;
+                        ui += na;
+                        if (ui >= d) {  // row wi ended: next row (at least 64 long while iv < lv)
+                            const uint32_t over = ui - d;
+                            wi++;
+                            ui = wi + 1 + over;
+                            wcur = uni(na > inrow ? wb[k] : iv < lv ? colIndU[vb + wi] : 0u);
+                        }
+                        pos = uni(pos);
+                        iv = uni(iv);
+                        wi = uni(wi);
+                        ui = uni(ui);
+                    } else {
+                        const uint32_t mt = d < kTail ? d : kTail;
+                        const uint64_t tp = iv - lv;  // < C(mt, 2) <= 2016
+                        const uint32_t ct = mt * (mt - 1) / 2;
+                        uint64_t m = ct - tp;
+                        m = m < left ? m : left;
+                        const uint32_t na = m < kWave ? (uint32_t)m : kWave;
+                        tail[k] = true;
+                        n[k] = na;
+                        split[k] = kTail - mt;
+                        pa[k] = vb + d - kTail;  // lanes split..63 hold the last mt entries
+                        pb[k] = kTailPairs - ct + (uint32_t)tp;
+                        pos = uni(pos + na);
+                        iv = uni(iv + na);
+                    }
+                }
+                // loads: u of a long step, the short-triangle window of a tail step
+                uint32_t ld[kSteps];
+#pragma unroll
+                for (int k = 0; k < kSteps; k++) {
+                    ld[k] = 0;
+                    if (!tail[k]) {
+                        if (lane < n[k]) ld[k] = colInd[lane < split[k] ? pa[k] + lane : pb[k] + (lane - split[k])];
+                    } else if (lane >= split[k]) {
+                        ld[k] = colInd[pa[k] + lane];
+                    }
+                }
+                // (w, u) of each lane's wedge; finished lanes get w = kSentinel
+                uint32_t pw[kSteps], pu[kSteps];
+#pragma unroll
+                for (int k = 0; k < kSteps; k++) {
+                    const bool act = lane < n[k];
+                    if (!tail[k]) {  // wave-uniform
+                        pw[k] = lane < split[k] ? wa[k] : wb[k];
+                        pu[k] = ld[k];
+                    } else {
+                        const uint32_t pr = s_pair[(uint32_t)pb[k] + (act ? lane : 0u)];
+                        pw[k] = shfl_off(ld[k], pr & 0xffu);
+                        pu[k] = shfl_off(ld[k], pr >> 8);
+                    }
+                    if (!act) {
+                        pw[k] = kSentinel;
+                        pu[k] = 0;
+                    }
+                }
+                // probes: a bitmap word for w in the top-a ranks
+                // (tc.cu:262-272), else the sorted-list probe of adj+(w)
+                // (tc.cu:273-286) by the halving search of k_count.  Each
+                // way's loads are issued before any result is used; w dies
+                // with them, so only u, the bit's column and the search
+                // state stay live.
+                uint32_t bw[kSteps], bcol[kSteps];
+                OffT r0[kSteps], r1[kSteps];
+#pragma unroll
+                for (int k = 0; k < kSteps; k++) {
+                    const uint32_t r = pw[k] - thr;
+                    bcol[k] = pu[k] - thr;
+                    bw[k] = 0;
+                    if (r < bm_rows) bw[k] = bitmap[(uint64_t)r * a32 + (bcol[k] >> 5)];
+                    r0[k] = 0;
+                    r1[k] = 0;
+                    if (pw[k] < thr) {
+                        const gptr<OffT> rp = rowPtr + pw[k];
+                        r0[k] = rp[0];
+                        r1[k] = rp[1];
+                    }
+                }
+                OffT bs[kSteps];
+                uint32_t len[kSteps], x[kSteps];
+                uint64_t hit[kSteps];
+#pragma unroll
+                for (int k = 0; k < kSteps; k++) {
+                    hits += (uint64_t)__popcll(__ballot(__builtin_amdgcn_ubfe(bw[k], bcol[k], 1u)));
+                    bs[k] = r0[k] - 1;
+                    len[k] = (uint32_t)(r1[k] - r0[k]) + 1u;  // 1 (nothing to search) unless a list probe
+                    x[k] = kSentinel;
+                    hit[k] = 0;
+                }
+                for (;;) {
+                    bool more = false;
+#pragma unroll
+                    for (int k = 0; k < kSteps; k++) more |= len[k] > 1u;
+                    if (__ballot(more) == 0) break;
+#pragma unroll
+                    for (int k = 0; k < kSteps; k++)
+                        if (len[k] > 1u) x[k] = colInd[bs[k] + (len[k] >> 1)];
+#pragma unroll
+                    for (int k = 0; k < kSteps; k++) {
+                        const uint32_t half = len[k] >> 1;
+                        hit[k] |= __ballot(x[k] == pu[k]);
+                        bs[k] = x[k] < pu[k] ? bs[k] + half : bs[k];
+                        len[k] -= half;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < kSteps; k++) hits += (uint64_t)__popcll(hit[k]);
+            }
+            cur_cnt += hits;
+        }
+    }
+
+    if (lane == 0) {
+        s_g[wid] = cur_g;
+        s_c[wid] = cur_cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t a = 0; a < wpb; a++) {
+            if (s_g[a] == kSentinel || s_c[a] == 0) continue;
+            uint64_t sum = s_c[a];
+            for (uint32_t b = a + 1; b < wpb; b++)
                 if (s_g[b] == s_g[a]) {
                     sum += s_c[b];
                     s_c[b] = 0;
@@ -1029,10 +1351,10 @@ hipError_t allow_small_lds(uint32_t lds_bytes) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
-size_t count_lds_bytes(uint32_t max_spread) {
+size_t count_lds_bytes(uint32_t max_spread, int wpb) {
     const uint32_t sc = max_spread < kChunkMax ? max_spread : kChunkMax;
     const uint32_t pad = sc | 1u;
-    return (size_t)kWavesPerBlock * kWave * pad * sizeof(uint2);
+    return (size_t)wpb * kWave * pad * sizeof(uint2);
 }
 
 static uint32_t count_pad(uint32_t max_spread) {
@@ -1040,49 +1362,39 @@ static uint32_t count_pad(uint32_t max_spread) {
     return sc | 1u;
 }
 
-// Calls f.template run<W, V>() for probe_ways W and probe window V.
+// Calls f.template run<W>() for probe_ways W.
 template <class F>
-static hipError_t with_variant(int ways, int win, F &f) {
-    switch (win) {
-        case 8:
-            switch (ways) {
-                case 1: return f.template run<1, 8>();
-                case 2: return f.template run<2, 8>();
-                case 8: return f.template run<8, 8>();
-                default: return f.template run<4, 8>();
-            }
-        case 16:
-            switch (ways) {
-                case 1: return f.template run<1, 16>();
-                case 2: return f.template run<2, 16>();
-                case 8: return f.template run<8, 16>();
-                default: return f.template run<4, 16>();
-            }
-        default:
-            switch (ways) {
-                case 1: return f.template run<1, 0>();
-                case 2: return f.template run<2, 0>();
-                case 8: return f.template run<8, 0>();
-                default: return f.template run<4, 0>();
-            }
+static hipError_t with_ways(int ways, F &f) {
+    switch (ways) {
+        case 1: return f.template run<1>();
+        case 2: return f.template run<2>();
+        case 8: return f.template run<8>();
+        default: return f.template run<4>();
     }
 }
 
 struct OccupancyQuery {
-    int ob;
+    int ob, wpb, kernel;
     size_t shm;
     int *per_cu;
-    template <int W, int V>
+    template <int W>
     hipError_t run() {
-        return ob == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint64_t, W, V>, kBlock, shm)
-                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint32_t, W, V>, kBlock, shm);
+        const int bs = wpb * kWave;
+        if (kernel == kKernelWalk)
+            return ob == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_walk<uint64_t, W>, bs, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_walk<uint32_t, W>, bs, 0);
+        return ob == 8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint64_t, W>, bs, shm)
+                       : hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_count<uint32_t, W>, bs, shm);
     }
 };
 
-hipError_t count_grid(int ob, uint32_t max_spread, int ways, int win, int num_cus, uint32_t *grid) {
+bool valid_count_waves(int wpb) { return wpb == 1 || wpb == 2 || wpb == 4 || wpb == 8; }
+
+hipError_t count_grid(int kernel, int ob, uint32_t max_spread, int ways, int wpb, int num_cus, uint32_t *grid) {
+    if (!valid_count_waves(wpb)) return hipErrorInvalidValue;
     int per_cu = 0;
-    OccupancyQuery q{ob, count_lds_bytes(max_spread), &per_cu};
-    hipError_t e = with_variant(ways, win, q);
+    OccupancyQuery q{ob, wpb, kernel, count_lds_bytes(max_spread, wpb), &per_cu};
+    hipError_t e = with_ways(ways, q);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     uint64_t g = (uint64_t)per_cu * (uint64_t)(num_cus > 0 ? num_cus : 1);
@@ -1095,7 +1407,8 @@ hipError_t count_grid(int ob, uint32_t max_spread, int ways, int win, int num_cu
 struct CountLaunch {
     const GraphDesc *desc;
     uint32_t G;
-    int ob;
+    int ob, wpb, kernel;
+    uint32_t work;
     const uint64_t *tree;
     uint64_t *meta;
     unsigned long long *counts;
@@ -1103,23 +1416,30 @@ struct CountLaunch {
     size_t shm;
     uint32_t grid;
     hipStream_t st;
-    template <int W, int V>
+    template <int W>
     hipError_t run() {
-        if (ob == 8)
-            k_count<uint64_t, W, V><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
-        else
-            k_count<uint32_t, W, V><<<dim3(grid), dim3(kBlock), shm, st>>>(desc, G, tree, meta, counts, pad, chunk);
+        const dim3 blk(wpb * kWave);
+        if (kernel == kKernelWalk) {
+            if (ob == 8) k_walk<uint64_t, W><<<dim3(grid), blk, 0, st>>>(desc, G, tree, meta, counts, work);
+            else k_walk<uint32_t, W><<<dim3(grid), blk, 0, st>>>(desc, G, tree, meta, counts, work);
+        } else if (ob == 8) {
+            k_count<uint64_t, W><<<dim3(grid), blk, shm, st>>>(desc, G, tree, meta, counts, pad, chunk, work);
+        } else {
+            k_count<uint32_t, W><<<dim3(grid), blk, shm, st>>>(desc, G, tree, meta, counts, pad, chunk, work);
+        }
         return hipGetLastError();
     }
 };
 
-hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
-                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, int win,
-                        uint32_t grid, hipStream_t st) {
-    if (chunk < 1 || chunk > kChunkMax) return hipErrorInvalidValue;
+hipError_t launch_count(int kernel, const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
+                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, int wpb,
+                        uint32_t work, uint32_t grid, hipStream_t st) {
+    if (chunk < 1 || chunk > kChunkMax || !valid_count_waves(wpb)) return hipErrorInvalidValue;
+    if (kernel != kKernelWalk && kernel != kKernelStaged) return hipErrorInvalidValue;
     const uint32_t eff = max_spread < chunk ? max_spread : chunk;
-    CountLaunch l{desc, G, ob, tree, meta, counts, count_pad(eff), chunk, count_lds_bytes(eff), grid, st};
-    return with_variant(ways, win, l);
+    CountLaunch l{desc, G, ob, wpb, kernel, work, tree, meta, counts, count_pad(eff), chunk,
+                  count_lds_bytes(eff, wpb), grid, st};
+    return with_ways(ways, l);
 }
 
 }  // namespace wtc

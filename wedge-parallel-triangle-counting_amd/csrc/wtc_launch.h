@@ -31,15 +31,28 @@ hipError_t launch_count_small(const GraphDesc *desc, const SmallPart *parts, uin
                               unsigned long long *counts, uint32_t lds_bytes, hipStream_t st);
 // Raises the small-graph kernel's dynamic-LDS limit to lds_bytes.
 hipError_t allow_small_lds(uint32_t lds_bytes);
-// LDS bytes of a counting block that stages `staged` wedges per lane.
-size_t count_lds_bytes(uint32_t staged);
-// `ways`: independent probes each lane advances together (1, 2, 4 or 8);
-// `win`: probe window in elements (0 = plain binary search, 8 or 16);
-// `staged` = min(max spread, chunk).
-hipError_t count_grid(int ob, uint32_t staged, int ways, int win, int num_cus, uint32_t *grid);
-// `chunk` (1..kChunkMax): wedges per lane staged in LDS per phase.
-hipError_t launch_count(const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
-                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, int win,
-                        uint32_t grid, hipStream_t st);
+// LDS bytes of a counting block of wpb waves that stages `staged` wedges
+// per lane.
+size_t count_lds_bytes(uint32_t staged, int wpb);
+// Waves per counting block the kernel accepts: 1, 2, 4 or 8.
+bool valid_count_waves(int wpb);
+// Counting kernels: kKernelWalk (k_walk: waves walk the wedge order in steps
+// of 64 consecutive wedges) and kKernelStaged (k_count: per-lane spreads
+// staged through LDS).  Both count the same wedges of the same tiles.
+enum : int { kKernelWalk = 0, kKernelStaged = 1 };
+// `ways`: independent probes each lane advances together (1, 2, 4 or 8; for
+// k_walk, the steps in flight); `wpb`: waves per block; `staged` = min(max
+// spread, chunk) (k_count's LDS staging).
+hipError_t count_grid(int kernel, int ob, uint32_t staged, int ways, int wpb, int num_cus, uint32_t *grid);
+// `chunk` (1..kChunkMax): wedges per lane staged in LDS per phase (k_count).
+// `work`: bits 0-15 the most tiles per grab (0: 32), bit 16 set to deal
+// super-chunks of grabs to the 8 XCDs in turn (else one stream of grabs).
+hipError_t launch_count(int kernel, const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
+                        unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, int wpb,
+                        uint32_t work, uint32_t grid, hipStream_t st);
+// The `work` word of a grab size and XCD dealing.
+inline uint32_t count_work(uint32_t tiles_per_grab, bool deal) {
+    return (tiles_per_grab & 0xffffu) | (deal ? 1u << 16 : 0u);
+}
 
 }  // namespace wtc

@@ -341,9 +341,9 @@ int wtc_server_create(int device, const wtc_server_config *cfg, wtc_server **out
     s->cfg.queue_capacity = cfg && cfg->queue_capacity ? cfg->queue_capacity : 65536;
     s->cfg.probe_ways = cfg && cfg->probe_ways ? cfg->probe_ways : 4;
     s->cfg.lds_chunk = cfg && cfg->lds_chunk ? cfg->lds_chunk : 16;
-    s->cfg.probe_window = cfg && cfg->probe_window ? cfg->probe_window : -1;
+    s->cfg.waves_per_block = cfg && cfg->waves_per_block ? cfg->waves_per_block : 4;
     s->cfg.small_graph_words = cfg ? cfg->small_graph_words : 0;
-    if (s->cfg.probe_window != -1 && s->cfg.probe_window != 8 && s->cfg.probe_window != 16) {
+    if (!valid_count_waves(s->cfg.waves_per_block)) {
         delete s;
         return WTC_ERR_ARG;
     }
@@ -356,7 +356,7 @@ int wtc_server_create(int device, const wtc_server_config *cfg, wtc_server **out
     s->tune.num_cus = prop.multiProcessorCount;
     s->tune.ways = s->cfg.probe_ways;
     s->tune.chunk = s->cfg.lds_chunk;
-    s->tune.win = s->cfg.probe_window > 0 ? s->cfg.probe_window : 0;
+    s->tune.wpb = s->cfg.waves_per_block;
     int rc = init_small_graphs(s->tune, prop, s->cfg.small_graph_words < 0    ? 0u
                                               : s->cfg.small_graph_words == 0 ? default_small_words()
                                                                               : (uint32_t)s->cfg.small_graph_words);

@@ -72,7 +72,7 @@ class CountStats(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("slots", C.c_uint32),
                 ("queue_capacity", C.c_uint32), ("probe_ways", C.c_int32), ("lds_chunk", C.c_uint32),
-                ("probe_window", C.c_int32), ("small_graph_words", C.c_int32)]
+                ("waves_per_block", C.c_int32), ("small_graph_words", C.c_int32)]
 
 
 class RequestStats(C.Structure):
@@ -149,6 +149,8 @@ _sig = {
     "wtc_debug_copy": (C.c_int, [_P, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "wtc_ctx_set_tuning": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_int]),
     "wtc_ctx_set_small_graphs": (C.c_int, [_P, C.c_uint32]),
+    "wtc_ctx_set_count_kernel": (C.c_int, [_P, C.c_int]),
+    "wtc_ctx_set_work_split": (C.c_int, [_P, C.c_uint32, C.c_int]),
     "wtc_ctx_get_small_graphs": (C.c_int, [_P, _P, _P]),
     "wtc_alg_bytes": (None, [_BGP, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "wtc_batch_upload": (C.c_int, [_P, C.POINTER(_BGP), C.c_size_t, C.POINTER(_P)]),
@@ -381,6 +383,7 @@ def tc_arrow_gpu_big(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -
 
 WEDGE_OUTGOING, WEDGE_ARROW, WEDGE_MIXED = 0, 1, 2
 PARALLEL_VERTEX, PARALLEL_EDGE = 0, 1
+KERNEL_WALK, KERNEL_STAGED = 0, 1
 
 
 def _baseline_call(fn32, fn64, g: CSR, a: int, style: int, t: Optional[GpuTime], big: bool) -> int:
@@ -446,10 +449,18 @@ class Context:
         _check(_lib.wtc_ctx_debug_prefix(self._p, n, ws.ctypes.data, tree.ctypes.data), "debug_prefix")
         return ws, tree
 
-    def set_tuning(self, probe_ways: int = 4, lds_chunk: int = 16, probe_window: int = 0):
+    def set_tuning(self, probe_ways: int = 4, lds_chunk: int = 16, waves_per_block: int = 4):
         """Counting-kernel knobs: probes advanced together per lane, wedges
-        per lane staged in LDS per phase, probe window (0, 8 or 16)."""
-        _check(_lib.wtc_ctx_set_tuning(self._p, probe_ways, lds_chunk, probe_window), "set_tuning")
+        per lane staged in LDS per phase, waves per block (1, 2, 4 or 8)."""
+        _check(_lib.wtc_ctx_set_tuning(self._p, probe_ways, lds_chunk, waves_per_block), "set_tuning")
+
+    def set_count_kernel(self, kernel: int):
+        """KERNEL_WALK (0) or KERNEL_STAGED (1); counts do not depend on it."""
+        _check(_lib.wtc_ctx_set_count_kernel(self._p, kernel), "set_count_kernel")
+
+    def set_work_split(self, tiles_per_grab: int = 0, deal_to_xcds: bool = True):
+        """Most tiles per grab (0: 32) and XCD dealing of the counting kernels."""
+        _check(_lib.wtc_ctx_set_work_split(self._p, tiles_per_grab, int(deal_to_xcds)), "set_work_split")
 
     def set_small_graphs(self, max_words: int):
         """In counts over two or more graphs, graphs with 2(n+1) + m <=
@@ -620,9 +631,9 @@ class Server:
     """Request loop with continuous batching over device-resident CSRs."""
 
     def __init__(self, device: int = 0, hbm_budget_bytes: int = 8 << 30, max_batch: int = 256, slots: int = 2,
-                 queue_capacity: int = 65536, probe_ways: int = 0, lds_chunk: int = 0, probe_window: int = 0,
+                 queue_capacity: int = 65536, probe_ways: int = 0, lds_chunk: int = 0, waves_per_block: int = 0,
                  small_graph_words: int = 0):
-        cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, queue_capacity, probe_ways, lds_chunk, probe_window,
+        cfg = ServerConfig(hbm_budget_bytes, max_batch, slots, queue_capacity, probe_ways, lds_chunk, waves_per_block,
                            small_graph_words)
         self._p = C.c_void_p()
         _check(_lib.wtc_server_create(device, C.byref(cfg), C.byref(self._p)), "server_create")
