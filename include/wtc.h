@@ -287,6 +287,25 @@ int wtc_dgraph_count_arrow(wtc_ctx *ctx, wtc_dgraph *dg, uint32_t spread, uint32
 int wtc_dgraph_count_baseline(wtc_ctx *ctx, wtc_dgraph *dg, int parallelism, int style,
                               uint32_t adjacency_matrix_len, uint64_t *triangles, wtc_count_stats *st);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The count of wtc_dgraph_count over a caller-owned
+ * oriented CSR already in device memory (d_row_ptr: n+1 offsets of
+ * offset_bytes 4 or 8; d_col_ind: m ids), with no device allocation: every
+ * workspace buffer comes from one fixed array of 256 MiB per device that the
+ * library holds until the process ends.  The workspace 8(n+1) +
+ * 8*ceil(n/4096) + a*a/8 bytes (plus ~34 KB) must fit it, else WTC_ERR_RANGE
+ * before anything is enqueued (R-MAT-24 fits with a <= 16384).
+ * Kernels only are enqueued on `stream` (a hipStream_t; NULL = the null
+ * stream of the current device); *d_triangles (8 bytes of device memory)
+ * holds the count once the stream has reached that point.  The CSR is
+ * checked on the device first: one that is not an oriented CSR is not
+ * counted and *d_triangles becomes UINT64_MAX.  Calls that overlap on one
+ * GPU share its array and race: serialise them (one stream, or a lock). */
+int wtc_count_noalloc(const void *d_row_ptr, uint32_t offset_bytes, const uint32_t *d_col_ind, uint32_t n,
+                      uint64_t m, uint32_t spread, uint32_t adjacency_matrix_len, uint64_t w_begin, uint64_t w_end,
+                      uint64_t *d_triangles, void *stream);
+
 /* Diagnostics for parity tests: the device wedge prefix (n+1 entries, the
  * h_wedgeSum of tc.cu:348-361), the 4095-entry search tree (the
  * c_binary_search_cache of tc.cu:364-368, filled for single-graph counts) and

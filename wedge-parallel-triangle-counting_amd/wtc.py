@@ -143,6 +143,8 @@ _sig = {
                                         C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
     "wtc_dgraph_count_baseline": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_uint64),
                                            C.POINTER(CountStats)]),
+    "wtc_count_noalloc": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
+                                   C.c_uint64, C.c_uint64, _P, _P]),
     "wtc_ctx_debug_prefix": (C.c_int, [_P, C.c_uint64, _P, _P]),
     "wtc_ctx_debug_bitmap": (C.c_int, [_P, C.c_uint64, _P]),
     "wtc_debug_decode": (C.c_int, [_P, _P, _P, C.c_uint64, _P, _P]),
