@@ -223,9 +223,20 @@ int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int wav
  *   WTC_KERNEL_STAGED each lane decodes its spread of s wedges and stages
  *                     them through LDS, as the reference does
  *                     (tc.cu:174-244); lds_chunk applies.
+ *   WTC_KERNEL_SPLIT  a single graph with a > 0: the wedges whose middle
+ *                     vertex w is among the top-a ranks (the reference's
+ *                     bitmap probes, tc.cu:262-272) are counted grouped by
+ *                     w against adj+(w) held as a bitmap row in LDS (the
+ *                     in-edges of the top rows are grouped by a counting
+ *                     sort per count; no a*a/8-byte bitmap in HBM, an extra
+ *                     8 or 16 bytes per edge of workspace instead); the rest
+ *                     (sorted-list probes, tc.cu:273-286) by the row walk.
+ *                     Batches, a = 0 and colInd not 16-byte aligned: the
+ *                     row walk.  Same count for every wedge range.
  * Default WTC_KERNEL_WALK or $WTC_COUNT_KERNEL. */
 #define WTC_KERNEL_WALK 0
 #define WTC_KERNEL_STAGED 1
+#define WTC_KERNEL_SPLIT 2
 int wtc_ctx_set_count_kernel(wtc_ctx *ctx, int kernel);
 /* How the counting kernels hand out work: the wedge order is cut into tiles
  * of 64*s wedges, and a wave takes up to tiles_per_grab consecutive tiles
@@ -398,13 +409,31 @@ int wtc_server_get_stats(wtc_server *srv, wtc_server_stats *st);
 /* Finishes queued and running requests, then frees everything. */
 void wtc_server_destroy(wtc_server *srv);
 
-/* The autotuner's (spread, a) for a graph of n vertices, m oriented edges and
- * W wedges, from per-size-bucket tables measured on MI355X: wtc_autotune for
- * a graph counted alone, wtc_autotune_batch for a member of a batch (where
- * many graphs share one launch).  A bitmap (a > 0) is chosen only for
- * degree-skewed graphs (W/m >= 24). */
+/* The autotuner: choices for a graph of n vertices, m oriented edges and W
+ * wedges from a per-size-bucket table measured on MI355X (the sweep
+ * `bench.py --tune` writes tuning/mi355x.tsv beside the library;
+ * $WTC_TUNING_FILE names another; built-in rows otherwise).  Buckets are
+ * floor(log2 W) by family (counted alone, or as a member of a batch, where
+ * many graphs share one launch) and degree skew (W/m >= 24).  No counterpart
+ * in the reference, which leaves s and a to the user (tc.cu:828-881). */
+typedef struct wtc_tuned {
+    uint32_t spread;               /* s */
+    uint32_t adjacency_matrix_len; /* a (0: no bitmap) */
+    int32_t kernel;                /* WTC_KERNEL_* */
+    int32_t waves_per_block;       /* counting block: 1, 2, 4 or 8 waves */
+    int32_t probe_ways;            /* probes (walk: steps) in flight per lane */
+    int32_t split_waves;           /* split count's row kernel: 4, 8 or 16 waves */
+} wtc_tuned;
+int wtc_autotune_shape(uint32_t n, uint64_t m, uint64_t wedges, int batch, wtc_tuned *out);
+/* The table's (spread, a) alone: wtc_autotune for a graph counted alone,
+ * wtc_autotune_batch for a member of a batch. */
 void wtc_autotune(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *adjacency_matrix_len);
 void wtc_autotune_batch(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *adjacency_matrix_len);
+/* Path of the table in use, or "built-in". */
+const char *wtc_tuning_source(void);
+/* Applies a tuned shape's kernel, block waves, probe ways and row-kernel
+ * waves to a context (spread and a stay per count). */
+int wtc_ctx_use_tuned(wtc_ctx *ctx, const wtc_tuned *t);
 
 #ifdef __cplusplus
 }

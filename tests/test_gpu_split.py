@@ -1,0 +1,113 @@
+"""GPU parity of the split count (KERNEL_SPLIT): the top-a wedges counted by
+middle vertex against an LDS bitmap row after a per-count transposition of
+the top rows' in-edges, the list-probe wedges by the row walk over their own
+prefix.  Every count, whole graph or wedge range, must equal the oracle's
+count of the same wedges (tc.cu:173-293 over [w_begin, w_end))."""
+import numpy as np
+import pytest
+
+import oracle as O
+import wtc
+
+pytestmark = pytest.mark.gpu
+
+
+def _rmat(scale, ef=16, seed=1):
+    return wtc.preprocess(wtc.graph_from_pairs(wtc.gen_rmat_pairs(scale, ef, seed=seed)))
+
+
+def _complete(k):
+    return wtc.preprocess(wtc.graph_from_pairs(
+        np.array([[i, j] for i in range(k) for j in range(i + 1, k)], np.uint32)))
+
+
+@pytest.fixture
+def split_ctx(gpu_ctx):
+    gpu_ctx.set_count_kernel(wtc.KERNEL_SPLIT)
+    try:
+        yield gpu_ctx
+    finally:
+        gpu_ctx.set_count_kernel(wtc.KERNEL_WALK)
+
+
+def _ranges(W, rng, k=6):
+    """Whole order, halves and thirds, an empty range, a one-wedge range,
+    ranges past the end and k random ranges."""
+    out = [(0, 2 ** 64 - 1), (0, W), (0, W // 2), (W // 2, W), (W // 3, 2 * W // 3), (W // 7, W // 7),
+           (W // 5, W // 5 + 1), (W - 1, W + 100), (W, 2 ** 64 - 1)]
+    for _ in range(k):
+        x, y = sorted(int(t) for t in rng.integers(0, W + 1, size=2))
+        out.append((x, y))
+    return out
+
+
+@pytest.mark.parametrize("ob", [4, 8])
+def test_split_whole_and_ranges_match_oracle(split_ctx, ob):
+    """R-MAT, Barabasi-Albert and Erdos-Renyi graphs at several bitmap
+    lengths (a below n, a at n, a above n so that every vertex is a top
+    row), whole graph and wedge ranges straddling vertices, against the
+    oracle's range count."""
+    rng = np.random.default_rng(5)
+    cases = [(_rmat(12, seed=3), (32, 1024, 4096)),
+             (_rmat(10, ef=32, seed=4), (64, 1024)),
+             (wtc.preprocess(wtc.gen_barabasi_albert(3000, 7, seed=9)), (32, 2048, 3008)),
+             (wtc.preprocess(wtc.gen_erdos_renyi(700, 0.05, seed=2)), (96, 704, 4096))]
+    for g, alens in cases:
+        W = g.wedges()
+        expect = O.tc_merge(g)
+        dg = split_ctx.upload(g, offset_bytes=ob)
+        for a in alens:
+            for s in (1, 4, 12):
+                assert dg.count(s, a) == expect, (g.n, a, s)
+            for lo, hi in _ranges(W, rng):
+                got = dg.count(8, a, lo, hi)
+                assert got == O.tc_wedge(g, 8, a, lo, hi), (g.n, a, lo, hi)
+        dg.free()
+
+
+def test_split_complete_graph_closed_form(split_ctx):
+    """K_400: every wedge closes, so every range [lo, hi) holds hi - lo
+    triangles (clipped tails at both ends included)."""
+    g = _complete(400)
+    W = g.wedges()
+    assert W == 400 * 399 * 398 // 6
+    dg = split_ctx.upload(g)
+    rng = np.random.default_rng(11)
+    for a in (32, 128, 416):
+        assert dg.count(5, a) == W
+        for lo, hi in _ranges(W, rng, k=10):
+            lo_c, hi_c = min(lo, W), min(hi, W)
+            assert dg.count(5, a, lo, hi) == max(0, hi_c - lo_c), (a, lo, hi)
+    dg.free()
+
+
+def test_split_sum_of_parts(split_ctx):
+    """A five-way range split of an R-MAT-14 graph sums to the whole count
+    at u32 and u64 offsets, and both equal the walk's count."""
+    g = _rmat(14, seed=8)
+    expect = O.tc_merge(g)
+    W = g.wedges()
+    cuts = [0, W // 9, W // 3, W // 2, (5 * W) // 6, W]
+    for ob in (4, 8):
+        dg = split_ctx.upload(g, offset_bytes=ob)
+        for a in (512, 2048):
+            assert dg.count(12, a) == expect
+            parts = [dg.count(12, a, cuts[i], cuts[i + 1]) for i in range(5)]
+            assert sum(parts) == expect, (ob, a, parts)
+        dg.free()
+
+
+def test_split_stats_report_the_full_order(split_ctx):
+    """Stats of a split count report the wedges of the counted range of the
+    full order (not of the list part alone)."""
+    g = _rmat(13, seed=12)
+    expect = O.tc_merge(g)
+    W = g.wedges()
+    dg = split_ctx.upload(g)
+    cnt, st = dg.count(6, 2048, stats=True)
+    assert cnt == expect
+    assert st.wedges == W
+    cnt, st = dg.count(6, 2048, W // 4, W // 2, stats=True)
+    assert cnt == O.tc_wedge(g, 6, 2048, W // 4, W // 2)
+    assert st.wedges == W // 2 - W // 4
+    dg.free()

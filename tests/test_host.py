@@ -209,23 +209,16 @@ def test_tc_gpu_drop_ins_reject_bad_csr_on_host():
 
 
 def test_autotune_table():
-    # the measured buckets: alone, R-MAT-like skew (W/m >= 24) gets a bitmap
-    # of about n/32 top ranks once W >= 2^26, capped at 32768 / 131072 /
-    # 262144 / 393216 as n grows (R-MAT 20-24, 26, 27-28)
-    assert wtc.autotune(4083, 48254, 780919) == (2, 0)
-    assert wtc.autotune(65482, 909788, 33312005) == (8, 2048)
-    assert wtc.autotune(262089, 3805718, 204948349) == (16, 8192)
-    assert wtc.autotune(1048465, 15701962, 1226291937) == (16, 32768)
-    assert wtc.autotune(4193857, 64156969, 7204823400) == (12, 32768)
-    assert wtc.autotune(16775427, 260385033, 41737375097) == (12, 131072)
-    assert wtc.autotune(67104851, 1051918711, 239067018554) == (12, 262144)
-    assert wtc.autotune(268428511, 4236159419, 1357709474934) == (12, 393216)
-    # low skew (ER/BA): no bitmap; batch members use larger tiles
-    assert wtc.autotune(40000, 600000, 4000000)[1] == 0
-    assert wtc.autotune(100, 300, 900, batch=True) == (4, 0)
-    assert wtc.autotune(200, 600, 1800, batch=True) == (8, 0)
-    assert wtc.autotune(30000, 500000, 5000000, batch=True) == (12, 0)
+    # whatever table is in force (the committed sweep's or the built-in
+    # rows): no bitmap for graphs without wedges or of low skew, a bitmap
+    # length that is a multiple of 32 and no larger than n rounded up, and a
+    # spread of at least 1
     assert wtc.autotune(10, 0, 0) == (1, 0)
+    assert wtc.autotune(40000, 600000, 4000000)[1] == 0
+    for n, m, W in ((4083, 48254, 780919), (1048465, 15701962, 1226291937),
+                    (16775427, 260385033, 41737375097), (268428511, 4236159419, 1357709474934)):
+        s, a = wtc.autotune(n, m, W)
+        assert s >= 1 and a % 32 == 0 and a <= ((n + 31) // 32) * 32
     for n in (64, 1000, 5000, 65536, 10 ** 6):
         for batch in (False, True):
             s, a = wtc.autotune(n, 20 * n, 600 * n, batch=batch)

@@ -93,6 +93,12 @@ void Slot::release_device() {
     bm.release();
     desc.release();
     out.release();
+    sp_kpos.release();
+    sp_wsfull.release();
+    sp_hist.release();
+    sp_tot.release();
+    sp_rpin.release();
+    sp_ie.release();
 }
 
 void Slot::destroy() {
@@ -177,24 +183,37 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
         small_parts += small_parts_of(W[g]);
         small_words = std::max(small_words, (uint32_t)words);
     }
+    // The split count takes a single graph with a bitmap length whose colInd
+    // is 16-byte aligned (its row kernel loads four ids at a time); its top
+    // rows then need no bitmap in HBM.
+    const bool split = t.kernel == kKernelSplit && G == 1 && hd[0].mode == kModeTiles && hd[0].a > 0 &&
+                       hd[0].bm_rows > 0 && m != nullptr && (reinterpret_cast<uintptr_t>(hd[0].colInd) & 15u) == 0;
+    const uint32_t split_rows = split ? hd[0].bm_rows : 0;
+    if (split) {
+        hd[0].bm_rows = 0;
+        hd[0].bm_words = 0;
+        hd[0].bitmap = nullptr;
+    }
     uint32_t max_spread = 1;
     uint64_t ws_n = 0, bm_words = 0, rows = 0, blocks = 0;
     for (const auto &d : hd) {
         if (d.n == 0 || d.spread == 0) return WTC_ERR_ARG;
         if (d.mode == kModeTiles) max_spread = std::max(max_spread, d.spread);
         if (!d.ws) ws_n += (uint64_t)d.n + 1;
-        if (d.a && !d.bitmap) bm_words += bm_words_for(d.a);
+        if (d.bm_rows && !d.bitmap) bm_words += bm_words_for(d.a);
         rows += d.bm_rows;
         blocks += prefix_blocks(d.n);
     }
     if (blocks > 0x7fffffffULL) return WTC_ERR_RANGE;
     const bool single = G == 1;
     if (ws_n) WTC_HIPC(s.ws.ensure(ws_n * sizeof(uint64_t)));
-    WTC_HIPC(s.partials.ensure(blocks * sizeof(uint64_t)));
+    WTC_HIPC(s.partials.ensure((split ? 3 : 1) * blocks * sizeof(uint64_t)));
     if (single) WTC_HIPC(s.tree.ensure(kTreeSize * sizeof(uint64_t)));
     // descriptors, then the block list of the small-graph kernel
     if (small_parts > 0x7fffffffULL) return WTC_ERR_RANGE;
-    const size_t desc_bytes = (size_t)G * sizeof(GraphDesc) + (size_t)small_parts * sizeof(SmallPart);
+    const size_t list_bytes = (size_t)G * sizeof(GraphDesc) + (size_t)small_parts * sizeof(SmallPart);
+    const size_t split_off = (list_bytes + 255) & ~(size_t)255;
+    const size_t desc_bytes = split ? split_off + sizeof(SplitDesc) : list_bytes;
     WTC_HIPC(s.desc.ensure(desc_bytes));
     WTC_HIPC(s.out.ensure((size_t)(kMetaSlots + G) * sizeof(uint64_t)));
     if (bm_words) WTC_HIPC(s.bm.ensure(bm_words * sizeof(uint32_t)));
@@ -203,12 +222,12 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     uint64_t ws_off = 0, bm_off = 0, row_off = 0, blk_off = 0, clear_off = 0;
     for (auto &d : hd) {
         d.bm_clear_base = clear_off;
-        clear_off += d.a ? bm_words_for(d.a) : 0;
+        clear_off += d.bm_rows ? bm_words_for(d.a) : 0;
         if (!d.ws) {
             d.ws = s.ws.as<uint64_t>() + ws_off;
             ws_off += (uint64_t)d.n + 1;
         }
-        if (d.a && !d.bitmap) {
+        if (d.bm_rows && !d.bitmap) {
             d.bitmap = s.bm.as<uint32_t>() + bm_off;
             bm_off += bm_words_for(d.a);
         }
@@ -217,7 +236,30 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
         row_off += d.bm_rows;
         blk_off += prefix_blocks(d.n);
     }
+    SplitDesc sd{};
+    if (split) {
+        const uint32_t n = hd[0].n;
+        sd.rows = split_rows;
+        sd.chunks = std::min<uint32_t>(kSplitChunksMax, std::max<uint32_t>(1, n / kSplitChunkVerts));
+        sd.parts = (split_rows + kSplitPart - 1) / kSplitPart;
+        sd.cap = std::max<uint64_t>(m[0], 1);  // an in-edge is an edge
+        WTC_HIPC(s.sp_kpos.ensure((size_t)n * sizeof(uint32_t)));
+        WTC_HIPC(s.sp_wsfull.ensure(2 * ((size_t)n + 1) * sizeof(uint64_t)));
+        WTC_HIPC(s.sp_hist.ensure((size_t)sd.chunks * split_rows * sizeof(uint32_t)));
+        WTC_HIPC(s.sp_tot.ensure((size_t)split_rows * sizeof(uint32_t)));
+        WTC_HIPC(s.sp_rpin.ensure(((size_t)split_rows + 1) * (size_t)ob));
+        WTC_HIPC(s.sp_ie.ensure((size_t)sd.cap * 2 * (size_t)ob));
+        sd.kpos = s.sp_kpos.as<uint32_t>();
+        sd.ws_full = s.sp_wsfull.as<uint64_t>();
+        sd.ie_pre = sd.ws_full + (size_t)n + 1;
+        sd.hist = s.sp_hist.as<uint32_t>();
+        sd.row_tot = s.sp_tot.as<uint32_t>();
+        sd.rp_in = s.sp_rpin.p;
+        sd.ie = s.sp_ie.p;
+        WTC_HIPC(split_rows_grid(ob, split_rows, t.split_wpb, t.num_cus, &s.rows_grid));
+    }
     std::memcpy(s.h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
+    if (split) std::memcpy(s.h_desc.as<char>() + split_off, &sd, sizeof(SplitDesc));
     SmallPart *h_parts = reinterpret_cast<SmallPart *>(s.h_desc.as<GraphDesc>() + G);
     for (uint32_t g = 0, k = 0; g < G; g++) {
         if (hd[g].mode != kModeSmall) continue;
@@ -227,8 +269,10 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
 
     const int oi = ob == 8 ? 1 : 0;
     const uint32_t staged = std::min<uint32_t>(max_spread, t.chunk);
-    uint32_t &gc = t.grid_cache[t.kernel == kKernelWalk ? 0 : 1][oi][ways_index(t.ways)][wpb_index(t.wpb)][staged];
-    if (!gc) WTC_HIPC(count_grid(t.kernel, ob, staged, t.ways, t.wpb, t.num_cus, &gc));
+    // the split count's list part, and a set the split count does not take, use the row walk
+    const int kernel = t.kernel == kKernelStaged ? kKernelStaged : kKernelWalk;
+    uint32_t &gc = t.grid_cache[kernel == kKernelWalk ? 0 : 1][oi][ways_index(t.ways)][wpb_index(t.wpb)][staged];
+    if (!gc) WTC_HIPC(count_grid(kernel, ob, staged, t.ways, t.wpb, t.num_cus, &gc));
     s.G = G;
     s.grid = gc;
     s.small = small_n;
@@ -241,13 +285,19 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->single = single;
     plan->ways = t.ways;
     plan->wpb = t.wpb;
-    plan->kernel = t.kernel;
+    plan->kernel = kernel;
     plan->work = count_work(t.grab, t.deal);
     plan->chunk = t.chunk;
     plan->desc_bytes = desc_bytes;
     plan->small_count = (uint32_t)small_parts;
     plan->tile_graphs = G - small_n;
     plan->small_lds = std::min((small_words * 4u + 255u) & ~255u, t.small_words_max * 4u);
+    plan->split = split;
+    plan->split_off = split_off;
+    plan->split_rows = split_rows;
+    plan->split_chunks = sd.chunks;
+    plan->split_parts = sd.parts;
+    plan->split_wpb = t.split_wpb;
     return WTC_OK;
 }
 
@@ -256,6 +306,25 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
     unsigned long long *dcounts = reinterpret_cast<unsigned long long *>(meta + kMetaSlots);
     GraphDesc *ddesc = s.desc.as<GraphDesc>();
     uint64_t *tree = p.single ? s.tree.as<uint64_t>() : nullptr;
+    if (p.split) {
+        SplitDesc *sd = reinterpret_cast<SplitDesc *>(s.desc.as<char>() + p.split_off);
+        if (timed) WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
+        WTC_HIPC(launch_split_prefix(ddesc, sd, p.ob, p.blocks, s.partials.as<uint64_t>(), s.stream));
+        WTC_HIPC(launch_split_range(ddesc, sd, s.stream));
+        WTC_HIPC(launch_setup(ddesc, 1, tree, meta, dcounts, s.stream));
+        if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
+        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_parts, p.split_rows, meta,
+                                        s.stream));
+        if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
+        WTC_HIPC(launch_split_rows(ddesc, sd, p.ob, p.split_rows, p.split_wpb, meta, dcounts, s.rows_grid,
+                                   s.stream));
+        WTC_HIPC(launch_count(p.kernel, ddesc, 1, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways,
+                              p.wpb, p.work, s.grid, s.stream));
+        if (timed) WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
+        WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + p.G) * sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, s.stream));
+        return WTC_OK;
+    }
     if (timed) WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
     WTC_HIPC(launch_prefix(ddesc, p.G, p.ob, p.blocks, s.partials.as<uint64_t>(), s.stream));
     WTC_HIPC(launch_setup(ddesc, p.G, tree, meta, dcounts, s.stream));
@@ -336,10 +405,11 @@ int default_count_waves() {
 }
 
 // Default counting kernel for new contexts: WTC_COUNT_KERNEL (0 row walk,
-// 1 staged), else the row walk.
+// 1 staged, 2 split), else the row walk.
 int default_count_kernel() {
     const char *e = std::getenv("WTC_COUNT_KERNEL");
-    return e && std::atoi(e) == kKernelStaged ? kKernelStaged : kKernelWalk;
+    const int v = e ? std::atoi(e) : kKernelWalk;
+    return v == kKernelStaged || v == kKernelSplit ? v : kKernelWalk;
 }
 
 int set_device(int dev) {
@@ -449,7 +519,8 @@ int tc_gpu_impl(uint32_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colI
     std::vector<GraphDesc> hd{make_desc(drow.p, dcol.as<uint32_t>(), n, spread, a, 0, UINT64_MAX)};
     uint64_t cnt = 0;
     RunTimes rt;
-    rc = run_set(c, (int)sizeof(OffT), hd, &cnt, nullptr, &rt);
+    const uint64_t m64 = m;
+    rc = run_set(c, (int)sizeof(OffT), hd, &cnt, nullptr, &rt, &m64);
     cleanup();
     if (rc) return rc;
     if (t) {
@@ -534,8 +605,22 @@ int wtc_ctx_set_work_split(wtc_ctx *c, uint32_t tiles_per_grab, int deal_to_xcds
 }
 
 int wtc_ctx_set_count_kernel(wtc_ctx *c, int kernel) {
-    if (!c || (kernel != WTC_KERNEL_WALK && kernel != WTC_KERNEL_STAGED)) return WTC_ERR_ARG;
-    c->tune.kernel = kernel == WTC_KERNEL_WALK ? kKernelWalk : kKernelStaged;
+    if (!c || (kernel != WTC_KERNEL_WALK && kernel != WTC_KERNEL_STAGED && kernel != WTC_KERNEL_SPLIT))
+        return WTC_ERR_ARG;
+    c->tune.kernel = kernel == WTC_KERNEL_WALK ? kKernelWalk : kernel == WTC_KERNEL_STAGED ? kKernelStaged
+                                                                                          : kKernelSplit;
+    return WTC_OK;
+}
+
+int wtc_ctx_use_tuned(wtc_ctx *c, const wtc_tuned *t) {
+    if (!c || !t || !valid_ways(t->probe_ways) || !valid_waves(t->waves_per_block) ||
+        !valid_split_waves(t->split_waves))
+        return WTC_ERR_ARG;
+    int rc = wtc_ctx_set_count_kernel(c, t->kernel);
+    if (rc) return rc;
+    c->tune.ways = t->probe_ways;
+    c->tune.wpb = t->waves_per_block;
+    c->tune.split_wpb = t->split_waves;
     return WTC_OK;
 }
 
@@ -626,7 +711,7 @@ int wtc_dgraph_count(wtc_ctx *c, wtc_dgraph *dg, uint32_t spread, uint32_t a, ui
     if (rc) return rc;
     WTC_HIPC(hipSetDevice(c->device));
     std::vector<GraphDesc> hd{make_desc(dg->rowPtr.p, dg->colInd.as<uint32_t>(), dg->n, spread, a, w_begin, w_end)};
-    return run_set(c, dg->ob, hd, triangles, st);
+    return run_set(c, dg->ob, hd, triangles, st, nullptr, &dg->m);
 }
 
 // Copies the wedge prefix and search tree left by the last single-graph count
@@ -703,59 +788,6 @@ int wtc_debug_copy(wtc_ctx *c, uint64_t bytes, int reps, double *ms) {
 
 }  // extern "C"
 
-namespace {
-
-// Per-size-bucket choices measured on MI355X (device time of prefix, bitmap
-// and count; best of a sweep of s in {1..16} and a in {0, 256 .. 2^20}):
-//  * R-MAT graphs counted alone (scales 12-24): s = 2 / 4 / 8 / 16 / 12 for W
-//    below 2^20 / 2^23 / 2^26 / 2^31 / above; a = 0 below 2^23 wedges, 2048
-//    below 2^26, then about n/32 capped at 32768 (n < 2^22) or 131072.
-//  * Erdos-Renyi / Barabasi-Albert graphs in a batch of 4096 (n 64..65536,
-//    degree 4..64): a = 0 in every size bucket; s = 4 / 8 / 12 for n below
-//    128 / 256 / above.
-// What separates the two families is degree skew, read as wedges per edge:
-// the bitmap over the top ranks only pays where a few vertices carry most of
-// the wedges (R-MAT: 16 at scale 12 rising to 160 at scale 24; ER/BA: below
-// 20 here).
-constexpr uint64_t kSkewWedgesPerEdge = 24;
-
-uint32_t skewed_bitmap_len(uint32_t n) {
-    // caps measured on R-MAT 20-28 at s = 12: 131072 best at scale 24, 262144
-    // at 26 (-10% count time), 393216 at 27 and 28 (-14%, -24%)
-    const uint32_t cap = n < (1u << 22) ? 32768u : n < (1u << 25) ? 131072u : n < (1u << 26) ? 262144u : 393216u;
-    const uint32_t a = ((n / 32) + 31u) & ~31u;
-    return a < cap ? a : cap;
-}
-
-void autotune_params(uint32_t n, uint64_t m, uint64_t wedges, bool in_batch, uint32_t *spread, uint32_t *a) {
-    uint32_t s, aa = 0;
-    const bool skewed = m > 0 && wedges / m >= kSkewWedgesPerEdge;
-    if (wedges == 0) {
-        s = 1;
-    } else if (in_batch) {  // many graphs share the launch: parallelism is not scarce
-        s = n < 128 ? 4 : n < 256 ? 8 : 12;
-        if (skewed && wedges >= (1u << 23)) aa = wedges < (1u << 26) ? 2048u : skewed_bitmap_len(n);
-    } else {  // alone: small graphs need small tiles to fill the GPU
-        s = wedges < (1u << 20) ? 2 : wedges < (1u << 23) ? 4 : wedges < (1u << 26) ? 8 : wedges < (1ull << 31) ? 16 : 12;
-        if (skewed && wedges >= (1u << 23)) aa = wedges < (1u << 26) ? 2048u : skewed_bitmap_len(n);
-    }
-    if (spread) *spread = s;
-    if (a) *a = aa;
-}
-
-}  // namespace
-
-extern "C" {
-
-void wtc_autotune(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *a) {
-    autotune_params(n, m, wedges, false, spread, a);
-}
-
-void wtc_autotune_batch(uint32_t n, uint64_t m, uint64_t wedges, uint32_t *spread, uint32_t *a) {
-    autotune_params(n, m, wedges, true, spread, a);
-}
-
-}  // extern "C"
 
 namespace {
 

@@ -3,6 +3,17 @@
 #pragma once
 #include "wtc_kernels.h"
 
+// Launches KERNEL<uint64_t> or KERNEL<uint32_t> by the set's offset width and
+// returns the launch status.
+#define WTC_LAUNCH_OFF(ob, KERNEL, GRID, BLOCK, SHMEM, STREAM, ...)               \
+    [&]() -> hipError_t {                                                         \
+        if ((ob) == 8)                                                            \
+            KERNEL<uint64_t><<<GRID, BLOCK, SHMEM, STREAM>>>(__VA_ARGS__);        \
+        else                                                                      \
+            KERNEL<uint32_t><<<GRID, BLOCK, SHMEM, STREAM>>>(__VA_ARGS__);        \
+        return hipGetLastError();                                                 \
+    }()
+
 namespace wtc {
 
 __device__ __forceinline__ uint64_t binom2_d(uint64_t x) {
@@ -95,6 +106,39 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t x, uint64_t *s_tot,
     __syncthreads();
     total = tot;
     return off + inc - x;
+}
+
+// Global-memory views of the CSR, prefix and bitmap arrays.  Their pointers
+// are read from descriptors in memory, so without the cast the compiler must
+// treat them as generic addresses: flat loads, which also count against the
+// LDS counter and make every wait cover the LDS traffic too.
+template <class T>
+using gptr = const __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(const T *p) {
+    return (gptr<T>)p;
+}
+// Constant-address-space views of the same read-only arrays, for reads whose
+// address is wave-uniform: they become scalar loads into SGPRs, so the
+// arithmetic that follows stays on the scalar unit.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T *;
+template <class T>
+__device__ __forceinline__ cptr<T> to_const(const T *p) {
+    return (cptr<T>)p;
+}
+
+// Rightmost v in [lo, hi] with ws[v] <= t, given ws[lo] <= t.  Called with
+// wave-uniform arguments (scalar loads) and per lane (vector loads).
+template <class P>
+__device__ __forceinline__ uint32_t rightmost_le(P ws, uint32_t lo, uint32_t hi, uint64_t t) {
+    uint32_t s = lo + 1, e = hi + 1;
+    while (s < e) {
+        const uint32_t mid = s + ((e - s) >> 1);
+        if (ws[mid] <= t) s = mid + 1;
+        else e = mid;
+    }
+    return s - 1;
 }
 
 }  // namespace wtc

@@ -84,9 +84,12 @@ struct Slot {
     hipEvent_t ev[4] = {};  // prefix start, bitmap start, count start, count end
     hipEvent_t ev_ready = nullptr;  // input copied (waited on by the slot's stream)
     DevBuf ws, partials, tree, bm, desc, out;
+    // split count: kpos, full prefix, transposition counts/totals/offsets, in-edge tails
+    DevBuf sp_kpos, sp_wsfull, sp_hist, sp_tot, sp_rpin, sp_ie;
     PinBuf h_desc, h_out;
     uint32_t G = 0;       // graphs of the last enqueue
     uint32_t grid = 0;    // counting grid of the last enqueue
+    uint32_t rows_grid = 0;  // row-kernel grid of the last split count
     uint32_t small = 0;   // graphs of the last enqueue taken by the small-graph kernel
     int create();
     void release_device();  // frees the device workspace (keeps stream/events)
@@ -98,7 +101,8 @@ struct Tuning {
     int num_cus = 0;
     int ways = 4;         // probes advanced together per lane
     int wpb = 4;          // waves per counting block: 1, 2, 4 or 8
-    int kernel = 0;       // kKernelWalk or kKernelStaged
+    int kernel = 0;       // kKernelWalk, kKernelStaged or kKernelSplit
+    int split_wpb = 8;    // waves per block of the split count's row kernel
     uint32_t grab = 0;    // most tiles per grab (0: the kernel's default, 32)
     bool deal = true;     // deal super-chunks of grabs to the XCDs in turn
     uint32_t chunk = 16;  // wedges per lane staged in LDS per phase
@@ -145,6 +149,10 @@ struct SetPlan {
     uint32_t small_count = 0, small_lds = 0;
     uint32_t tile_graphs = 0;  // graphs counted by the wedge-parallel kernel
     size_t desc_bytes = 0;
+    // split count: SplitDesc at byte split_off of the descriptor buffer
+    bool split = false;
+    size_t split_off = 0;
+    uint32_t split_rows = 0, split_chunks = 0, split_parts = 0, split_wpb = 8;
 };
 // Sizes the slot's workspace for the set, assigns each graph its prefix and
 // bitmap (unless given), fills the pinned host descriptors and the plan.

@@ -76,7 +76,39 @@ struct GraphDesc {
 static_assert(sizeof(GraphDesc) == 128, "GraphDesc layout");
 
 // meta[] slots written by the setup kernel
-// (kMetaNext..kMetaNext+7: per-XCD work counters of the counting kernel)
-enum : int { kMetaTiles = 0, kMetaWedges = 1, kMetaNext = 8, kMetaSlots = 16 };
+// (kMetaNext..kMetaNext+7: per-XCD work counters of the counting kernel;
+// kMetaRows: tile counter of the split count's row kernel)
+enum : int { kMetaTiles = 0, kMetaWedges = 1, kMetaRows = 2, kMetaNext = 8, kMetaSlots = 16 };
+
+// Split count of one graph (wtc_split.hip): top rows per transposition part.
+constexpr uint32_t kSplitPart = 32768;
+// Threads of a transposition block (its LDS holds one part's counters).
+constexpr int kSplitBlock = 1024;
+// In-edges per tile of the row kernel.
+constexpr uint32_t kSplitTile = 1024;
+
+// Workspace and device-filled state of a split count.  The top rows are the
+// ranks [thr, n) of the graph's GraphDesc; OffT arrays use the set's offset
+// width.
+struct SplitDesc {
+    uint32_t *kpos;     // n: entries of adj+(v) below thr
+    uint64_t *ws_full;  // n+1: the full wedge prefix (ws of the GraphDesc holds the list part's)
+    uint64_t *ie_pre;   // n+1: prefix of the top in-edges (entries >= thr, not last) of each row
+    uint32_t *hist;     // chunks x rows: per-chunk in-edge counts, then their column prefix
+    uint32_t *row_tot;  // rows: in-edges of each top row
+    void *rp_in;        // rows+1 OffT: first in-edge of each top row
+    void *ie;           // cap pairs of OffT: the tail [lo, hi) of colInd of each in-edge
+    uint64_t cap;       // pairs ie holds (>= the in-edges of the top rows)
+    uint32_t rows;      // top rows: n - thr
+    uint32_t chunks;    // vertex chunks of the transposition (about equal shares of the in-edges)
+    uint32_t parts;     // row parts of kSplitPart rows
+    uint32_t pad0_;
+    // device-filled by the range kernel
+    uint64_t w_lo, w_hi;  // the counted wedges of the full order, clamped to [0, W]
+    uint32_t v_lo, v_hi;  // vertices holding w_lo and w_hi - 1 (v_lo > v_hi: none)
+    uint32_t clip;        // 1 unless [w_lo, w_hi) is the whole order
+    uint32_t pad_;
+};
+static_assert(sizeof(SplitDesc) == 112, "SplitDesc layout");
 
 }  // namespace wtc

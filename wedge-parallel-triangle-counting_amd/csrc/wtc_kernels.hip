@@ -234,39 +234,6 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_build(const GraphDesc *__rest
 
 // ------------------------------------------------------------------- count
 
-// Global-memory views of the CSR, prefix and bitmap arrays.  Their pointers
-// are read from descriptors in memory, so without the cast the compiler must
-// treat them as generic addresses: flat loads, which also count against the
-// LDS counter and make every wait cover the LDS traffic too.
-template <class T>
-using gptr = const __attribute__((address_space(1))) T *;
-template <class T>
-__device__ __forceinline__ gptr<T> to_global(const T *p) {
-    return (gptr<T>)p;
-}
-// Constant-address-space views of the same read-only arrays, for reads whose
-// address is wave-uniform: they become scalar loads into SGPRs, so the
-// arithmetic that follows stays on the scalar unit.
-template <class T>
-using cptr = const __attribute__((address_space(4))) T *;
-template <class T>
-__device__ __forceinline__ cptr<T> to_const(const T *p) {
-    return (cptr<T>)p;
-}
-
-// Rightmost v in [lo, hi] with ws[v] <= t, given ws[lo] <= t.  Called with
-// wave-uniform arguments (scalar loads) and per lane (vector loads).
-template <class P>
-__device__ __forceinline__ uint32_t rightmost_le(P ws, uint32_t lo, uint32_t hi, uint64_t t) {
-    uint32_t s = lo + 1, e = hi + 1;
-    while (s < e) {
-        const uint32_t mid = s + ((e - s) >> 1);
-        if (ws[mid] <= t) s = mid + 1;
-        else e = mid;
-    }
-    return s - 1;
-}
-
 // Wave-uniform search for the tile's first vertex over [0, n) of a single
 // graph: 12 levels from the compact search tree, then wedgeSum
 // (tc.cu:139-170 semantics).
@@ -883,7 +850,6 @@ __global__ __launch_bounds__(kWave * kCountWavesMax) void k_walk(const GraphDesc
             const GraphDesc &D = desc[g];
             const gptr<OffT> rowPtr = to_global(static_cast<const OffT *>(D.rowPtr));
             const gptr<uint32_t> colInd = to_global(D.colInd);
-            const gptr<uint64_t> ws = to_global(static_cast<const uint64_t *>(D.ws));
             const gptr<uint32_t> bitmap = to_global(static_cast<const uint32_t *>(D.bitmap));
             // the same arrays for wave-uniform reads (scalar loads)
             const cptr<OffT> rowPtrU = to_const(static_cast<const OffT *>(D.rowPtr));
@@ -906,14 +872,17 @@ __global__ __launch_bounds__(kWave * kCountWavesMax) void k_walk(const GraphDesc
                 cur_cnt = 0;
             }
 
-            // walk state (wave-uniform): vertex v with vb, d, its C(d,2)
-            // wedges cv of which lv in long rows, position iv in it; in the
-            // long rows the current row wi, position ui and w id wcur
+            // walk state (wave-uniform): vertex v with vb, d, its cv wedges
+            // (ws[v+1] - ws[v]: C(d,2), or in a list-part prefix the first
+            // wedges of its row-major order) of which the first lv lie in
+            // long rows, position iv in them; in the long rows the current
+            // row wi, position ui and w id wcur
             uint32_t v = uni((tree != nullptr) ? tree_search(to_const(tree), wsU, gn, pos)
                                                : rightmost_le(wsU, 0u, vmax, pos));
             OffT vb = uni(rowPtrU[v]);
             uint32_t d = uni((uint32_t)(rowPtrU[v + 1] - vb));
-            uint64_t cv = binom2_u32(d), lv = long_wedges(d), iv = uni(pos - wsU[v]);
+            uint64_t wsv = uni(wsU[v]);
+            uint64_t cv = uni(wsU[v + 1] - wsv), lv = long_wedges(d), iv = uni(pos - wsv);
             uint32_t wi = 0, ui = 1, wcur = 0;
             if (iv < lv) {
                 decode_wedge(d, iv, wi, ui);
@@ -942,14 +911,17 @@ __global__ __launch_bounds__(kWave * kCountWavesMax) void k_walk(const GraphDesc
                         v = uni(wsU[v + 2] > pos ? v + 1 : gallop_le(wsU, v + 2, vmax, pos));
                         vb = uni(rowPtrU[v]);
                         d = uni((uint32_t)(rowPtrU[v + 1] - vb));
-                        cv = binom2_u32(d);
+                        cv = uni(wsU[v + 1] - wsU[v]);
                         lv = long_wedges(d);
                         iv = 0;
                         wi = 0;
                         ui = 1;
                         if (lv) wcur = uni(colIndU[vb]);
                     }
-                    const uint64_t left = pend - pos;
+                    // wedges this step may take: up to the range end and
+                    // the vertex's last wedge
+                    const uint64_t left0 = pend - pos, rem = cv - iv;
+                    const uint64_t left = rem < left0 ? rem : left0;
                     if (iv < lv) {
                         uint64_t m = lv - iv;
                         m = m < left ? m : left;
@@ -1267,14 +1239,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_u32x4(const uint4 *__restrict__
 // ---------------------------------------------------------------- launches
 
 // Launches KERNEL<uint64_t> or KERNEL<uint32_t> by row-offset width.
-#define WTC_LAUNCH_OFF(ob, KERNEL, GRID, BLOCK, SHMEM, STREAM, ...)               \
-    [&]() -> hipError_t {                                                         \
-        if ((ob) == 8)                                                            \
-            KERNEL<uint64_t><<<GRID, BLOCK, SHMEM, STREAM>>>(__VA_ARGS__);        \
-        else                                                                      \
-            KERNEL<uint32_t><<<GRID, BLOCK, SHMEM, STREAM>>>(__VA_ARGS__);        \
-        return hipGetLastError();                                                 \
-    }()
+
 
 hipError_t launch_copy(const void *src, void *dst, uint64_t bytes, hipStream_t st) {
     const uint64_t n = bytes / 16;

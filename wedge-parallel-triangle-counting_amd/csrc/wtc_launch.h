@@ -39,7 +39,11 @@ bool valid_count_waves(int wpb);
 // Counting kernels: kKernelWalk (k_walk: waves walk the wedge order in steps
 // of 64 consecutive wedges) and kKernelStaged (k_count: per-lane spreads
 // staged through LDS).  Both count the same wedges of the same tiles.
-enum : int { kKernelWalk = 0, kKernelStaged = 1 };
+// kKernelSplit: a single graph with a > 0 counted by the split count (the
+// top-a part grouped by middle vertex against one LDS bitmap row, the rest
+// by k_walk over the list part's own prefix; wtc_split.hip); any other set
+// with it by k_walk.
+enum : int { kKernelWalk = 0, kKernelStaged = 1, kKernelSplit = 2 };
 // `ways`: independent probes each lane advances together (1, 2, 4 or 8; for
 // k_walk, the steps in flight); `wpb`: waves per block; `staged` = min(max
 // spread, chunk) (k_count's LDS staging).
@@ -50,6 +54,25 @@ hipError_t count_grid(int kernel, int ob, uint32_t staged, int ways, int wpb, in
 hipError_t launch_count(int kernel, const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
                         unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, int wpb,
                         uint32_t work, uint32_t grid, hipStream_t st);
+// Split count (wtc_split.hip).  partials holds 3*nblocks entries.
+hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t nblocks,
+                               uint64_t *partials, hipStream_t st);
+// Maps the GraphDesc's range onto the list part (before launch_setup).
+hipError_t launch_split_range(GraphDesc *desc, SplitDesc *sd, hipStream_t st);
+// The counting sort of the top rows' in-edges; zeroes meta[kMetaRows].
+hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
+                                  uint32_t parts, uint32_t rows, uint64_t *meta, hipStream_t st);
+// Waves per block of the row kernel: 4, 8 or 16.
+bool valid_split_waves(int wpb);
+// Persistent grid of the row kernel for `rows` top rows (its LDS bitmap).
+hipError_t split_rows_grid(int ob, uint32_t rows, int wpb, int num_cus, uint32_t *grid);
+hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t rows, int wpb,
+                             uint64_t *meta, unsigned long long *counts, uint32_t grid, hipStream_t st);
+// Vertex chunks of the transposition (equal shares of the in-edges): one
+// per kSplitChunkVerts vertices, at least 1 and at most kSplitChunksMax.
+constexpr uint32_t kSplitChunksMax = 1024;
+constexpr uint32_t kSplitChunkVerts = 2048;
+
 // The `work` word of a grab size and XCD dealing.
 inline uint32_t count_work(uint32_t tiles_per_grab, bool deal) {
     return (tiles_per_grab & 0xffffu) | (deal ? 1u << 16 : 0u);

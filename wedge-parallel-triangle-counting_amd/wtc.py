@@ -143,6 +143,9 @@ _sig = {
                                         C.POINTER(C.c_uint64), C.POINTER(CountStats)]),
     "wtc_dgraph_count_baseline": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_uint32, C.POINTER(C.c_uint64),
                                            C.POINTER(CountStats)]),
+    "wtc_autotune_shape": (C.c_int, [C.c_uint32, C.c_uint64, C.c_uint64, C.c_int, _P]),
+    "wtc_tuning_source": (C.c_char_p, []),
+    "wtc_ctx_use_tuned": (C.c_int, [_P, _P]),
     "wtc_count_noalloc": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint32,
                                    C.c_uint64, C.c_uint64, _P, _P]),
     "wtc_ctx_debug_prefix": (C.c_int, [_P, C.c_uint64, _P, _P]),
@@ -336,6 +339,27 @@ def autotune(n: int, m: int = 0, wedges: int = 0, batch: bool = False):
     return int(s.value), int(a.value)
 
 
+class Tuned(C.Structure):
+    """wtc_tuned: the autotuner's whole choice for one graph."""
+    _fields_ = [("spread", C.c_uint32), ("adjacency_matrix_len", C.c_uint32), ("kernel", C.c_int32),
+                ("waves_per_block", C.c_int32), ("probe_ways", C.c_int32), ("split_waves", C.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def autotune_shape(n: int, m: int = 0, wedges: int = 0, batch: bool = False) -> Tuned:
+    """The tuning table's row for a graph: spread, a, kernel and block shape."""
+    t = Tuned()
+    _check(_lib.wtc_autotune_shape(n, m, wedges, int(batch), C.byref(t)), "autotune_shape")
+    return t
+
+
+def tuning_source() -> str:
+    """Path of the tuning table in use, or 'built-in'."""
+    return _lib.wtc_tuning_source().decode()
+
+
 # ---------------------------------------------------------- reference-style
 
 def tc_gpu(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -> int:
@@ -385,7 +409,7 @@ def tc_arrow_gpu_big(g: CSR, spread: int, a: int, t: Optional[GpuTime] = None) -
 
 WEDGE_OUTGOING, WEDGE_ARROW, WEDGE_MIXED = 0, 1, 2
 PARALLEL_VERTEX, PARALLEL_EDGE = 0, 1
-KERNEL_WALK, KERNEL_STAGED = 0, 1
+KERNEL_WALK, KERNEL_STAGED, KERNEL_SPLIT = 0, 1, 2
 
 
 def _baseline_call(fn32, fn64, g: CSR, a: int, style: int, t: Optional[GpuTime], big: bool) -> int:
@@ -457,8 +481,14 @@ class Context:
         _check(_lib.wtc_ctx_set_tuning(self._p, probe_ways, lds_chunk, waves_per_block), "set_tuning")
 
     def set_count_kernel(self, kernel: int):
-        """KERNEL_WALK (0) or KERNEL_STAGED (1); counts do not depend on it."""
+        """KERNEL_WALK (0), KERNEL_STAGED (1) or KERNEL_SPLIT (2: a single graph
+        with a > 0 counts its top-a wedges by middle vertex against an LDS
+        bitmap row, the rest by the walk); counts do not depend on it."""
         _check(_lib.wtc_ctx_set_count_kernel(self._p, kernel), "set_count_kernel")
+
+    def use_tuned(self, t: "Tuned"):
+        """Applies an autotuned shape (kernel, block waves, probe ways, row-kernel waves)."""
+        _check(_lib.wtc_ctx_use_tuned(self._p, C.byref(t)), "use_tuned")
 
     def set_work_split(self, tiles_per_grab: int = 0, deal_to_xcds: bool = True):
         """Most tiles per grab (0: 32) and XCD dealing of the counting kernels."""
