@@ -43,70 +43,94 @@ def load_traffic(workload, spread, a):
         return json.load(f)
 
 
-def cpu_baseline(g, dg, spread, a, target_s, threads):
-    """Oracle merge-intersection over a contiguous vertex sample, timed on the
-    host; the GPU counts the same sample's wedge range for a parity check."""
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo) and logical CPU count."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count() or 0
+
+
+def stratified_rows(ws, frac, strata):
+    """`strata` runs of rows spread evenly over the wedge order, each starting
+    at the first row at or after k/strata of the wedges and covering about
+    frac/strata of them: a sample whose per-wedge cost follows the whole graph
+    (hub rows at the front, low-degree rows at the back)."""
+    W = int(ws[-1])
+    n = len(ws) - 1
+    runs = []
+    for k in range(strata):
+        v0 = int(np.searchsorted(ws, (W * k) // strata, side="left"))
+        v0 = min(v0, n - 1)
+        v1 = int(np.searchsorted(ws, int(ws[v0]) + max(1, int(W * frac / strata)), side="left"))
+        v1 = min(max(v1, v0 + 1), n)
+        if runs and v0 < runs[-1][1]:
+            v0 = runs[-1][1]
+        if v0 < v1:
+            runs.append((v0, v1))
+    return runs
+
+
+def cpu_baseline(g, dg, spread, a, target_s, threads, strata=32, reps=5):
+    """The oracle's merge intersection (the CPU restatement of the count) over
+    a stratified sample of the graph's rows, timed on the host: one warm-up
+    and the median of `reps` timed runs, each sized to about target_s/(reps+1)
+    seconds.  The GPU counts the same rows' wedge ranges for a parity check."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
     d = np.diff(g.row_ptr.astype(np.int64))
     ws = np.concatenate([[0], np.cumsum(d * (d - 1) // 2)]).astype(np.uint64)
     W = int(ws[-1])
-    v0 = g.n // 2
-    # calibrate on a small sample, then size the sample to ~target_s seconds
-    frac = 1e-4
-    for _ in range(6):
-        v1 = int(np.searchsorted(ws, ws[v0] + int(W * frac), side="right"))
-        v1 = min(max(v1, v0 + 1), g.n)
+
+    def run(runs):
         t0 = time.perf_counter()
-        O.tc_merge_range(g, v0, v1, threads)
-        dt = time.perf_counter() - t0
-        if dt > 0.5 or v1 >= g.n:
+        c = sum(O.tc_merge_range(g, v0, v1, threads) for v0, v1 in runs)
+        return c, time.perf_counter() - t0
+
+    per_rep = target_s / (reps + 1)
+    frac = 2e-4
+    for _ in range(8):  # calibrate: grow the sample until one run takes a noticeable time
+        runs = stratified_rows(ws, frac, strata)
+        _, dt = run(runs)
+        if dt > 0.3 or frac >= 1.0:
             break
-        frac *= 8
-    frac = min(frac * target_s / max(dt, 1e-3), 1.0)
-    v1 = int(np.searchsorted(ws, ws[v0] + int(W * frac), side="right"))
-    v1 = min(max(v1, v0 + 1), g.n)
-    sample_w = int(ws[v1] - ws[v0])
-    t0 = time.perf_counter()
-    cpu_count = O.tc_merge_range(g, v0, v1, threads)
-    dt = time.perf_counter() - t0
-    gpu_count = dg.count(spread, a, int(ws[v0]), int(ws[v1]))
-    # the same oracle on one thread, over the first eighth of the sample's rows
-    v1s = int(np.searchsorted(ws, ws[v0] + sample_w // 8, side="right"))
-    v1s = min(max(v1s, v0 + 1), v1)
-    t0 = time.perf_counter()
-    O.tc_merge_range(g, v0, v1s, 1)
-    dt1 = time.perf_counter() - t0
+        frac = min(1.0, frac * 4)
+    frac = min(1.0, frac * per_rep / max(dt, 1e-3))
+    runs = stratified_rows(ws, frac, strata)
+    sample_w = sum(int(ws[v1] - ws[v0]) for v0, v1 in runs)
+    cpu_count, _ = run(runs)  # warm-up
+    times = []
+    for _ in range(reps):
+        c, dt = run(runs)
+        assert c == cpu_count
+        times.append(dt)
+    dt = statistics.median(times)
+    gpu_count = sum(dg.count(spread, a, int(ws[v0]), int(ws[v1])) for v0, v1 in runs)
+    model, logical = cpu_model()
     return {
         "value": sample_w / dt, "unit": "wedge checks/s", "cores": threads, "kind": "port",
-        "value_1thread": int(ws[v1s] - ws[v0]) / dt1,
-        "sample": f"oracle merge-intersection over rows [{v0},{v1}) = {sample_w} of {W} wedges "
-                  f"({100.0 * sample_w / W:.2f}%), {dt:.1f} s on {threads} host threads",
-        "seconds": dt, "sample_triangles": cpu_count, "gpu_sample_triangles": gpu_count,
+        "cpu_model": model, "host_logical_cpus": logical,
+        "sample": f"oracle merge intersection over {len(runs)} row runs spread evenly over the wedge order "
+                  f"({sample_w} of {W} wedges, {100.0 * sample_w / W:.2f}%), {threads} host threads, "
+                  f"one warm-up then median of {reps}: {dt:.2f} s",
+        "seconds": dt, "seconds_all": times, "sample_wedges": sample_w,
+        "sample_triangles": cpu_count, "gpu_sample_triangles": gpu_count,
         "sample_parity": cpu_count == gpu_count,
     }
 
 
 def batch_graphs(k, base_seed):
-    """The batching workload: k seeded graphs, each Erdos-Renyi or
-    Barabasi-Albert (50/50), n log-uniform in [64, 65536], average degree
-    uniform in [4, 64]; returned oriented (degree order)."""
-    from concurrent.futures import ThreadPoolExecutor
-
-    def one(i):
-        r = np.random.default_rng(base_seed + i)
-        er = r.random() < 0.5
-        n = int(round(np.exp(r.uniform(np.log(64), np.log(65536)))))
-        kdeg = r.uniform(4, 64)
-        if er:
-            g = wtc.gen_erdos_renyi(n, min(1.0, kdeg / (n - 1)), seed=base_seed + i)
-        else:
-            g = wtc.gen_barabasi_albert(n, max(1, min(n - 1, int(np.ceil(kdeg / 2)))), seed=base_seed + i)
-        return wtc.preprocess(g)
-
-    with ThreadPoolExecutor(max_workers=min(16, len(os.sched_getaffinity(0)))) as ex:
-        return list(ex.map(one, range(k)))
+    """The batching workload (Erdos-Renyi / Barabasi-Albert, n log-uniform in
+    [64, 65536], average degree uniform in [4, 64]), oriented."""
+    import workloads as WL
+    return WL.batch_graphs(k, base_seed)
 
 
 def batch_bench(ctx, k, base_seed):
@@ -300,23 +324,21 @@ def reference_configs(iters=10):
     return out
 
 
-def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, waves):
+def serve_bench(lams, seconds, queue_capacity, ways, chunk, waves, slots=2):
     """Serving workload: Poisson arrivals of count requests on seeded R-MAT
-    graphs (scale uniform in [10, 22], edge factor 16), each with the
+    graphs (scales 10..22, edge factor 16, two seeds per scale), each with the
     autotuner's spread and bitmap length.  Graphs are registered once and stay
     resident; requests are admitted by HBM footprint and batched continuously.
     Per arrival rate: p50/p99 latency, completed graphs/s, rejected requests,
-    and exactness against each graph's count served alone.  The plain path on
-    the same mix is one reference-style tc_GPU call per request (alloc, copy,
-    count, copy back, free)."""
-    import threading
-    from concurrent.futures import ThreadPoolExecutor
+    and exactness of every completed request against the oracle's count of
+    its graph (tests/golden) and against the graph served alone.  The plain
+    path on the same mix is one reference-style tc_GPU call per request
+    (alloc, copy, count, copy back, free)."""
+    import workloads as WL
 
-    scales = list(range(10, 23))
-    keys = [(sc, k) for sc in scales for k in range(seeds_per_scale)]
-    with ThreadPoolExecutor(max_workers=4) as ex:
-        graphs = list(ex.map(lambda key: wtc.preprocess(wtc.gen_rmat(key[0], 16, seed=7000 + 31 * key[0] + key[1])),
-                             keys))
+    keys = WL.serve_keys()
+    graphs = WL.serve_graphs()
+    oracle_counts = load_golden_serve(keys)
     tune = [wtc.autotune(g.n, g.m, g.wedges()) for g in graphs]
     # plain path on the same mix: mean time of one reference-style call per graph
     plain_ms = []
@@ -328,7 +350,7 @@ def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, wav
         plain_ms.append((time.perf_counter() - t0) / 2 * 1e3)
     plain_gps = 1e3 / float(np.mean(plain_ms))
     log(f"serve: plain path {plain_gps:.1f} graphs/s on the uniform mix")
-    srv = wtc.Server(0, hbm_budget_bytes=32 << 30, max_batch=256, slots=2, queue_capacity=queue_capacity,
+    srv = wtc.Server(0, hbm_budget_bytes=32 << 30, max_batch=256, slots=slots, queue_capacity=queue_capacity,
                      probe_ways=ways, lds_chunk=chunk, waves_per_block=waves)
     gids = [srv.register(g) for g in graphs]
     for gid, (s_g, a_g) in zip(gids, tune):  # each graph alone once: resident, server warmed up
@@ -338,60 +360,153 @@ def serve_bench(lams, seconds, seeds_per_scale, queue_capacity, ways, chunk, wav
         cnt, st = srv.wait(srv.submit(gid, s_g, a_g))
         alone.append(cnt)
         alone_ms.append(st.service_ms)
+    alone_oracle = oracle_counts is not None and alone == oracle_counts
     log(f"serve: {len(graphs)} graphs resident, alone service ms by scale: "
-        + ", ".join(f"{sc}:{alone_ms[i]:.2f}" for i, (sc, k) in enumerate(keys) if k == 0))
+        + ", ".join(f"{sc}:{alone_ms[i]:.2f}" for i, (sc, _) in enumerate(keys) if i % WL.SERVE_SEEDS_PER_SCALE == 0))
+    expect = oracle_counts if oracle_counts is not None else alone
     results = []
     for lam in lams:
-        rng = np.random.default_rng(int(lam))
-        n_req = int(lam * seconds)
-        gaps = rng.exponential(1.0 / lam, size=n_req)
-        arrive = np.cumsum(gaps)
-        pick = [rng.integers(0, len(scales)) * seeds_per_scale + rng.integers(0, seeds_per_scale)
-                for _ in range(n_req)]
-        rids = [None] * n_req
-        t0 = time.perf_counter()
-
-        def submitter():
-            for i in range(n_req):
-                while True:
-                    left = arrive[i] - (time.perf_counter() - t0)
-                    if left <= 0:
-                        break
-                    time.sleep(left - 2e-4 if left > 1e-3 else 0)
-                try:
-                    rids[i] = srv.submit(gids[pick[i]], *tune[pick[i]])
-                except wtc.WtcError:
-                    rids[i] = -1  # rejected: queue full
-
-        th = threading.Thread(target=submitter)
-        th.start()
-        lat, exact, done = [], True, 0
-        t_last = t0
-        for i in range(n_req):
-            while rids[i] is None:
-                time.sleep(0.0002)
-            if rids[i] == -1:
-                continue
-            cnt, st = srv.wait(rids[i])
-            t_last = time.perf_counter()
-            exact &= cnt == alone[pick[i]]
-            lat.append(st.latency_ms)
-            done += 1
-        th.join()
-        span = max(t_last - t0, 1e-9)
-        lat = np.asarray(lat) if lat else np.zeros(1)
-        rec = {"lambda": lam, "requests": n_req, "completed": done, "rejected": n_req - done,
-               "graphs_per_s": done / span, "p50_ms": float(np.percentile(lat, 50)),
+        pick, res, span = WL.poisson_run(srv, gids, tune, lam, seconds, seed=int(lam))
+        done = [(i, r) for i, r in enumerate(res) if r is not None]
+        lat = np.asarray([r[1] for _, r in done]) if done else np.zeros(1)
+        exact = all(r[0] == expect[pick[i]] for i, r in done)
+        rec = {"lambda": lam, "requests": len(res), "completed": len(done), "rejected": len(res) - len(done),
+               "graphs_per_s": len(done) / span, "p50_ms": float(np.percentile(lat, 50)),
                "p99_ms": float(np.percentile(lat, 99)), "exact": bool(exact)}
         log(f"serve lambda={lam}: {rec}")
         results.append(rec)
     stats = srv.stats()
     srv.close()
     best = max(r["graphs_per_s"] for r in results) if results else 0.0
-    return {"graphs": len(graphs), "scales": [scales[0], scales[-1]], "seconds_per_rate": seconds,
-            "queue_capacity": queue_capacity, "params": "autotuned per graph", "rates": results,
+    return {"graphs": len(graphs), "scales": [WL.SERVE_SCALES[0], WL.SERVE_SCALES[-1]], "seconds_per_rate": seconds,
+            "slots": slots, "queue_capacity": queue_capacity, "params": "autotuned per graph",
+            "exact_against": "oracle counts (tests/golden)" if oracle_counts is not None else "each graph served alone",
+            "alone_equals_oracle": alone_oracle, "alone_service_ms": alone_ms, "rates": results,
             "plain_graphs_per_s": plain_gps, "served_max_graphs_per_s": best,
             "served_over_plain": best / plain_gps, "server": stats}
+
+
+def load_golden_serve(keys):
+    """The oracle's triangle count of every serving-mix graph, in `keys`
+    order, from tests/golden (None when absent or for another mix)."""
+    path = os.path.join(ROOT, "tests", "golden", "big_golden.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        mix = json.load(f).get("serve_mix", [])
+    by_key = {(r["scale"], r["seed"]): int(r["triangles"]) for r in mix if r.get("edge_factor", 16) == 16}
+    if not all(k in by_key for k in keys):
+        return None
+    return [by_key[k] for k in keys]
+
+
+def _time_count(dg, s, a, reps=3):
+    """Median device time (prefix + bitmap/transposition + count, ms) of a
+    count, after one untimed count, and the count itself."""
+    c0 = dg.count(s, a)
+    ts = []
+    for _ in range(reps):
+        c, st = dg.count(s, a, stats=True)
+        assert c == c0
+        ts.append(st.prefix_ms + st.bitmap_ms + st.count_ms)
+    return statistics.median(ts), c0
+
+
+def tune_sweep(args):
+    """The autotuner's sweep: per size bucket, the fastest (spread, a,
+    kernel, block waves, probe ways, row-kernel waves) measured on this GPU,
+    written as the table the library loads (tuning/mi355x.tsv beside it).
+    Skewed buckets: R-MAT graphs (edge factor 16, seed 1) at --tune-scales;
+    flat buckets: Erdos-Renyi graphs of average degree 16; batch: the
+    4096-graph batching workload.  Stage 1 takes (kernel, s, a) at the
+    default block shape, stage 2 the block shape for that choice."""
+    import math
+
+    import workloads as WL
+
+    ctx = wtc.Context(0)
+    rows = []
+
+    def sweep_alone(g, family_skew, workload):
+        n, m, W = g.n, g.m, g.wedges()
+        dg = ctx.upload(g)
+        alens = [0] + [x for x in (2048, 8192, 32768, 131072, 262144) if x <= ((n + 31) // 32) * 32 and W > 0]
+        best = None
+        base = wtc.Tuned(0, 0, wtc.KERNEL_WALK, 4, 4, 8)
+        for kernel in (wtc.KERNEL_WALK, wtc.KERNEL_SPLIT):
+            for a in alens:
+                if kernel == wtc.KERNEL_SPLIT and a == 0:
+                    continue
+                for sp in (2, 4, 8, 12, 16):
+                    t = wtc.Tuned(sp, a, kernel, base.waves_per_block, base.probe_ways, base.split_waves)
+                    ctx.use_tuned(t)
+                    ms, _ = _time_count(dg, sp, a)
+                    if best is None or ms < best[0]:
+                        best = (ms, t)
+        ms0, t0 = best
+        for waves in (2, 4, 8):
+            for ways in (2, 4, 8):
+                for sw in ((4, 8, 16) if t0.kernel == wtc.KERNEL_SPLIT else (8,)):
+                    t = wtc.Tuned(t0.spread, t0.adjacency_matrix_len, t0.kernel, waves, ways, sw)
+                    ctx.use_tuned(t)
+                    ms, _ = _time_count(dg, t.spread, t.adjacency_matrix_len)
+                    if ms < best[0]:
+                        best = (ms, t)
+        dg.free()
+        ms, t = best
+        a = t.adjacency_matrix_len
+        kname = {wtc.KERNEL_WALK: "walk", wtc.KERNEL_STAGED: "staged", wtc.KERNEL_SPLIT: "split"}[t.kernel]
+        row = ("alone", family_skew, int(math.floor(math.log2(W))) if W else 0, t.spread, 1 if a else 0, a, kname,
+               t.waves_per_block, t.probe_ways, t.split_waves, ms, workload)
+        log(f"tune {workload}: n={n} m={m} W={W} -> {row}")
+        rows.append(row)
+
+    for scale in [int(x) for x in args.tune_scales.split(",") if x]:
+        sweep_alone(WL.rmat(scale), "skew", f"rmat{scale}")
+    for lg in [int(x) for x in args.tune_flat.split(",") if x]:
+        n = 1 << lg
+        g = wtc.preprocess(wtc.gen_erdos_renyi(n, 16.0 / (n - 1), seed=lg))
+        sweep_alone(g, "flat", f"er{lg}-deg16")
+    if args.tune_batch:
+        graphs = WL.batch_graphs(args.tune_batch)
+        b = wtc.Batch(ctx, graphs)
+        best = None
+        for sp in (4, 8, 12, 16):
+            for waves in (2, 4, 8):
+                for ways in (2, 4, 8):
+                    ctx.use_tuned(wtc.Tuned(sp, 0, wtc.KERNEL_WALK, waves, ways, 8))
+                    spreads, alens = [sp] * len(graphs), [0] * len(graphs)
+                    b.count(spreads, alens)
+                    ts = []
+                    for _ in range(3):
+                        t0 = time.perf_counter()
+                        b.count(spreads, alens)
+                        ts.append((time.perf_counter() - t0) * 1e3)
+                    ms = statistics.median(ts)
+                    if best is None or ms < best[0]:
+                        best = (ms, sp, waves, ways)
+        b.free()
+        ms, sp, waves, ways = best
+        rows.append(("batch", "any", 0, sp, 0, 0, "walk", waves, ways, 8, ms, f"batch{args.tune_batch}"))
+        log(f"tune batch: {rows[-1]}")
+    ctx.close()
+    # buckets below the smallest measured graph of a family take its row
+    out_dir = os.path.join(PKG, "tuning")
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, "mi355x.tsv")
+    with open(path, "w") as f:
+        f.write("# Autotuner table for MI355X, written by: python bench.py --tune --tune-scales "
+                f"{args.tune_scales} --tune-flat {args.tune_flat} --tune-batch {args.tune_batch}\n")
+        f.write("# family skew log2w spread a_div a_cap kernel waves ways split_waves  measured_ms workload\n")
+        by_fam = {}
+        for r in rows:
+            by_fam.setdefault((r[0], r[1]), []).append(r)
+        for key, rs in by_fam.items():
+            rs.sort(key=lambda r: r[2])
+            rs[0] = (rs[0][0], rs[0][1], 0) + tuple(rs[0][3:])  # the smallest graph's row covers the buckets below it
+            for r in rs:
+                f.write(" ".join(str(x) for x in r[:10]) + f"  {r[10]:.3f} {r[11]}\n")
+    print(json.dumps({"tuning_table": path, "rows": rows}), flush=True)
 
 
 def ingest_bench(scale, seed):
@@ -490,6 +605,45 @@ def pmc_summary(args):
     print(json.dumps(out))
 
 
+def load_golden_rmat(scale, edge_factor, seed):
+    """The oracle's whole-graph triangle count of a seeded R-MAT graph, where
+    tests/golden holds one (R-MAT-24, edge factor 16, seed 1)."""
+    path = os.path.join(ROOT, "tests", "golden", "big_golden.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        gold = json.load(f)
+    rec = gold.get(f"rmat{scale}")
+    if rec and rec.get("edge_factor", 16) == edge_factor and rec.get("seed", 1) == seed:
+        return int(rec["triangles"])
+    return None
+
+
+def offsets_u64_leg(ctx, g, W, spread, a, steps, warmup, expect):
+    """The headline graph uploaded again with 8-byte row offsets and counted
+    the same way: the 64-bit CSR layout at the same size."""
+    dg8 = ctx.upload(g, offset_bytes=8)
+    try:
+        for _ in range(warmup):
+            dg8.count(spread, a)
+        counts, stats = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            c, st = dg8.count(spread, a, stats=True)
+            counts.append(c)
+            stats.append(st.as_dict())
+        el = time.perf_counter() - t0
+    finally:
+        dg8.free()
+    _, b_min8 = g.alg_bytes(8)
+    count_ms = statistics.median(s["count_ms"] for s in stats)
+    return {"offset_bytes": 8, "value": W / (el / steps), "unit": "wedge checks/s", "ms_per_step": 1e3 * el / steps,
+            "kernel_ms": {"count": count_ms, "prefix": statistics.median(s["prefix_ms"] for s in stats),
+                          "bitmap": statistics.median(s["bitmap_ms"] for s in stats)},
+            "roofline_frac": b_min8 / (count_ms / 1e3) / HBM_PEAK,
+            "triangles": counts[0], "counts_equal_u32_offsets": all(c == expect for c in counts)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -529,10 +683,19 @@ def main():
     ap.add_argument("--small-batches-only", action="store_true",
                     help="run only the small-graph kernel vs wedge kernel comparison on batches of small graphs")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving (Poisson arrivals) workload")
-    ap.add_argument("--serve-rates", default="100,1000",
+    ap.add_argument("--serve-rates", default="100,250,500,1000,2000,3500,5000",
                     help="Poisson arrival rates of the serving leg (empty: skip it)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-slots", type=int, default=2, help="execution slots (streams) of the server")
+    ap.add_argument("--no-u64-leg", action="store_true", help="skip the 8-byte-offset leg of the headline graph")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU baseline (at most the CPUs this process may use)")
     ap.add_argument("--ingest-only", action="store_true", help="run only the text/CSR-file ingestion timing")
+    ap.add_argument("--tune", action="store_true",
+                    help="run the autotuner's sweep and write the tuning table the library loads")
+    ap.add_argument("--tune-scales", default="12,14,16,18,20,22,24", help="R-MAT scales of the skewed buckets")
+    ap.add_argument("--tune-flat", default="14,18,22", help="log2 n of the Erdos-Renyi (flat) buckets")
+    ap.add_argument("--tune-batch", type=int, default=4096, help="graphs of the batch bucket (0: skip)")
     args = ap.parse_args()
 
     if args.pmc_summary:
@@ -540,6 +703,9 @@ def main():
         return
     if args.ingest_only:
         print(json.dumps(ingest_bench(args.scale, args.seed)), flush=True)
+        return
+    if args.tune:
+        tune_sweep(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -556,8 +722,8 @@ def main():
         return
 
     if args.serve_only:
-        res = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 2, 512,
-                          args.ways, args.chunk, args.waves)
+        res = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 512,
+                          args.ways, args.chunk, args.waves, args.serve_slots)
         print(json.dumps(res), flush=True)
         return
     if args.reference_configs:
@@ -626,8 +792,11 @@ def main():
 
     ms_per_step = 1e3 * elapsed / args.steps
     count_ms = statistics.median(s["count_ms"] for s in stats)
-    achieved = b_alg / (count_ms / 1e3)
+    # algorithmic bytes: the CSR once plus, per wedge, the 4-byte id of its
+    # far end and one 4-byte probe word -- bytes any exact wedge check moves
+    achieved = b_min / (count_ms / 1e3)
     traffic = load_traffic(workload, args.spread, args.a)
+    golden = load_golden_rmat(args.scale, args.edge_factor, args.seed)
     out = {
         "metric": "wedge checks/s & edges/s at RMAT-24 (1 GPU, exact); batched vs plain graphs/s",
         "value": W / (ms_per_step / 1e3),
@@ -653,15 +822,30 @@ def main():
         "wedge_checks_per_s_kernel": W / (count_ms / 1e3),
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "b_alg": b_alg, "b_min": b_min,
-                     # measured HBM bytes per launch over the same launch time: the rate the memory
-                     # actually ran at (achieved/frac count every probe as HBM bytes, most hit caches)
+                     "algorithmic_bytes": b_min, "bytes_per_wedge": 8,
+                     "kernel_ms": count_ms,
+                     # the reference's own probe words (every binary-search step one word): an upper
+                     # figure, most of whose words a bitmap or the caches never fetch from HBM
+                     "b_ref_alg": b_alg,
+                     # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on
+                     # a copy of known size) over the same launch time: the rate the memory ran at
                      "traffic_GBps": traffic["bytes_per_launch"] / (count_ms / 1e3) / 1e9 if traffic else None,
-                     "traffic_frac": traffic["bytes_per_launch"] / (count_ms / 1e3) / HBM_PEAK if traffic else None},
+                     "traffic_frac": traffic["bytes_per_launch"] / (count_ms / 1e3) / HBM_PEAK if traffic else None,
+                     "traffic_profile": traffic_name(workload, args.spread, args.a) if traffic else None},
         "prepro_s": t2 - t1,
         "builder": "device" if device_build else "host",
         "tuning": {"probe_ways": args.ways, "lds_chunk": args.chunk, "waves_per_block": args.waves},
     }
+    if golden is not None:
+        out["triangles_expected"] = golden
+        out["triangles_match_expected"] = counts[0] == golden
+        log(f"triangles {'match' if counts[0] == golden else 'DIFFER FROM'} the oracle's whole-graph count {golden}")
+    # the same graph with 8-byte row offsets (the 64-bit build's CSR layout)
+    if not args.no_u64_leg and ob == 4:
+        out["offsets_u64"] = offsets_u64_leg(ctx, g, W, args.spread, args.a, args.steps, args.warmup,
+                                             counts[0])
+        log(f"u64 offsets: {out['offsets_u64']['ms_per_step']:.2f} ms/step, "
+            f"count kernel {out['offsets_u64']['kernel_ms']['count']:.2f} ms")
     # size-independent check at full size: four wedge ranges sum to the whole count
     if not args.no_range_check:
         cuts = [0, W // 5, W // 2, (4 * W) // 5, W]
@@ -669,7 +853,8 @@ def main():
         out["range_split"] = {"cuts": cuts, "parts": parts, "sum_equals_total": sum(parts) == counts[0]}
         log(f"range split: {parts} sum ok = {sum(parts) == counts[0]}")
     if not args.no_cpu:
-        threads = min(len(os.sched_getaffinity(0)), 16)
+        # every host CPU this process may use, up to the 16 a one-GPU share of the host holds
+        threads = min(len(os.sched_getaffinity(0)), args.cpu_threads)
         out["cpu_baseline"] = cpu_baseline(g, dg, args.spread, args.a, args.cpu_seconds, threads)
         log(f"cpu baseline: {out['cpu_baseline']['sample']}")
     dg.free()
@@ -681,8 +866,8 @@ def main():
             f"{k} {v['speedup']:.2f}x" for k, v in out["small_graph_batches"].items()))
     ctx.close()
     if args.serve_rates and not args.no_extras:
-        out["serving"] = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 2, 512,
-                                     args.ways, args.chunk, args.waves)
+        out["serving"] = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 512,
+                                     args.ways, args.chunk, args.waves, args.serve_slots)
         log(f"served/plain = {out['serving']['served_over_plain']:.2f}x")
     print(json.dumps(out), flush=True)
 
