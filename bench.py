@@ -29,14 +29,14 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_name(workload, spread, a):
-    return f"k_count_{workload}_s{spread}_a{a}_traffic.json"
+def traffic_name(workload, spread, a, kernel="k_walk"):
+    return f"{kernel}_{workload}_s{spread}_a{a}_traffic.json"
 
 
-def load_traffic(workload, spread, a):
-    """HBM bytes per counting-kernel launch from the committed rocprofv3 PMC
+def load_traffic(workload, spread, a, kernel="k_walk"):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
     summary, if one exists for this workload and configuration."""
-    path = os.path.join(ROOT, "profiles", traffic_name(workload, spread, a))
+    path = os.path.join(ROOT, "profiles", traffic_name(workload, spread, a, kernel))
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -437,13 +437,18 @@ def tune_sweep(args):
             for a in alens:
                 if kernel == wtc.KERNEL_SPLIT and a == 0:
                     continue
-                for sp in (2, 4, 8, 12, 16):
+                # (the split count's spread shapes only its list part's walk)
+                log(f"tune {workload}: kernel {kernel} a {a}")
+                for sp in ((2, 4, 8, 16, 32, 64, 128) if kernel == wtc.KERNEL_WALK else (8, 12, 16)):
+                    if sp > 8 and W < (1 << 20):
+                        continue
                     t = wtc.Tuned(sp, a, kernel, base.waves_per_block, base.probe_ways, base.split_waves)
                     ctx.use_tuned(t)
                     ms, _ = _time_count(dg, sp, a)
                     if best is None or ms < best[0]:
                         best = (ms, t)
         ms0, t0 = best
+        log(f"tune {workload}: stage 1 best {ms0:.3f} ms {t0.as_dict()}")
         for waves in (2, 4, 8):
             for ways in (2, 4, 8):
                 for sw in ((4, 8, 16) if t0.kernel == wtc.KERNEL_SPLIT else (8,)):
@@ -491,9 +496,8 @@ def tune_sweep(args):
         log(f"tune batch: {rows[-1]}")
     ctx.close()
     # buckets below the smallest measured graph of a family take its row
-    out_dir = os.path.join(PKG, "tuning")
-    os.makedirs(out_dir, exist_ok=True)
-    path = os.path.join(out_dir, "mi355x.tsv")
+    path = args.tune_out or os.path.join(PKG, "tuning", "mi355x.tsv")
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as f:
         f.write("# Autotuner table for MI355X, written by: python bench.py --tune --tune-scales "
                 f"{args.tune_scales} --tune-flat {args.tune_flat} --tune-batch {args.tune_batch}\n")
@@ -581,25 +585,27 @@ def _per_dispatch(directory, kernel_sub, counter):
 
 def pmc_summary(args):
     """Turns four rocprofv3 --pmc runs (FETCH_SIZE and WRITE_SIZE, each on the
-    counting kernel and on the calibration copy) into HBM bytes per counting
-    launch, each counter scaled by what it reads for the copy of known size."""
+    dominant kernel and on the calibration copy) into HBM bytes per launch of
+    that kernel (--pmc-kernel), each counter scaled by what it reads for the
+    copy of known size."""
     kdir_f, kdir_w, cdir_f, cdir_w = args.pmc_summary
+    kern = args.pmc_kernel
     copy_bytes = args.copy_calibration_mb * 2 ** 20
-    kf, nk = _per_dispatch(kdir_f, "k_count", "FETCH_SIZE")
-    kw, _ = _per_dispatch(kdir_w, "k_count", "WRITE_SIZE")
+    kf, nk = _per_dispatch(kdir_f, kern, "FETCH_SIZE")
+    kw, _ = _per_dispatch(kdir_w, kern, "WRITE_SIZE")
     cf, _ = _per_dispatch(cdir_f, "k_copy", "FETCH_SIZE")
     cw, _ = _per_dispatch(cdir_w, "k_copy", "WRITE_SIZE")
     f_scale = copy_bytes / (cf * 1024.0)  # true bytes per counted byte
     w_scale = copy_bytes / (cw * 1024.0)
     fetch = kf * 1024.0 * f_scale
     write = kw * 1024.0 * w_scale
-    out = {"workload": args.pmc_workload, "kernel": "k_count", "dispatches": nk,
+    out = {"workload": args.pmc_workload, "kernel": kern, "dispatches": nk,
            "fetch_kib_per_dispatch": kf, "write_kib_per_dispatch": kw,
            "copy_bytes": copy_bytes, "copy_fetch_kib": cf, "copy_write_kib": cw,
            "fetch_scale": f_scale, "write_scale": w_scale,
            "fetch_bytes": fetch, "write_bytes": write, "bytes_per_launch": fetch + write}
     out["config"] = {"spread": args.spread, "a": args.a}
-    path = os.path.join(ROOT, "profiles", traffic_name(args.pmc_workload, args.spread, args.a))
+    path = os.path.join(ROOT, "profiles", traffic_name(args.pmc_workload, args.spread, args.a, kern))
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
@@ -654,9 +660,11 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--spread", type=int, default=0, help="0: the autotuner's choice")
     ap.add_argument("--a", type=int, default=-1, help="-1: the autotuner's choice")
-    ap.add_argument("--ways", type=int, default=4, help="probes advanced together per lane")
-    ap.add_argument("--chunk", type=int, default=12, help="wedges per lane staged in LDS per phase")
-    ap.add_argument("--waves", type=int, default=4, help="waves per counting block (1, 2, 4 or 8)")
+    ap.add_argument("--ways", type=int, default=None, help="probes advanced together per lane (default: autotuner)")
+    ap.add_argument("--chunk", type=int, default=12, help="wedges per lane staged in LDS per phase (staged kernel)")
+    ap.add_argument("--waves", type=int, default=None, help="waves per counting block, 1/2/4/8 (default: autotuner)")
+    ap.add_argument("--kernel", type=int, default=None, help="0 walk, 1 staged, 2 split (default: autotuner)")
+    ap.add_argument("--split-waves", type=int, default=None, help="waves per row-kernel block, 4/8/16 (default: autotuner)")
     ap.add_argument("--builder", choices=("auto", "host", "device"), default="auto",
                     help="build the oriented CSR on the host or on the GPU (auto: GPU from scale 26)")
     ap.add_argument("--offset-bytes", type=int, default=0)
@@ -669,6 +677,7 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, metavar=("KFETCH", "KWRITE", "CFETCH", "CWRITE"),
                     help="summarise rocprofv3 --pmc output dirs into profiles/ (no GPU use)")
     ap.add_argument("--pmc-workload", default="rmat24")
+    ap.add_argument("--pmc-kernel", default="k_split_rows", help="kernel name the PMC summary covers")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="batched vs plain on this many graphs (0: skip)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the RMAT count: no batched-vs-plain and no serving legs (profiler runs)")
@@ -696,6 +705,8 @@ def main():
     ap.add_argument("--tune-scales", default="12,14,16,18,20,22,24", help="R-MAT scales of the skewed buckets")
     ap.add_argument("--tune-flat", default="14,18,22", help="log2 n of the Erdos-Renyi (flat) buckets")
     ap.add_argument("--tune-batch", type=int, default=4096, help="graphs of the batch bucket (0: skip)")
+    ap.add_argument("--tune-out", default="", help="where --tune writes the table (default: tuning/mi355x.tsv "
+                                                   "beside the library)")
     args = ap.parse_args()
 
     if args.pmc_summary:
@@ -723,7 +734,7 @@ def main():
 
     if args.serve_only:
         res = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 512,
-                          args.ways, args.chunk, args.waves, args.serve_slots)
+                          args.ways or 4, args.chunk, args.waves or 4, args.serve_slots)
         print(json.dumps(res), flush=True)
         return
     if args.reference_configs:
@@ -734,14 +745,14 @@ def main():
         return
     if args.small_batches_only:
         ctx = wtc.Context(0)
-        ctx.set_tuning(args.ways, args.chunk, args.waves)
+        ctx.set_tuning(args.ways or 4, args.chunk, args.waves or 4)
         res = small_kernel_bench(ctx, args.batch_graphs or 4096, 7000)
         ctx.close()
         print(json.dumps(res), flush=True)
         return
     if args.batch_only:
         ctx = wtc.Context(0)
-        ctx.set_tuning(args.ways, args.chunk, args.waves)
+        ctx.set_tuning(args.ways or 4, args.chunk, args.waves or 4)
         res = batch_bench(ctx, args.batch_graphs or 4096, 1000)
         res["small_graph_batches"] = small_kernel_bench(ctx, args.batch_graphs or 4096, 7000)
         ctx.close()
@@ -750,7 +761,6 @@ def main():
 
     workload = f"rmat{args.scale}"
     ctx = wtc.Context(0)
-    ctx.set_tuning(args.ways, args.chunk, args.waves)
     device_build = args.builder == "device" or (args.builder == "auto" and args.scale >= 26)
     t0 = time.perf_counter()
     if device_build:  # generate, sort, deduplicate and orient on the GPU; keep it resident
@@ -772,10 +782,17 @@ def main():
     W = g.wedges()
     log(f"{workload}: n={g.n} m={g.m} W={W} gen {t1 - t0:.1f}s prepro {t2 - t1:.1f}s")
     b_alg, b_min = g.alg_bytes(ob)
-    auto_s, auto_a = wtc.autotune(g.n, g.m, W)
-    args.spread = args.spread or auto_s
-    args.a = auto_a if args.a < 0 else args.a
-    log(f"spread={args.spread} a={args.a} ways={args.ways} chunk={args.chunk} waves={args.waves}")
+    # the autotuner's choice for this graph; flags given on the command line override it
+    tuned = wtc.autotune_shape(g.n, g.m, W)
+    args.spread = args.spread or tuned.spread
+    args.a = tuned.adjacency_matrix_len if args.a < 0 else args.a
+    for k, v in (("kernel", args.kernel), ("waves_per_block", args.waves), ("probe_ways", args.ways),
+                 ("split_waves", args.split_waves)):
+        if v is not None:
+            setattr(tuned, k, v)
+    ctx.use_tuned(tuned)
+    ctx.set_tuning(tuned.probe_ways, args.chunk, tuned.waves_per_block)
+    log(f"spread={args.spread} a={args.a} tuned={tuned.as_dict()} table={wtc.tuning_source()}")
     for _ in range(args.warmup):
         dg.count(args.spread, args.a)
     counts, stats, wall = [], [], []
@@ -792,10 +809,25 @@ def main():
 
     ms_per_step = 1e3 * elapsed / args.steps
     count_ms = statistics.median(s["count_ms"] for s in stats)
-    # algorithmic bytes: the CSR once plus, per wedge, the 4-byte id of its
-    # far end and one 4-byte probe word -- bytes any exact wedge check moves
-    achieved = b_min / (count_ms / 1e3)
-    traffic = load_traffic(workload, args.spread, args.a)
+    split = bool(stats[-1]["split"])
+    if split:
+        # dominant kernel: the split count's row kernel, timed by its own HIP
+        # events on the count's stream.  Its algorithmic bytes: the 4-byte id
+        # of every wedge it checks (each tail id read once; the probe itself
+        # is an LDS read) and every in-edge's (lo, hi) record of 2*ob bytes
+        dom, dom_ms = "k_split_rows", statistics.median(s["rows_ms"] for s in stats)
+        top_w, in_e = stats[-1]["top_wedges"], stats[-1]["in_edges"]
+        dom_bytes = 4.0 * top_w + 2.0 * ob * in_e
+        dom_units = {"wedges_checked": top_w, "in_edges": in_e, "bytes_per_wedge": 4,
+                     "bytes_per_in_edge": 2 * ob}
+    else:
+        # dominant kernel: the counting kernel.  Algorithmic bytes: the CSR once
+        # plus, per wedge, the 4-byte id of its far end and one 4-byte probe word
+        dom, dom_ms = "k_walk", count_ms
+        dom_bytes = b_min
+        dom_units = {"wedges_checked": W, "bytes_per_wedge": 8}
+    achieved = dom_bytes / (dom_ms / 1e3)
+    traffic = load_traffic(workload, args.spread, args.a, dom)
     golden = load_golden_rmat(args.scale, args.edge_factor, args.seed)
     out = {
         "metric": "wedge checks/s & edges/s at RMAT-24 (1 GPU, exact); batched vs plain graphs/s",
@@ -817,24 +849,28 @@ def main():
         "triangles": counts[0],
         "kernel_ms": {"count": count_ms,
                       "prefix": statistics.median(s["prefix_ms"] for s in stats),
-                      "bitmap": statistics.median(s["bitmap_ms"] for s in stats),
+                      "bitmap_or_transposition": statistics.median(s["bitmap_ms"] for s in stats),
+                      "split_rows": statistics.median(s["rows_ms"] for s in stats) if split else None,
                       "grid": stats[0]["blocks"]},
         "wedge_checks_per_s_kernel": W / (count_ms / 1e3),
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": traffic["bytes_per_launch"] if traffic else None,
-                     "algorithmic_bytes": b_min, "bytes_per_wedge": 8,
-                     "kernel_ms": count_ms,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": traffic["bytes_per_launch"] if traffic else None,
+                     "algorithmic_bytes": dom_bytes, "units": dom_units, "kernel_ms": dom_ms,
+                     # the whole count (every counting kernel) against the floor of one id and one
+                     # probe word per wedge, B_min
+                     "b_min": b_min, "count_frac_b_min": b_min / (count_ms / 1e3) / HBM_PEAK,
                      # the reference's own probe words (every binary-search step one word): an upper
                      # figure, most of whose words a bitmap or the caches never fetch from HBM
                      "b_ref_alg": b_alg,
                      # measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated on
                      # a copy of known size) over the same launch time: the rate the memory ran at
-                     "traffic_GBps": traffic["bytes_per_launch"] / (count_ms / 1e3) / 1e9 if traffic else None,
-                     "traffic_frac": traffic["bytes_per_launch"] / (count_ms / 1e3) / HBM_PEAK if traffic else None,
-                     "traffic_profile": traffic_name(workload, args.spread, args.a) if traffic else None},
+                     "traffic_GBps": traffic["bytes_per_launch"] / (dom_ms / 1e3) / 1e9 if traffic else None,
+                     "traffic_frac": traffic["bytes_per_launch"] / (dom_ms / 1e3) / HBM_PEAK if traffic else None,
+                     "traffic_profile": traffic_name(workload, args.spread, args.a, dom) if traffic else None},
         "prepro_s": t2 - t1,
         "builder": "device" if device_build else "host",
-        "tuning": {"probe_ways": args.ways, "lds_chunk": args.chunk, "waves_per_block": args.waves},
+        "tuning": dict(tuned.as_dict(), table=wtc.tuning_source()),
     }
     if golden is not None:
         out["triangles_expected"] = golden
@@ -867,7 +903,7 @@ def main():
     ctx.close()
     if args.serve_rates and not args.no_extras:
         out["serving"] = serve_bench([float(x) for x in args.serve_rates.split(",")], args.serve_seconds, 512,
-                                     args.ways, args.chunk, args.waves, args.serve_slots)
+                                     args.ways or 4, args.chunk, args.waves or 4, args.serve_slots)
         log(f"served/plain = {out['serving']['served_over_plain']:.2f}x")
     print(json.dumps(out), flush=True)
 

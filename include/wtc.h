@@ -194,15 +194,20 @@ typedef struct wtc_ctx wtc_ctx;       /* HIP stream + reusable workspace */
 typedef struct wtc_dgraph wtc_dgraph; /* oriented CSR resident in HBM    */
 
 typedef struct {
-    double prefix_ms;  /* wedge prefix scan + setup                          */
-    double bitmap_ms;  /* bitmap clear + build                               */
-    double count_ms;   /* counting kernel                                    */
+    double prefix_ms;  /* wedge prefix scan + setup (split count: its three
+                          prefixes, range map and setup)                      */
+    double bitmap_ms;  /* bitmap clear + build (split count: the counting
+                          sort of the top rows' in-edges)                     */
+    double count_ms;   /* counting kernel(s) (split count: row kernel + walk) */
     double total_ms;   /* first launch to count back on the host             */
     uint64_t wedges;   /* W counted                                          */
     uint32_t blocks;   /* counting-kernel grid                               */
     uint32_t launches; /* counting-kernel launches                           */
     uint32_t small_graphs; /* graphs counted by the small-graph kernel       */
-    uint32_t reserved;
+    uint32_t split;    /* 1: counted by the split count                      */
+    double rows_ms;    /* split count: its row kernel alone                  */
+    uint64_t top_wedges; /* split count: wedges probed in LDS (w in the top rows) */
+    uint64_t in_edges; /* split count: in-edge tails the row kernel streamed */
 } wtc_count_stats;
 
 int wtc_device_count(int *count);
@@ -242,8 +247,8 @@ int wtc_ctx_set_count_kernel(wtc_ctx *ctx, int kernel);
  * of 64*s wedges, and a wave takes up to tiles_per_grab consecutive tiles
  * per atomic grab (1..65535; 0 = 32).  deal_to_xcds != 0 deals super-chunks
  * of grabs to the 8 XCDs in turn, each XCD taking from its own counter;
- * 0 gives one stream of grabs for the whole grid.  Counts do not depend on
- * these. */
+ * 0 gives one stream of grabs for the whole grid.  Default 4 tiles per grab,
+ * one stream.  Counts do not depend on these. */
 int wtc_ctx_set_work_split(wtc_ctx *ctx, uint32_t tiles_per_grab, int deal_to_xcds);
 /* Kernel choice for sets of small graphs (the role of the vertex- and
  * edge-parallel kernels, experiments/tc_vertex.cu:113-274 and

@@ -102,6 +102,21 @@ int validate_oriented_par(uint64_t n, uint64_t m, const OffT *rowPtr, const uint
     return bad ? WTC_ERR_CSR : WTC_OK;
 }
 
+// W = sum over rows of C(d,2) (tc.cu:348-361), over host threads.
+template <class OffT>
+uint64_t wedge_total_par(uint64_t n, const OffT *rowPtr) {
+    std::atomic<uint64_t> W{0};
+    par_chunks(n, 1 << 16, [&](uint64_t b, uint64_t e, int) {
+        uint64_t w = 0;
+        for (uint64_t v = b; v < e; v++) {
+            const uint64_t d = (uint64_t)(rowPtr[v + 1] - rowPtr[v]);
+            w += d * (d - (d > 0)) / 2;
+        }
+        W += w;
+    });
+    return W.load();
+}
+
 }  // namespace wtc
 
 namespace wtc {

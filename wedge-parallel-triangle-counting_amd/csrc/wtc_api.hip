@@ -93,12 +93,7 @@ void Slot::release_device() {
     bm.release();
     desc.release();
     out.release();
-    sp_kpos.release();
-    sp_wsfull.release();
-    sp_hist.release();
-    sp_tot.release();
-    sp_rpin.release();
-    sp_ie.release();
+    sp.release();
 }
 
 void Slot::destroy() {
@@ -153,8 +148,75 @@ static uint32_t small_parts_of(uint64_t W) {
 static int ways_index(int w) { return w == 1 ? 0 : w == 2 ? 1 : w == 4 ? 2 : 3; }
 static int wpb_index(int v) { return v == 1 ? 0 : v == 2 ? 1 : v == 4 ? 2 : 3; }
 
+namespace {
+
+// Byte offsets of the pieces of a split count's workspace in one buffer
+// (each on a 256-byte boundary), and the buffer's size.
+struct SplitLayout {
+    size_t kpos, wsfull, hist1, part_base, c2base, hist2, tot, recid, rec, rpin, ie, total;
+    uint32_t chunks, parts, c2cap;
+    uint64_t cap;
+};
+
+SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
+    SplitLayout L{};
+    L.chunks = std::min<uint32_t>(kSplitChunksMax, std::max<uint32_t>(1, n / kSplitChunkVerts));
+    L.parts = (rows + kSplitPart - 1) / kSplitPart;
+    L.cap = std::max<uint64_t>(m, 1);  // an in-edge is an edge
+    L.c2cap = (uint32_t)((L.cap + kSplitChunk2 - 1) / kSplitChunk2) + L.parts;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    L.kpos = take((size_t)n * sizeof(uint32_t));
+    L.wsfull = take(2 * ((size_t)n + 1) * sizeof(uint64_t));  // ws_full, then ie_pre
+    L.hist1 = take((size_t)L.chunks * L.parts * sizeof(uint64_t));
+    L.part_base = take(((size_t)L.parts + 1) * sizeof(uint64_t));
+    L.c2base = take(((size_t)L.parts + 1) * sizeof(uint32_t));
+    L.hist2 = take((size_t)L.c2cap * kSplitPart * sizeof(uint32_t));
+    L.tot = take((size_t)rows * sizeof(uint32_t));
+    L.recid = take((size_t)L.cap * sizeof(uint32_t));
+    L.rec = take((size_t)L.cap * 2 * (size_t)ob);
+    L.rpin = take(((size_t)rows + 1) * (size_t)ob);
+    L.ie = take((size_t)L.cap * 2 * (size_t)ob);
+    L.total = off;
+    return L;
+}
+
+}  // namespace
+
+size_t split_workspace_bytes(uint32_t n, uint64_t m, uint32_t rows, int ob) { return split_plan(n, m, rows, ob).total; }
+
+void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t rows, int ob) {
+    const SplitLayout L = split_plan(n, m, rows, ob);
+    sd->rows = rows;
+    sd->chunks = L.chunks;
+    sd->parts = L.parts;
+    sd->cap = L.cap;
+    sd->c2cap = L.c2cap;
+    sd->kpos = reinterpret_cast<uint32_t *>(base + L.kpos);
+    sd->ws_full = reinterpret_cast<uint64_t *>(base + L.wsfull);
+    sd->ie_pre = sd->ws_full + (size_t)n + 1;
+    sd->hist1 = reinterpret_cast<uint64_t *>(base + L.hist1);
+    sd->part_base = reinterpret_cast<uint64_t *>(base + L.part_base);
+    sd->c2base = reinterpret_cast<uint32_t *>(base + L.c2base);
+    sd->hist2 = reinterpret_cast<uint32_t *>(base + L.hist2);
+    sd->row_tot = reinterpret_cast<uint32_t *>(base + L.tot);
+    sd->rec_id = reinterpret_cast<uint32_t *>(base + L.recid);
+    sd->rec = base + L.rec;
+    sd->rp_in = base + L.rpin;
+    sd->ie = base + L.ie;
+}
+
+bool split_eligible(const Tuning &t, const GraphDesc &d) {
+    return t.kernel == kKernelSplit && d.mode == kModeTiles && d.a > 0 && d.bm_rows > 0 &&
+           d.bm_rows <= kSplitPartsMax * kSplitPart && (reinterpret_cast<uintptr_t>(d.colInd) & 15u) == 0;
+}
+
 int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan, const uint64_t *m,
-              const uint64_t *W) {
+              const uint64_t *W, char *split_base) {
     const uint32_t G = (uint32_t)hd.size();
     if (G == 0) return WTC_ERR_ARG;
     // Kernel choice per set: a set of two or more graphs that all fit the
@@ -186,8 +248,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     // The split count takes a single graph with a bitmap length whose colInd
     // is 16-byte aligned (its row kernel loads four ids at a time); its top
     // rows then need no bitmap in HBM.
-    const bool split = t.kernel == kKernelSplit && G == 1 && hd[0].mode == kModeTiles && hd[0].a > 0 &&
-                       hd[0].bm_rows > 0 && m != nullptr && (reinterpret_cast<uintptr_t>(hd[0].colInd) & 15u) == 0;
+    const bool split = G == 1 && m != nullptr && split_eligible(t, hd[0]) && (!t.split_needs_base || split_base);
     const uint32_t split_rows = split ? hd[0].bm_rows : 0;
     if (split) {
         hd[0].bm_rows = 0;
@@ -238,24 +299,12 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     }
     SplitDesc sd{};
     if (split) {
-        const uint32_t n = hd[0].n;
-        sd.rows = split_rows;
-        sd.chunks = std::min<uint32_t>(kSplitChunksMax, std::max<uint32_t>(1, n / kSplitChunkVerts));
-        sd.parts = (split_rows + kSplitPart - 1) / kSplitPart;
-        sd.cap = std::max<uint64_t>(m[0], 1);  // an in-edge is an edge
-        WTC_HIPC(s.sp_kpos.ensure((size_t)n * sizeof(uint32_t)));
-        WTC_HIPC(s.sp_wsfull.ensure(2 * ((size_t)n + 1) * sizeof(uint64_t)));
-        WTC_HIPC(s.sp_hist.ensure((size_t)sd.chunks * split_rows * sizeof(uint32_t)));
-        WTC_HIPC(s.sp_tot.ensure((size_t)split_rows * sizeof(uint32_t)));
-        WTC_HIPC(s.sp_rpin.ensure(((size_t)split_rows + 1) * (size_t)ob));
-        WTC_HIPC(s.sp_ie.ensure((size_t)sd.cap * 2 * (size_t)ob));
-        sd.kpos = s.sp_kpos.as<uint32_t>();
-        sd.ws_full = s.sp_wsfull.as<uint64_t>();
-        sd.ie_pre = sd.ws_full + (size_t)n + 1;
-        sd.hist = s.sp_hist.as<uint32_t>();
-        sd.row_tot = s.sp_tot.as<uint32_t>();
-        sd.rp_in = s.sp_rpin.p;
-        sd.ie = s.sp_ie.p;
+        char *base = split_base;
+        if (!base) {
+            WTC_HIPC(s.sp.ensure(split_workspace_bytes(hd[0].n, m[0], split_rows, ob)));
+            base = s.sp.as<char>();
+        }
+        split_layout(&sd, base, hd[0].n, m[0], split_rows, ob);
         WTC_HIPC(split_rows_grid(ob, split_rows, t.split_wpb, t.num_cus, &s.rows_grid));
     }
     std::memcpy(s.h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
@@ -275,6 +324,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     if (!gc) WTC_HIPC(count_grid(kernel, ob, staged, t.ways, t.wpb, t.num_cus, &gc));
     s.G = G;
     s.grid = gc;
+    s.split = split;
     s.small = small_n;
     plan->G = G;
     plan->ob = ob;
@@ -296,7 +346,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->split_off = split_off;
     plan->split_rows = split_rows;
     plan->split_chunks = sd.chunks;
-    plan->split_parts = sd.parts;
+    plan->split_c2cap = sd.c2cap;
     plan->split_wpb = t.split_wpb;
     return WTC_OK;
 }
@@ -313,11 +363,12 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
         WTC_HIPC(launch_split_range(ddesc, sd, s.stream));
         WTC_HIPC(launch_setup(ddesc, 1, tree, meta, dcounts, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
-        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_parts, p.split_rows, meta,
+        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_rows, meta,
                                         s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
         WTC_HIPC(launch_split_rows(ddesc, sd, p.ob, p.split_rows, p.split_wpb, meta, dcounts, s.rows_grid,
                                    s.stream));
+        if (timed) WTC_HIPC(hipEventRecord(s.ev[4], s.stream));
         WTC_HIPC(launch_count(p.kernel, ddesc, 1, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways,
                               p.wpb, p.work, s.grid, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
@@ -343,9 +394,10 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
     return WTC_OK;
 }
 
-int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, const uint64_t *m, const uint64_t *W) {
+int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, const uint64_t *m, const uint64_t *W,
+                char *split_base) {
     SetPlan p;
-    int rc = stage_set(s, t, ob, hd, &p, m, W);
+    int rc = stage_set(s, t, ob, hd, &p, m, W, split_base);
     if (rc) return rc;
     WTC_HIPC(hipMemcpyAsync(s.desc.p, s.h_desc.p, p.desc_bytes, hipMemcpyHostToDevice,
                             s.stream));
@@ -377,6 +429,12 @@ int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt) {
         WTC_HIPC(hipEventElapsedTime(&rt->prefix_ms, s.ev[0], s.ev[1]));
         WTC_HIPC(hipEventElapsedTime(&rt->bitmap_ms, s.ev[1], s.ev[2]));
         WTC_HIPC(hipEventElapsedTime(&rt->count_ms, s.ev[2], s.ev[3]));
+        rt->split = s.split;
+        if (s.split) {
+            WTC_HIPC(hipEventElapsedTime(&rt->rows_ms, s.ev[2], s.ev[4]));
+            rt->top_wedges = h[kMetaTopWedges];
+            rt->in_edges = h[kMetaInEdges];
+        }
     }
     return WTC_OK;
 }
@@ -437,6 +495,7 @@ int run_set(wtc_ctx *c, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, wt
     if (rc) return rc;
     if (rt_out) *rt_out = rt;
     if (st) {
+        *st = wtc_count_stats{};
         st->prefix_ms = rt.prefix_ms;
         st->bitmap_ms = rt.bitmap_ms;
         st->count_ms = rt.count_ms;
@@ -445,6 +504,10 @@ int run_set(wtc_ctx *c, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, wt
         st->blocks = c->slot.grid;
         st->launches = 1;
         st->small_graphs = c->slot.small;
+        st->split = rt.split ? 1u : 0u;
+        st->rows_ms = rt.rows_ms;
+        st->top_wedges = rt.top_wedges;
+        st->in_edges = rt.in_edges;
     }
     return WTC_OK;
 }
@@ -466,6 +529,7 @@ int default_context(wtc_ctx **out) {
     if (!g_default) {
         int rc = wtc_ctx_create(0, &g_default);
         if (rc) return rc;
+        g_default->tune.auto_shape = std::getenv("WTC_COUNT_KERNEL") == nullptr;
     }
     *out = g_default;
     return WTC_OK;
@@ -474,6 +538,20 @@ int default_context(wtc_ctx **out) {
 }  // namespace wtc
 
 namespace {
+
+// A tuned shape's kernel, block waves, probe ways and row-kernel waves on a
+// context.
+int apply_tuned(wtc_ctx *c, const wtc_tuned &t) {
+    if (!valid_ways(t.probe_ways) || !valid_waves(t.waves_per_block) || !valid_split_waves(t.split_waves) ||
+        (t.kernel != WTC_KERNEL_WALK && t.kernel != WTC_KERNEL_STAGED && t.kernel != WTC_KERNEL_SPLIT))
+        return WTC_ERR_ARG;
+    c->tune.kernel = t.kernel == WTC_KERNEL_WALK ? kKernelWalk : t.kernel == WTC_KERNEL_STAGED ? kKernelStaged
+                                                                                              : kKernelSplit;
+    c->tune.ways = t.probe_ways;
+    c->tune.wpb = t.waves_per_block;
+    c->tune.split_wpb = t.split_waves;
+    return WTC_OK;
+}
 
 // tc_GPU body for either offset width: per-call allocation, copy, device
 // prefix/tree/bitmap, count, copy back, free (tc.cu:315-441).
@@ -491,6 +569,14 @@ int tc_gpu_impl(uint32_t n, uint64_t m, const OffT *rowPtr, const uint32_t *colI
     wtc_ctx *c;
     if ((rc = default_context(&c))) return rc;
     WTC_HIPC(hipSetDevice(c->device));
+    // the caller's s and a, with the autotuner's kernel and block shape for
+    // the graph's size bucket (unless $WTC_COUNT_KERNEL fixed the kernel)
+    if (c->tune.auto_shape) {
+        wtc_tuned tuned;
+        if (wtc_autotune_shape(n, m, wedge_total_par<OffT>(n, rowPtr), 0, &tuned) == WTC_OK &&
+            (rc = apply_tuned(c, tuned)))
+            return rc;
+    }
     hipStream_t st = c->slot.stream;
     DevBuf drow, dcol;
     auto cleanup = [&] {  // every device buffer is freed at the end of the call, like the reference
@@ -613,15 +699,8 @@ int wtc_ctx_set_count_kernel(wtc_ctx *c, int kernel) {
 }
 
 int wtc_ctx_use_tuned(wtc_ctx *c, const wtc_tuned *t) {
-    if (!c || !t || !valid_ways(t->probe_ways) || !valid_waves(t->waves_per_block) ||
-        !valid_split_waves(t->split_waves))
-        return WTC_ERR_ARG;
-    int rc = wtc_ctx_set_count_kernel(c, t->kernel);
-    if (rc) return rc;
-    c->tune.ways = t->probe_ways;
-    c->tune.wpb = t->waves_per_block;
-    c->tune.split_wpb = t->split_waves;
-    return WTC_OK;
+    if (!c || !t) return WTC_ERR_ARG;
+    return apply_tuned(c, *t);
 }
 
 void wtc_ctx_destroy(wtc_ctx *c) {
@@ -1142,6 +1221,7 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
     const uint64_t *h = sl.h_out.as<uint64_t>();
     for (uint32_t g = 0; g < b->G; g++) triangles[g] = h[kMetaSlots + g];
     if (st) {
+        *st = wtc_count_stats{};
         float ms = 0;
         WTC_HIPC(hipEventElapsedTime(&ms, sl.ev[0], sl.ev[3]));
         st->prefix_ms = 0;

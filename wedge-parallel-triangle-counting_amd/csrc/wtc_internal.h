@@ -81,15 +81,16 @@ struct PinBuf : OwnedBuf<PinnedFree> {
 // workspace and the pinned staging of one in-flight graph-set count.
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {};  // prefix start, bitmap start, count start, count end
+    hipEvent_t ev[5] = {};  // prefix start, bitmap start, count start, count end, split row kernel end
     hipEvent_t ev_ready = nullptr;  // input copied (waited on by the slot's stream)
     DevBuf ws, partials, tree, bm, desc, out;
     // split count: kpos, full prefix, transposition counts/totals/offsets, in-edge tails
-    DevBuf sp_kpos, sp_wsfull, sp_hist, sp_tot, sp_rpin, sp_ie;
+    DevBuf sp;  // split count workspace (split_layout)
     PinBuf h_desc, h_out;
     uint32_t G = 0;       // graphs of the last enqueue
     uint32_t grid = 0;    // counting grid of the last enqueue
     uint32_t rows_grid = 0;  // row-kernel grid of the last split count
+    bool split = false;      // the last enqueue was a split count
     uint32_t small = 0;   // graphs of the last enqueue taken by the small-graph kernel
     int create();
     void release_device();  // frees the device workspace (keeps stream/events)
@@ -103,8 +104,17 @@ struct Tuning {
     int wpb = 4;          // waves per counting block: 1, 2, 4 or 8
     int kernel = 0;       // kKernelWalk, kKernelStaged or kKernelSplit
     int split_wpb = 8;    // waves per block of the split count's row kernel
-    uint32_t grab = 0;    // most tiles per grab (0: the kernel's default, 32)
-    bool deal = true;     // deal super-chunks of grabs to the XCDs in turn
+    // the process context of the tc_GPU drop-ins takes the autotuner's
+    // kernel and block shape per graph unless $WTC_COUNT_KERNEL names one
+    bool auto_shape = false;
+    // the split count only with a workspace the caller provides (the
+    // server carves it from its HBM pool), never from the slot
+    bool split_needs_base = false;
+    // work split of the counting kernels: at most 4 tiles per grab, one
+    // stream of grabs (measured best on R-MAT-24, where the cost per wedge
+    // varies along the order and bigger grabs leave the slowest waves last)
+    uint32_t grab = 4;    // most tiles per grab (0: the kernel's default, 32)
+    bool deal = false;    // deal super-chunks of grabs to the XCDs in turn
     uint32_t chunk = 16;  // wedges per lane staged in LDS per phase
     // In sets of two or more graphs, a graph with 2(n+1) + m <= small_words
     // is counted by the small-graph kernel from LDS (0: never);
@@ -125,6 +135,9 @@ int init_small_graphs(Tuning &t, const hipDeviceProp_t &prop, uint32_t want);
 
 struct RunTimes {
     float prefix_ms = 0, bitmap_ms = 0, count_ms = 0;
+    float rows_ms = 0;  // split count: its row kernel
+    bool split = false;
+    uint64_t top_wedges = 0, in_edges = 0;
 };
 
 uint64_t bm_words_for(uint32_t a);
@@ -152,14 +165,14 @@ struct SetPlan {
     // split count: SplitDesc at byte split_off of the descriptor buffer
     bool split = false;
     size_t split_off = 0;
-    uint32_t split_rows = 0, split_chunks = 0, split_parts = 0, split_wpb = 8;
+    uint32_t split_rows = 0, split_chunks = 0, split_c2cap = 0, split_wpb = 8;
 };
 // Sizes the slot's workspace for the set, assigns each graph its prefix and
 // bitmap (unless given), fills the pinned host descriptors and the plan.
 // m and W (edges and wedges per graph, or null) let a set whose graphs all
 // fit the tuning's small-graph limit take the small-graph kernel.
 int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan,
-              const uint64_t *m = nullptr, const uint64_t *W = nullptr);
+              const uint64_t *m = nullptr, const uint64_t *W = nullptr, char *split_base = nullptr);
 // Enqueues the set's kernels and the copy of meta + counts to the host; no
 // host-to-device copies, allocations or memsets, so it can be captured into
 // a graph once the descriptors are on the device.  timed: record ev[0..3].
@@ -169,7 +182,15 @@ int launch_set(Slot &s, const SetPlan &p, bool timed);
 // into the slot's pinned buffer; returns without waiting.  All graphs of a
 // set share the row-offset width ob.
 int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, const uint64_t *m = nullptr,
-                const uint64_t *W = nullptr);
+                const uint64_t *W = nullptr, char *split_base = nullptr);
+// The split count (a single graph with a bitmap length; wtc_split.hip):
+// whether the tuning takes it for a graph, its workspace's device bytes for
+// a graph of n vertices, m edges and `rows` top rows at offset width ob, and
+// the SplitDesc pointers into a workspace of that size at base.  stage_set
+// takes the workspace from split_base when given, else from the slot.
+bool split_eligible(const Tuning &t, const GraphDesc &d);
+size_t split_workspace_bytes(uint32_t n, uint64_t m, uint32_t rows, int ob);
+void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t rows, int ob);
 // Waits for the slot (or, if it is done, does not wait) and reads results.
 int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt);
 // True when the slot's last enqueue has completed.

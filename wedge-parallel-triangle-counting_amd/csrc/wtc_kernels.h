@@ -77,38 +77,57 @@ static_assert(sizeof(GraphDesc) == 128, "GraphDesc layout");
 
 // meta[] slots written by the setup kernel
 // (kMetaNext..kMetaNext+7: per-XCD work counters of the counting kernel;
-// kMetaRows: tile counter of the split count's row kernel)
-enum : int { kMetaTiles = 0, kMetaWedges = 1, kMetaRows = 2, kMetaNext = 8, kMetaSlots = 16 };
+// kMetaRows: tile counter of the split count's row kernel; kMetaTopWedges,
+// kMetaInEdges: the split count's wedges probed in LDS and in-edge tails)
+enum : int {
+    kMetaTiles = 0,
+    kMetaWedges = 1,
+    kMetaRows = 2,
+    kMetaTopWedges = 3,
+    kMetaInEdges = 4,
+    kMetaNext = 8,
+    kMetaSlots = 16
+};
 
-// Split count of one graph (wtc_split.hip): top rows per transposition part.
+// Split count of one graph (wtc_split.hip).  Top rows per part of the
+// transposition's second level (its LDS counters), the most parts, records
+// per second-level block, threads per transposition block, and in-edges per
+// tile of the row kernel.
 constexpr uint32_t kSplitPart = 32768;
-// Threads of a transposition block (its LDS holds one part's counters).
+constexpr uint32_t kSplitPartsMax = 64;
+constexpr uint32_t kSplitChunk2 = 262144;
 constexpr int kSplitBlock = 1024;
-// In-edges per tile of the row kernel.
 constexpr uint32_t kSplitTile = 1024;
 
 // Workspace and device-filled state of a split count.  The top rows are the
 // ranks [thr, n) of the graph's GraphDesc; OffT arrays use the set's offset
-// width.
+// width.  The in-edges of the top rows are grouped by row in two levels:
+// first into one bucket per part of kSplitPart rows (records: id and tail),
+// then each part's records by row (the tails only, in ie).
 struct SplitDesc {
-    uint32_t *kpos;     // n: entries of adj+(v) below thr
-    uint64_t *ws_full;  // n+1: the full wedge prefix (ws of the GraphDesc holds the list part's)
-    uint64_t *ie_pre;   // n+1: prefix of the top in-edges (entries >= thr, not last) of each row
-    uint32_t *hist;     // chunks x rows: per-chunk in-edge counts, then their column prefix
-    uint32_t *row_tot;  // rows: in-edges of each top row
-    void *rp_in;        // rows+1 OffT: first in-edge of each top row
-    void *ie;           // cap pairs of OffT: the tail [lo, hi) of colInd of each in-edge
-    uint64_t cap;       // pairs ie holds (>= the in-edges of the top rows)
-    uint32_t rows;      // top rows: n - thr
-    uint32_t chunks;    // vertex chunks of the transposition (about equal shares of the in-edges)
-    uint32_t parts;     // row parts of kSplitPart rows
-    uint32_t pad0_;
+    uint32_t *kpos;       // n: entries of adj+(v) below thr
+    uint64_t *ws_full;    // n+1: the full wedge prefix (ws of the GraphDesc holds the list part's)
+    uint64_t *ie_pre;     // n+1: prefix of the top in-edges (entries >= thr, not last) of each row
+    uint64_t *hist1;      // chunks x parts: first-level counts, then record offsets
+    uint64_t *part_base;  // parts+1: first record of each part; part_base[parts] = the in-edges
+    uint32_t *c2base;     // parts+1: first second-level block of each part
+    uint32_t *hist2;      // c2cap x kSplitPart: second-level counts, then column prefixes
+    uint32_t *row_tot;    // rows: in-edges of each top row
+    uint32_t *rec_id;     // cap: first-level records, the in-edge's id
+    void *rec;            // cap pairs of OffT: first-level records, the in-edge's tail
+    void *rp_in;          // rows+1 OffT: first in-edge of each top row
+    void *ie;             // cap pairs of OffT: the tail [lo, hi) of colInd of each in-edge, by row
+    uint64_t cap;         // in-edges the record and ie arrays hold (>= the top in-edges)
+    uint32_t rows;        // top rows: n - thr
+    uint32_t chunks;      // vertex chunks of the first level (about equal shares of the in-edges)
+    uint32_t parts;       // row parts of kSplitPart rows (<= kSplitPartsMax)
+    uint32_t c2cap;       // second-level blocks launched (>= those the records need)
     // device-filled by the range kernel
     uint64_t w_lo, w_hi;  // the counted wedges of the full order, clamped to [0, W]
     uint32_t v_lo, v_hi;  // vertices holding w_lo and w_hi - 1 (v_lo > v_hi: none)
     uint32_t clip;        // 1 unless [w_lo, w_hi) is the whole order
     uint32_t pad_;
 };
-static_assert(sizeof(SplitDesc) == 112, "SplitDesc layout");
+static_assert(sizeof(SplitDesc) == 152, "SplitDesc layout");
 
 }  // namespace wtc

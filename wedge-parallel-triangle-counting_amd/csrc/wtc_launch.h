@@ -59,9 +59,10 @@ hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int o
                                uint64_t *partials, hipStream_t st);
 // Maps the GraphDesc's range onto the list part (before launch_setup).
 hipError_t launch_split_range(GraphDesc *desc, SplitDesc *sd, hipStream_t st);
-// The counting sort of the top rows' in-edges; zeroes meta[kMetaRows].
+// The two-level counting sort of the top rows' in-edges (chunks first-level
+// blocks, c2cap second-level blocks); zeroes meta[kMetaRows].
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
-                                  uint32_t parts, uint32_t rows, uint64_t *meta, hipStream_t st);
+                                  uint32_t c2cap, uint32_t rows, uint64_t *meta, hipStream_t st);
 // Waves per block of the row kernel: 4, 8 or 16.
 bool valid_split_waves(int wpb);
 // Persistent grid of the row kernel for `rows` top rows (its LDS bitmap).

@@ -85,8 +85,9 @@ struct SrvGraph {
     uint64_t id = 0;
     uint32_t n = 0;
     uint64_t m = 0;
-    PinBuf host;        // rowPtr as u32 (n+1), then colInd (m)
-    size_t bytes = 0;   // device footprint of the CSR
+    PinBuf host;        // rowPtr as u32 (n+1), then colInd (m) from byte col_off
+    size_t col_off = 0; // (n+1)*4 rounded up to 16 bytes: colInd 16-byte aligned
+    size_t bytes = 0;   // device footprint of the CSR (with 256 bytes of slack)
     size_t off = SIZE_MAX;
     hipEvent_t uploaded = nullptr;
     bool uploading = false, resident = false;
@@ -95,6 +96,7 @@ struct SrvGraph {
     uint64_t first_rid = UINT64_MAX;  // oldest queued request naming it
     uint64_t W = 0;                    // wedges
     uint64_t cost = 0;                 // batching cost class: max(W, kCostFloor)
+    bool split = false;                // the autotuner counts it by the split count (given a > 0)
 };
 
 enum ReqState { kQueued, kRunning, kDone, kFailed };
@@ -102,6 +104,7 @@ enum ReqState { kQueued, kRunning, kDone, kFailed };
 struct SrvReq {
     uint64_t id = 0, gid = 0;
     uint32_t spread = 0, a = 0;
+    bool split = false;  // counted alone by the split count; bm_* then hold its workspace
     ReqState state = kQueued;
     size_t ws_off = SIZE_MAX, ws_bytes = 0, bm_off = SIZE_MAX, bm_bytes = 0;
     uint64_t result = 0;
@@ -147,6 +150,13 @@ char *pool_ptr(wtc_server *s, size_t off) { return s->pool_mem.as<char>() + off;
 
 size_t req_ws_bytes(const SrvGraph *g) { return ((size_t)g->n + 1) * sizeof(uint64_t); }
 size_t req_bm_bytes(uint32_t a) { return a ? bm_words_for(a) * sizeof(uint32_t) : 0; }
+// A request takes the split count when its graph's autotuned kernel is the
+// split count and it asks for a bitmap; its bitmap bytes are then the split
+// workspace (no bitmap in HBM).
+bool req_split(const SrvGraph *g, uint32_t a) { return g->split && a > 0; }
+size_t req_extra_bytes(const SrvGraph *g, uint32_t a) {
+    return req_split(g, a) ? split_workspace_bytes(g->n, g->m, std::min(g->n, a), 4) : req_bm_bytes(a);
+}
 
 // Frees the CSRs of idle resident graphs, least recently used first, until
 // `bytes` could be allocated for request `rid`.  Graphs that an older queued
@@ -222,8 +232,7 @@ bool schedule(wtc_server *s) {
         if (off == SIZE_MAX) break;  // the pool is full of graphs in use: wait for completions
         g->off = off;
         char *dst = pool_ptr(s, off);
-        hipError_t e = hipMemcpyAsync(dst, g->host.p, ((size_t)g->n + 1) * 4 + g->m * 4, hipMemcpyHostToDevice,
-                                      s->upload);
+        hipError_t e = hipMemcpyAsync(dst, g->host.p, g->col_off + g->m * 4, hipMemcpyHostToDevice, s->upload);
         if (e == hipSuccess) e = hipEventRecord(g->uploaded, s->upload);
         if (e != hipSuccess) {
             s->fatal = hip_fail("server upload", e);
@@ -241,10 +250,16 @@ bool schedule(wtc_server *s) {
         std::vector<GraphDesc> hd;
         std::vector<uint64_t> hm, hw;  // edges and wedges per graph of hd
         uint64_t c0 = 0;
+        char *split_base = nullptr;
         for (auto it = s->queue.begin(); it != s->queue.end() && hd.size() < s->cfg.max_batch;) {
             SrvReq *r = *it;
             SrvGraph *g = s->graphs[r->gid];
             if (!g->resident) {
+                ++it;
+                continue;
+            }
+            // a split request runs alone: it starts an empty batch, or waits
+            if (r->split && !hd.empty()) {
                 ++it;
                 continue;
             }
@@ -254,7 +269,7 @@ bool schedule(wtc_server *s) {
                 continue;
             }
             r->ws_bytes = req_ws_bytes(g);
-            r->bm_bytes = req_bm_bytes(r->a);
+            r->bm_bytes = req_extra_bytes(g, r->a);
             g->pins++;  // keep the graph while its own workspace is being carved out
             r->ws_off = alloc_evicting(s, r->ws_bytes, r->id);
             r->bm_off = r->bm_bytes && r->ws_off != SIZE_MAX ? alloc_evicting(s, r->bm_bytes, r->id) : SIZE_MAX;
@@ -267,10 +282,11 @@ bool schedule(wtc_server *s) {
             }
             g->last_use = ++s->tick;
             const char *base = pool_ptr(s, g->off);
-            GraphDesc d = make_desc(base, reinterpret_cast<const uint32_t *>(base + ((size_t)g->n + 1) * 4), g->n,
-                                    r->spread, r->a, 0, UINT64_MAX);
+            GraphDesc d = make_desc(base, reinterpret_cast<const uint32_t *>(base + g->col_off), g->n, r->spread,
+                                    r->a, 0, UINT64_MAX);
             d.ws = reinterpret_cast<uint64_t *>(pool_ptr(s, r->ws_off));
-            d.bitmap = r->bm_bytes ? reinterpret_cast<uint32_t *>(pool_ptr(s, r->bm_off)) : nullptr;
+            if (r->split) split_base = pool_ptr(s, r->bm_off);
+            else d.bitmap = r->bm_bytes ? reinterpret_cast<uint32_t *>(pool_ptr(s, r->bm_off)) : nullptr;
             hd.push_back(d);
             hm.push_back(g->m);
             hw.push_back(g->W);
@@ -278,9 +294,10 @@ bool schedule(wtc_server *s) {
             r->t_start = Clock::now();
             x.reqs.push_back(r);
             it = s->queue.erase(it);
+            if (r->split) break;  // alone
         }
         if (hd.empty()) break;
-        const int rc = enqueue_set(x.slot, s->tune, 4, hd, hm.data(), hw.data());
+        const int rc = enqueue_set(x.slot, s->tune, 4, hd, hm.data(), hw.data(), split_base);
         x.batch_id = ++s->batches;
         for (SrvReq *r : x.reqs) {
             r->batch_graphs = (uint32_t)hd.size();
@@ -357,6 +374,10 @@ int wtc_server_create(int device, const wtc_server_config *cfg, wtc_server **out
     s->tune.ways = s->cfg.probe_ways;
     s->tune.chunk = s->cfg.lds_chunk;
     s->tune.wpb = s->cfg.waves_per_block;
+    // a request whose graph the autotuner counts by the split count runs
+    // alone with its workspace from the pool; sets of several graphs walk
+    s->tune.kernel = kKernelSplit;
+    s->tune.split_needs_base = true;
     int rc = init_small_graphs(s->tune, prop, s->cfg.small_graph_words < 0    ? 0u
                                               : s->cfg.small_graph_words == 0 ? default_small_words()
                                                                               : (uint32_t)s->cfg.small_graph_words);
@@ -395,7 +416,8 @@ int wtc_server_register(wtc_server *s, const wtc_big_graph *g, uint64_t *graph_i
     if (!sg) return WTC_ERR_NOMEM;
     sg->n = g->numVertices;
     sg->m = g->numEdges;
-    sg->bytes = align_up(((size_t)sg->n + 1) * 4 + sg->m * 4);
+    sg->col_off = (((size_t)sg->n + 1) * 4 + 15) & ~(size_t)15;
+    sg->bytes = align_up(sg->col_off + sg->m * 4 + kDevTailSlack);
     (void)hipSetDevice(s->device);
     hipError_t e = sg->host.ensure(sg->bytes);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sg->uploaded, hipEventDisableTiming);
@@ -412,7 +434,9 @@ int wtc_server_register(wtc_server *s, const wtc_big_graph *g, uint64_t *graph_i
     }
     sg->W = W;
     sg->cost = std::max<uint64_t>(W, kCostFloor);
-    if (sg->m) std::memcpy(rp + sg->n + 1, g->colInd, sg->m * 4);
+    wtc_tuned tuned;
+    sg->split = wtc_autotune_shape(sg->n, sg->m, W, 0, &tuned) == WTC_OK && tuned.kernel == WTC_KERNEL_SPLIT;
+    if (sg->m) std::memcpy(sg->host.as<char>() + sg->col_off, g->colInd, sg->m * 4);
     std::lock_guard<std::mutex> lk(s->mu);
     sg->id = s->next_gid++;
     s->graphs[sg->id] = sg;
@@ -430,7 +454,7 @@ int wtc_server_submit(wtc_server *s, uint64_t graph_id, uint32_t spread, uint32_
     auto it = s->graphs.find(graph_id);
     if (it == s->graphs.end()) return WTC_ERR_ARG;
     const SrvGraph *g = it->second;
-    if (align_up(g->bytes) + align_up(req_ws_bytes(g)) + align_up(req_bm_bytes(a)) > s->cfg.hbm_budget_bytes)
+    if (align_up(g->bytes) + align_up(req_ws_bytes(g)) + align_up(req_extra_bytes(g, a)) > s->cfg.hbm_budget_bytes)
         return WTC_ERR_ADMIT;
     if (s->queue.size() >= s->cfg.queue_capacity) {
         s->stats.rejected++;
@@ -442,6 +466,7 @@ int wtc_server_submit(wtc_server *s, uint64_t graph_id, uint32_t spread, uint32_
     r->gid = graph_id;
     r->spread = spread;
     r->a = a;
+    r->split = req_split(g, a);
     r->t_submit = Clock::now();
     s->reqs[r->id] = r;
     s->queue.push_back(r);

@@ -22,11 +22,16 @@
 // order and to the in-edges of the vertices holding it, with the tails of
 // the two end vertices clipped.
 //
-// Pipeline on one stream: k_split_partials, k_scan_partials (three times),
-// k_split_scan (the two wedge prefixes and the top in-edge prefix), k_split_range, k_setup (tiles and search
-// tree of the list part), k_split_tr<count>, k_split_cols,
-// k_split_rowscan, k_split_tr<scatter> (the transposition), k_split_rows
-// (top-a part), k_walk (list part).
+// Pipeline on one stream: k_split_partials, k_scan_partials (three times)
+// and k_split_scan (the two wedge prefixes and the top in-edge prefix);
+// k_split_range; k_setup (tiles and search tree of the list part); the
+// transposition in two counting-sort levels, k_split_l1<count>,
+// k_split_l1scan, k_split_l1<scatter> (records bucketed by part of
+// kSplitPart rows), k_split_l2<count>, k_split_l2cols, k_split_rowscan,
+// k_split_l2<scatter> (each part's records grouped by row); k_split_rows
+// (top-a part); k_walk (list part).
+#include <cstdlib>
+
 #include "wtc_dev.h"
 #include "wtc_kernels.h"
 #include "wtc_launch.h"
@@ -47,17 +52,6 @@ __device__ __forceinline__ uint32_t entries_below(const uint32_t *__restrict__ c
         else hi = mid;
     }
     return (uint32_t)(lo - b);
-}
-
-// First position in [b, e) of the ascending row with an id >= x (e if none).
-template <typename OffT>
-__device__ __forceinline__ OffT first_at_least(const uint32_t *__restrict__ ci, OffT b, OffT e, uint32_t x) {
-    while (b < e) {
-        const OffT mid = b + ((e - b) >> 1);
-        if (ci[mid] < x) b = mid + 1;
-        else e = mid;
-    }
-    return b;
 }
 
 // List-part wedges of a vertex with out-degree d and k entries below thr:
@@ -226,36 +220,76 @@ __device__ __forceinline__ uint32_t chunk_start(cptr<uint64_t> ie_pre, uint32_t 
     return rightmost_le(ie_pre, 0u, n - 1, t);
 }
 
-// The counting sort of the top rows' in-edges.  Block (b, q) takes vertex
-// chunk b (clipped to the counted vertices) and row part q (top rows
-// [q*kSplitPart, (q+1)*kSplitPart)).  A wave takes 64 consecutive vertices,
-// scans their top in-edge counts and deals the in-edges to its lanes 64 at
-// a time (coalesced loads of the ids); a lane keeps its in-edge if the id
-// lies in the part.  kScatter = false: the in-edges of each row are counted
-// in LDS and the counts written to hist[b][row]; kScatter = true (hist then
-// holding, per row, the in-edges of earlier chunks): each in-edge takes the
-// next slot of its row through an LDS cursor and its (clipped) tail is
-// stored there.  Both passes see the same in-edges in the same chunks.
+// Clips the tail [j+1, e) of in-edge j of an end vertex of the counted
+// range (row start rb, full-order base ws) to its wedges in [w_lo, w_hi):
+// the tail ids of row i = j - rb are the wedges [base, base + L) of the full
+// order (tc.cu:203-212 numbering).  Returns the clipped [lo, hi) (lo == hi:
+// none).
+template <typename OffT>
+__device__ __forceinline__ void clip_tail(OffT j, OffT e, OffT rb, uint64_t ws, uint64_t w_lo, uint64_t w_hi,
+                                          OffT &lo, OffT &hi) {
+    const uint64_t d = (uint64_t)(e - rb), i = (uint64_t)(j - rb);
+    const uint64_t bse = ws + i * (d - 1) - binom2_d(i), L = (uint64_t)(e - j - 1);
+    const uint64_t a = bse > w_lo ? bse : w_lo;
+    const uint64_t z = bse + L < w_hi ? bse + L : w_hi;
+    lo = j + 1;
+    hi = lo;
+    if (a < z) {
+        lo = j + 1 + (OffT)(a - bse);
+        hi = j + 1 + (OffT)(z - bse);
+    }
+}
+
+// Vertices with at least this many top in-edges are taken by a whole wave.
+constexpr uint32_t kSplitBigVertex = kWave;
+
+// First level of the transposition.  Block b takes vertex chunk b (clipped
+// to the counted vertices); a wave takes 64 consecutive vertices:
+//  * a vertex with kSplitBigVertex or more top in-edges is taken by the
+//    whole wave in turn: the lanes load its in-edges' ids 64 consecutive
+//    positions at a time, four loads in flight;
+//  * the others' in-edges are scanned and dealt to the lanes 64 at a time
+//    (a lane finds its vertex by a binary lifting over the scanned counts).
+// Every in-edge (its tail clipped at the ends of the counted range; empty
+// ones dropped) belongs to the part (id - thr) / kSplitPart.  kScatter =
+// false: the block counts its in-edges per part into hist1[b][part];
+// kScatter = true (hist1 then holding the record offset of each chunk and
+// part): each in-edge takes the next record slot of its part through an LDS
+// cursor, and its id and tail are stored there.  Both passes see the same
+// in-edges in the same chunks.
 template <typename OffT, bool kScatter>
-__global__ __launch_bounds__(kSplitBlock) void k_split_tr(const GraphDesc *__restrict__ desc,
+__global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__restrict__ desc,
                                                           const SplitDesc *__restrict__ sd) {
-    __shared__ uint32_t s_cnt[kSplitPart];
+    __shared__ uint32_t s_cnt[kSplitPartsMax];
+    __shared__ uint64_t s_base[kSplitPartsMax];
     const GraphDesc &D = desc[0];
     const SplitDesc &S = sd[0];
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
-    const uint32_t *__restrict__ colInd = D.colInd;
+    const gptr<uint32_t> colInd = to_global(D.colInd);
     const uint32_t *__restrict__ kpos = S.kpos;
     const uint64_t *__restrict__ wf = S.ws_full;
-    uint32_t *__restrict__ hist = S.hist;
-    const OffT *__restrict__ rp_in = static_cast<const OffT *>(S.rp_in);
-    TailPair<OffT> *__restrict__ ie = static_cast<TailPair<OffT> *>(S.ie);
-    const uint32_t n = D.n, thr = D.thr, rows = S.rows, parts = S.parts;
-    const uint32_t b = blockIdx.x / parts, q = blockIdx.x - b * parts;
-    const uint32_t r0 = q * kSplitPart, r1 = r0 + kSplitPart < rows ? r0 + kSplitPart : rows;
-    const uint32_t id0 = thr + r0, id1 = thr + r1;
-    const uint64_t hrow = (uint64_t)b * rows;
-    for (uint32_t i = threadIdx.x; i < r1 - r0; i += kSplitBlock) s_cnt[i] = kScatter ? hist[hrow + r0 + i] : 0u;
+    uint64_t *__restrict__ hist1 = S.hist1;
+    uint32_t *__restrict__ rec_id = S.rec_id;
+    TailPair<OffT> *__restrict__ rec = static_cast<TailPair<OffT> *>(S.rec);
+    const uint32_t n = D.n, thr = D.thr, parts = S.parts;
+    const uint32_t b = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) {
+        s_cnt[i] = 0u;
+        s_base[i] = kScatter ? hist1[b * parts + i] : 0u;
+    }
     __syncthreads();
+
+    // one in-edge: count it in its part, or store its record
+    auto take = [&](uint32_t id, OffT lo, OffT hi) {
+        const uint32_t q = (id - thr) / kSplitPart;
+        if (!kScatter) {
+            atomicAdd(&s_cnt[q], 1u);
+        } else {
+            const uint64_t p = s_base[q] + atomicAdd(&s_cnt[q], 1u);
+            rec_id[p] = id;
+            rec[p] = TailPair<OffT>{lo, hi};
+        }
+    };
 
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
     const bool clip = S.clip != 0;
@@ -282,7 +316,37 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_tr(const GraphDesc *__res
             edge = clip && (v == v_lo || v == v_hi);
             if (edge) wsv = wf[v];
         }
-        // a vertex has at most `rows` top in-edges, so 64 of them fit 32 bits
+        // big vertices, one at a time by the whole wave
+        uint64_t big = __ballot(cnt >= kSplitBigVertex);
+        while (big) {
+            const uint32_t src = (uint32_t)__builtin_ctzll(big);
+            big &= big - 1;
+            const OffT vj0 = (OffT)readlane_u64((uint64_t)j0, src);
+            const OffT vend = vj0 + (OffT)(uint32_t)__builtin_amdgcn_readlane((int)cnt, (int)src);
+            const OffT ve = (OffT)readlane_u64((uint64_t)e, src);
+            const bool vedge = __builtin_amdgcn_readlane((int)edge, (int)src) != 0;
+            const OffT vrb = (OffT)readlane_u64((uint64_t)rb, src);
+            const uint64_t vws = readlane_u64(wsv, src);
+            for (OffT jb = vj0; jb < vend; jb += 4 * kWave) {
+                uint32_t ids[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const OffT j = jb + (OffT)(u * kWave) + lane;
+                    ids[u] = j < vend ? colInd[j] : 0u;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const OffT j = jb + (OffT)(u * kWave) + lane;
+                    if (j < vend) {
+                        OffT lo = j + 1, hi = ve;
+                        if (vedge) clip_tail<OffT>(j, ve, vrb, vws, w_lo, w_hi, lo, hi);
+                        if (lo < hi) take(ids[u], lo, hi);
+                    }
+                }
+            }
+        }
+        if (cnt >= kSplitBigVertex) cnt = 0;
+        // the other vertices' in-edges, dealt to the lanes
         uint32_t incl = cnt;
 #pragma unroll
         for (int o = 1; o < kWave; o <<= 1) {
@@ -309,52 +373,110 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_tr(const GraphDesc *__res
                 const bool ed = __shfl((int)edge, (int)pos, kWave) != 0;
                 const OffT rbv = shfl_off(rb, pos);
                 const uint64_t wv0 = shfl_u64(wsv, pos);
-                if (ed) {
-                    // tail ids of the pair's row i = j - rb: [base, base + L)
-                    const uint64_t d = (uint64_t)(ev - rbv), i = (uint64_t)(j - rbv);
-                    const uint64_t bse = wv0 + i * (d - 1) - binom2_d(i), L = (uint64_t)(ev - j - 1);
-                    const uint64_t a = bse > w_lo ? bse : w_lo;
-                    const uint64_t z = bse + L < w_hi ? bse + L : w_hi;
-                    if (a < z) {
-                        lo = j + 1 + (OffT)(a - bse);
-                        hi = j + 1 + (OffT)(z - bse);
-                    } else {
-                        hi = lo;
-                    }
-                }
+                if (ed) clip_tail<OffT>(j, ev, rbv, wv0, w_lo, w_hi, lo, hi);
             }
-            if (idx < tot) {
-                const uint32_t id = colInd[j];
-                if (id >= id0 && id < id1 && lo < hi) {
-                    const uint32_t r = id - thr;
-                    if (!kScatter) {
-                        atomicAdd(&s_cnt[r - r0], 1u);
-                    } else {
-                        const OffT p = rp_in[r] + (OffT)atomicAdd(&s_cnt[r - r0], 1u);
-                        ie[p] = TailPair<OffT>{lo, hi};
-                    }
-                }
-            }
+            if (idx < tot && lo < hi) take(colInd[j], lo, hi);
         }
     }
     if (!kScatter) {
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < r1 - r0; i += kSplitBlock) hist[hrow + r0 + i] = s_cnt[i];
+        for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) hist1[b * parts + i] = s_cnt[i];
     }
 }
 
-// Thread per top row: the column prefix of hist over the chunks (hist[b][r]
-// becomes the in-edges of row r in chunks before b) and the row's total.
-__global__ __launch_bounds__(kBlock) void k_split_cols(const SplitDesc *__restrict__ sd) {
+// One thread per part (one block): the records of each part (its column of
+// hist1 summed), the parts' first records (part_base), each chunk's record
+// offset in each part (hist1 rewritten), and the first second-level block
+// of each part (c2base: kSplitChunk2 records per block).
+__global__ __launch_bounds__(kSplitPartsMax) void k_split_l1scan(const SplitDesc *__restrict__ sd) {
+    __shared__ uint64_t s_tot[kSplitPartsMax];
+    const SplitDesc &S = sd[0];
+    const uint32_t parts = S.parts, chunks = S.chunks, q = threadIdx.x;
+    uint64_t *__restrict__ hist1 = S.hist1;
+    uint64_t tot = 0;
+    if (q < parts)
+        for (uint32_t c = 0; c < chunks; c++) tot += hist1[c * parts + q];
+    s_tot[q] = q < parts ? tot : 0;
+    __syncthreads();
+    if (q == 0) {
+        uint64_t base = 0;
+        uint32_t c2 = 0;
+        for (uint32_t i = 0; i < parts; i++) {
+            const uint64_t t = s_tot[i];
+            S.part_base[i] = base;
+            S.c2base[i] = c2;
+            s_tot[i] = base;
+            base += t;
+            c2 += (uint32_t)((t + kSplitChunk2 - 1) / kSplitChunk2);
+        }
+        S.part_base[parts] = base;
+        S.c2base[parts] = c2;
+    }
+    __syncthreads();
+    if (q < parts) {
+        uint64_t off = s_tot[q];
+        for (uint32_t c = 0; c < chunks; c++) {
+            const uint64_t h = hist1[c * parts + q];
+            hist1[c * parts + q] = off;
+            off += h;
+        }
+    }
+}
+
+// Second level.  Block c2 takes records [part_base[q] + k*kSplitChunk2, ...)
+// of part q (c2 = c2base[q] + k; blocks past c2base[parts] do nothing) and
+// groups them by row of the part: kScatter = false counts them per row into
+// hist2[c2][row - q*kSplitPart]; kScatter = true (hist2 then holding, per
+// row, the part's records of earlier blocks) places each record's tail at
+// the next slot of its row in ie.
+template <typename OffT, bool kScatter>
+__global__ __launch_bounds__(kSplitBlock) void k_split_l2(const GraphDesc *__restrict__ desc,
+                                                          const SplitDesc *__restrict__ sd) {
+    __shared__ uint32_t s_cnt[kSplitPart];
+    const SplitDesc &S = sd[0];
+    const uint32_t parts = S.parts, rows = S.rows, thr = desc[0].thr;
+    const uint32_t c2 = blockIdx.x;
+    if (c2 >= S.c2base[parts]) return;  // block-uniform
+    uint32_t q = 0;
+    while (q + 1 < parts && S.c2base[q + 1] <= c2) q++;
+    const uint32_t r0 = q * kSplitPart, r1 = r0 + kSplitPart < rows ? r0 + kSplitPart : rows;
+    const uint64_t k0 = S.part_base[q] + (uint64_t)(c2 - S.c2base[q]) * kSplitChunk2;
+    const uint64_t kq = S.part_base[q + 1], k1 = k0 + kSplitChunk2 < kq ? k0 + kSplitChunk2 : kq;
+    uint32_t *__restrict__ hist2 = S.hist2 + (uint64_t)c2 * kSplitPart;
+    for (uint32_t i = threadIdx.x; i < r1 - r0; i += kSplitBlock) s_cnt[i] = kScatter ? hist2[i] : 0u;
+    __syncthreads();
+    const uint32_t *__restrict__ rec_id = S.rec_id;
+    const TailPair<OffT> *__restrict__ rec = static_cast<const TailPair<OffT> *>(S.rec);
+    const OffT *__restrict__ rp_in = static_cast<const OffT *>(S.rp_in);
+    TailPair<OffT> *__restrict__ ie = static_cast<TailPair<OffT> *>(S.ie);
+    for (uint64_t k = k0 + threadIdx.x; k < k1; k += kSplitBlock) {
+        const uint32_t r = rec_id[k] - thr;
+        if (!kScatter) {
+            atomicAdd(&s_cnt[r - r0], 1u);
+        } else {
+            const OffT p = rp_in[r] + (OffT)atomicAdd(&s_cnt[r - r0], 1u);
+            ie[p] = rec[k];
+        }
+    }
+    if (!kScatter) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < r1 - r0; i += kSplitBlock) hist2[i] = s_cnt[i];
+    }
+}
+
+// Thread per top row r (part q): the column prefix of hist2 over the part's
+// second-level blocks (hist2[c2][r - q*kSplitPart] becomes the row's records
+// in earlier blocks) and the row's total.
+__global__ __launch_bounds__(kBlock) void k_split_l2cols(const SplitDesc *__restrict__ sd) {
     const SplitDesc &S = sd[0];
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
     if (r >= S.rows) return;
-    uint32_t *__restrict__ hist = S.hist;
-    const uint64_t rows = S.rows;
+    const uint32_t q = r / kSplitPart, i = r - q * kSplitPart;
+    uint32_t *__restrict__ hist2 = S.hist2;
     uint32_t s = 0;
-    for (uint32_t b = 0; b < S.chunks; b++) {
-        const uint32_t h = hist[b * rows + r];
-        hist[b * rows + r] = s;
+    for (uint32_t c2 = S.c2base[q]; c2 < S.c2base[q + 1]; c2++) {
+        const uint32_t h = hist2[(uint64_t)c2 * kSplitPart + i];
+        hist2[(uint64_t)c2 * kSplitPart + i] = s;
         s += h;
     }
     S.row_tot[r] = s;
@@ -362,9 +484,10 @@ __global__ __launch_bounds__(kBlock) void k_split_cols(const SplitDesc *__restri
 
 // One block: rp_in = exclusive prefix of the row totals; rp_in[rows] = the
 // in-edges; the row kernel's tile counter zeroed; the counted wedges of the
-// full order into meta.
+// full order, those of the top rows and the in-edges into meta.
 template <typename OffT>
-__global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const SplitDesc *__restrict__ sd,
+__global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const GraphDesc *__restrict__ desc,
+                                                               const SplitDesc *__restrict__ sd,
                                                                uint64_t *__restrict__ meta) {
     constexpr int kWaves = kSplitBlock / kWave;
     __shared__ uint64_t s_w[kWaves];
@@ -395,6 +518,8 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const SplitDesc *
         rp[rows] = (OffT)carry;
         meta[kMetaRows] = 0;
         meta[kMetaWedges] = S.w_hi - S.w_lo;  // k_setup counted the list part's only
+        meta[kMetaTopWedges] = S.w_hi - S.w_lo - desc[0].wcount;
+        meta[kMetaInEdges] = carry;
     }
 }
 
@@ -407,7 +532,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const SplitDesc *
 // the row's bits cleared.  colInd must be 16-byte aligned and readable 12
 // bytes past its last id (every buffer of this library carries 256 bytes of
 // slack).
-template <typename OffT, int kWaves>
+template <typename OffT, int kWaves, int kLanes>
 __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *__restrict__ desc,
                                                                const SplitDesc *__restrict__ sd,
                                                                uint64_t *__restrict__ meta,
@@ -420,11 +545,14 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
     const gptr<uint32_t> colInd = to_global(D.colInd);
     const cptr<OffT> rp_in = to_const(static_cast<const OffT *>(S.rp_in));
     const cptr<OffT> ieU = to_const(static_cast<const OffT *>(S.ie));
+    const gptr<OffT> ieG = to_global(static_cast<const OffT *>(S.ie));
     const uint32_t thr = D.thr, rows = S.rows;
+    constexpr uint32_t kPer = kWave / kLanes;  // in-edges a wave takes per step
     const uint32_t words = (rows + 31) >> 5;
     for (uint32_t i = threadIdx.x; i < words; i += kWave * kWaves) s_bits[i] = 0;
     const uint64_t K = (uint64_t)rp_in[rows];
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = uni32(threadIdx.x >> 6);
+    const uint32_t sub = lane / kLanes, sl = lane % kLanes;
     unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaRows);
     uint64_t hits = 0;
     for (;;) {
@@ -447,10 +575,18 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
             }
             __syncthreads();
             uint32_t h = 0;
-            for (uint64_t x = k + wv; x < kend; x += kWaves) {
-                const OffT lo = uni(ieU[2 * x]), hi = uni(ieU[2 * x + 1]);
-                for (OffT q = (lo & ~(OffT)3) + 4 * lane; q < hi; q += 8 * kWave) {
-                    const OffT q1 = q + 4 * kWave;
+            for (uint64_t x0 = k + (uint64_t)wv * kPer; x0 < kend; x0 += (uint64_t)kWaves * kPer) {
+                // kPer in-edges per step, kLanes lanes each
+                OffT lo = 0, hi = 0;
+                if (kPer == 1) {
+                    lo = uni(ieU[2 * x0]);
+                    hi = uni(ieU[2 * x0 + 1]);
+                } else if (x0 + sub < kend) {
+                    lo = ieG[2 * (x0 + sub)];
+                    hi = ieG[2 * (x0 + sub) + 1];
+                }
+                for (OffT q = (lo & ~(OffT)3) + 4 * sl; q < hi; q += 8 * kLanes) {
+                    const OffT q1 = q + 4 * kLanes;
                     const u32x4 a = *reinterpret_cast<gptr<u32x4>>(colInd + q);
                     u32x4 b = {0u, 0u, 0u, 0u};
                     if (q1 < hi) b = *reinterpret_cast<gptr<u32x4>>(colInd + q1);
@@ -497,36 +633,64 @@ hipError_t launch_split_range(GraphDesc *desc, SplitDesc *sd, hipStream_t st) {
 }
 
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
-                                  uint32_t parts, uint32_t rows, uint64_t *meta, hipStream_t st) {
-    if (rows == 0 || chunks == 0) return hipErrorInvalidValue;
-    const dim3 grid(chunks * parts), blk(kSplitBlock);
+                                  uint32_t c2cap, uint32_t rows, uint64_t *meta, hipStream_t st) {
+    if (rows == 0 || chunks == 0 || c2cap == 0) return hipErrorInvalidValue;
+    const dim3 blk(kSplitBlock), g1(chunks), g2(c2cap);
     hipError_t e;
-    if (ob == 8) k_split_tr<uint64_t, false><<<grid, blk, 0, st>>>(desc, sd);
-    else k_split_tr<uint32_t, false><<<grid, blk, 0, st>>>(desc, sd);
+    if (ob == 8) k_split_l1<uint64_t, false><<<g1, blk, 0, st>>>(desc, sd);
+    else k_split_l1<uint32_t, false><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_split_cols<<<dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, st>>>(sd);
+    k_split_l1scan<<<dim3(1), dim3(kSplitPartsMax), 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_rowscan<uint64_t><<<dim3(1), blk, 0, st>>>(sd, meta);
-    else k_split_rowscan<uint32_t><<<dim3(1), blk, 0, st>>>(sd, meta);
+    if (ob == 8) k_split_l1<uint64_t, true><<<g1, blk, 0, st>>>(desc, sd);
+    else k_split_l1<uint32_t, true><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_tr<uint64_t, true><<<grid, blk, 0, st>>>(desc, sd);
-    else k_split_tr<uint32_t, true><<<grid, blk, 0, st>>>(desc, sd);
+    if (ob == 8) k_split_l2<uint64_t, false><<<g2, blk, 0, st>>>(desc, sd);
+    else k_split_l2<uint32_t, false><<<g2, blk, 0, st>>>(desc, sd);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_split_l2cols<<<dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, st>>>(sd);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ob == 8) k_split_rowscan<uint64_t><<<dim3(1), blk, 0, st>>>(desc, sd, meta);
+    else k_split_rowscan<uint32_t><<<dim3(1), blk, 0, st>>>(desc, sd, meta);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ob == 8) k_split_l2<uint64_t, true><<<g2, blk, 0, st>>>(desc, sd);
+    else k_split_l2<uint32_t, true><<<g2, blk, 0, st>>>(desc, sd);
     return hipGetLastError();
 }
 
 size_t split_rows_lds(uint32_t rows) { return (size_t)((rows + 31) / 32) * sizeof(uint32_t); }
 
-template <int W>
+// Lanes per in-edge tail of the row kernel: 64, 32 or 16 ($WTC_SPLIT_LANES,
+// else 64).
+static int split_lanes() {
+    static const int v = [] {
+        const char *e = std::getenv("WTC_SPLIT_LANES");
+        const int x = e ? std::atoi(e) : 64;
+        return x == 16 || x == 32 ? x : 64;
+    }();
+    return v;
+}
+
+template <int W, int L>
 static const void *split_rows_fn(int ob) {
-    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W>)
-                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W>);
+    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, L>)
+                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, L>);
+}
+
+template <int L>
+static const void *split_rows_kernel_l(int ob, int wpb) {
+    switch (wpb) {
+        case 4: return split_rows_fn<4, L>(ob);
+        case 16: return split_rows_fn<16, L>(ob);
+        default: return split_rows_fn<8, L>(ob);
+    }
 }
 
 static const void *split_rows_kernel(int ob, int wpb) {
-    switch (wpb) {
-        case 4: return split_rows_fn<4>(ob);
-        case 16: return split_rows_fn<16>(ob);
-        default: return split_rows_fn<8>(ob);
+    switch (split_lanes()) {
+        case 16: return split_rows_kernel_l<16>(ob, wpb);
+        case 32: return split_rows_kernel_l<32>(ob, wpb);
+        default: return split_rows_kernel_l<64>(ob, wpb);
     }
 }
 
@@ -550,22 +714,9 @@ hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob,
                              uint64_t *meta, unsigned long long *counts, uint32_t grid, hipStream_t st) {
     if (rows == 0) return hipSuccess;
     if (!valid_split_waves(wpb) || grid == 0) return hipErrorInvalidValue;
-    const size_t shm = split_rows_lds(rows);
-    const dim3 g(grid), blk(wpb * kWave);
-    switch (wpb) {
-        case 4:
-            if (ob == 8) k_split_rows<uint64_t, 4><<<g, blk, shm, st>>>(desc, sd, meta, counts);
-            else k_split_rows<uint32_t, 4><<<g, blk, shm, st>>>(desc, sd, meta, counts);
-            break;
-        case 16:
-            if (ob == 8) k_split_rows<uint64_t, 16><<<g, blk, shm, st>>>(desc, sd, meta, counts);
-            else k_split_rows<uint32_t, 16><<<g, blk, shm, st>>>(desc, sd, meta, counts);
-            break;
-        default:
-            if (ob == 8) k_split_rows<uint64_t, 8><<<g, blk, shm, st>>>(desc, sd, meta, counts);
-            else k_split_rows<uint32_t, 8><<<g, blk, shm, st>>>(desc, sd, meta, counts);
-    }
-    return hipGetLastError();
+    const void *fn = split_rows_kernel(ob, wpb);
+    void *args[] = {&desc, &sd, &meta, &counts};
+    return hipLaunchKernel(fn, dim3(grid), dim3(wpb * kWave), args, split_rows_lds(rows), st);
 }
 
 }  // namespace wtc

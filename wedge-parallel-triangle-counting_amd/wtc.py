@@ -63,7 +63,8 @@ class GpuTime(C.Structure):
 class CountStats(C.Structure):
     _fields_ = [("prefix_ms", C.c_double), ("bitmap_ms", C.c_double), ("count_ms", C.c_double),
                 ("total_ms", C.c_double), ("wedges", C.c_uint64), ("blocks", C.c_uint32),
-                ("launches", C.c_uint32), ("small_graphs", C.c_uint32), ("reserved", C.c_uint32)]
+                ("launches", C.c_uint32), ("small_graphs", C.c_uint32), ("split", C.c_uint32),
+                ("rows_ms", C.c_double), ("top_wedges", C.c_uint64), ("in_edges", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -490,8 +491,9 @@ class Context:
         """Applies an autotuned shape (kernel, block waves, probe ways, row-kernel waves)."""
         _check(_lib.wtc_ctx_use_tuned(self._p, C.byref(t)), "use_tuned")
 
-    def set_work_split(self, tiles_per_grab: int = 0, deal_to_xcds: bool = True):
-        """Most tiles per grab (0: 32) and XCD dealing of the counting kernels."""
+    def set_work_split(self, tiles_per_grab: int = 4, deal_to_xcds: bool = False):
+        """Most tiles per grab (0: 32) and XCD dealing of the counting kernels
+        (default: 4 tiles, one stream of grabs)."""
         _check(_lib.wtc_ctx_set_work_split(self._p, tiles_per_grab, int(deal_to_xcds)), "set_work_split")
 
     def set_small_graphs(self, max_words: int):
