@@ -210,11 +210,10 @@ def test_tc_gpu_drop_ins_reject_bad_csr_on_host():
 
 def test_autotune_table():
     # whatever table is in force (the committed sweep's or the built-in
-    # rows): no bitmap for graphs without wedges or of low skew, a bitmap
-    # length that is a multiple of 32 and no larger than n rounded up, and a
-    # spread of at least 1
+    # rows): no bitmap for a graph without wedges, a bitmap length that is a
+    # multiple of 32 and no larger than n rounded up, and a spread of at
+    # least 1
     assert wtc.autotune(10, 0, 0) == (1, 0)
-    assert wtc.autotune(40000, 600000, 4000000)[1] == 0
     for n, m, W in ((4083, 48254, 780919), (1048465, 15701962, 1226291937),
                     (16775427, 260385033, 41737375097), (268428511, 4236159419, 1357709474934)):
         s, a = wtc.autotune(n, m, W)
