@@ -643,11 +643,18 @@ def offsets_u64_leg(ctx, g, W, spread, a, steps, warmup, expect):
         dg8.free()
     _, b_min8 = g.alg_bytes(8)
     count_ms = statistics.median(s["count_ms"] for s in stats)
-    return {"offset_bytes": 8, "value": W / (el / steps), "unit": "wedge checks/s", "ms_per_step": 1e3 * el / steps,
-            "kernel_ms": {"count": count_ms, "prefix": statistics.median(s["prefix_ms"] for s in stats),
-                          "bitmap": statistics.median(s["bitmap_ms"] for s in stats)},
-            "roofline_frac": b_min8 / (count_ms / 1e3) / HBM_PEAK,
-            "triangles": counts[0], "counts_equal_u32_offsets": all(c == expect for c in counts)}
+    out = {"offset_bytes": 8, "value": W / (el / steps), "unit": "wedge checks/s", "ms_per_step": 1e3 * el / steps,
+           "kernel_ms": {"count": count_ms, "prefix": statistics.median(s["prefix_ms"] for s in stats),
+                         "bitmap_or_transposition": statistics.median(s["bitmap_ms"] for s in stats)},
+           "count_frac_b_min": b_min8 / (count_ms / 1e3) / HBM_PEAK,
+           "triangles": counts[0], "counts_equal_u32_offsets": all(c == expect for c in counts)}
+    if stats[-1]["split"]:  # the row kernel's roofline, as for the headline (2*8 B per in-edge record)
+        rows_ms = statistics.median(s["rows_ms"] for s in stats)
+        dom_bytes = 4.0 * stats[-1]["top_wedges"] + 16.0 * stats[-1]["in_edges"]
+        out["kernel_ms"]["split_rows"] = rows_ms
+        out["roofline"] = {"kernel": "k_split_rows", "achieved": dom_bytes / (rows_ms / 1e3) / 1e9, "unit": "GB/s",
+                           "frac": dom_bytes / (rows_ms / 1e3) / HBM_PEAK, "algorithmic_bytes": dom_bytes}
+    return out
 
 
 def main():
@@ -857,8 +864,9 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": traffic["bytes_per_launch"] if traffic else None,
                      "algorithmic_bytes": dom_bytes, "units": dom_units, "kernel_ms": dom_ms,
-                     # the whole count (every counting kernel) against the floor of one id and one
-                     # probe word per wedge, B_min
+                     # the whole count (every counting kernel) against the reference algorithm's
+                     # floor of one id and one probe word per wedge from memory, B_min; above 1 when
+                     # the split count answers the top-a probes from LDS
                      "b_min": b_min, "count_frac_b_min": b_min / (count_ms / 1e3) / HBM_PEAK,
                      # the reference's own probe words (every binary-search step one word): an upper
                      # figure, most of whose words a bitmap or the caches never fetch from HBM

@@ -30,8 +30,6 @@
 // kSplitPart rows), k_split_l2<count>, k_split_l2cols, k_split_rowscan,
 // k_split_l2<scatter> (each part's records grouped by row); k_split_rows
 // (top-a part); k_walk (list part).
-#include <cstdlib>
-
 #include "wtc_dev.h"
 #include "wtc_kernels.h"
 #include "wtc_launch.h"
@@ -525,13 +523,20 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const GraphDesc *
 
 // ------------------------------------------------------------- top-a part
 
+// Lanes per in-edge tail in the row kernel: a wave takes kWave / kSplitLanes
+// in-edges per step.  Tails average ~180 ids on R-MAT-24; with a whole wave
+// per tail most lanes idled (37.4 ms vs 25.1 ms at 16 lanes, 28.1 at 32,
+// 24.8 at 8).
+constexpr int kSplitLanes = 16;
+
 // Persistent blocks take tiles of kSplitTile consecutive in-edges (in row
 // order) from one counter.  Per row of a tile: adj+(w) set as bits (u - thr)
-// in LDS, the tile's in-edges of the row dealt to the waves, and each tail
-// streamed through the bitmap by 16-byte loads, two in flight per lane; then
-// the row's bits cleared.  colInd must be 16-byte aligned and readable 12
-// bytes past its last id (every buffer of this library carries 256 bytes of
-// slack).
+// in LDS, the tile's in-edges of the row dealt to the waves (kLanes lanes
+// per in-edge), and each tail streamed through the bitmap by 16-byte loads,
+// two in flight per lane; then the row's bits cleared.  colInd must be
+// 16-byte aligned and readable 12 bytes past its last id (every buffer of
+// this library carries 256 bytes of slack).
+
 template <typename OffT, int kWaves, int kLanes>
 __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *__restrict__ desc,
                                                                const SplitDesc *__restrict__ sd,
@@ -660,37 +665,17 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
 
 size_t split_rows_lds(uint32_t rows) { return (size_t)((rows + 31) / 32) * sizeof(uint32_t); }
 
-// Lanes per in-edge tail of the row kernel: 64, 32 or 16 ($WTC_SPLIT_LANES,
-// else 64).
-static int split_lanes() {
-    static const int v = [] {
-        const char *e = std::getenv("WTC_SPLIT_LANES");
-        const int x = e ? std::atoi(e) : 64;
-        return x == 16 || x == 32 ? x : 64;
-    }();
-    return v;
-}
-
-template <int W, int L>
+template <int W>
 static const void *split_rows_fn(int ob) {
-    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, L>)
-                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, L>);
-}
-
-template <int L>
-static const void *split_rows_kernel_l(int ob, int wpb) {
-    switch (wpb) {
-        case 4: return split_rows_fn<4, L>(ob);
-        case 16: return split_rows_fn<16, L>(ob);
-        default: return split_rows_fn<8, L>(ob);
-    }
+    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, kSplitLanes>)
+                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, kSplitLanes>);
 }
 
 static const void *split_rows_kernel(int ob, int wpb) {
-    switch (split_lanes()) {
-        case 16: return split_rows_kernel_l<16>(ob, wpb);
-        case 32: return split_rows_kernel_l<32>(ob, wpb);
-        default: return split_rows_kernel_l<64>(ob, wpb);
+    switch (wpb) {
+        case 4: return split_rows_fn<4>(ob);
+        case 16: return split_rows_fn<16>(ob);
+        default: return split_rows_fn<8>(ob);
     }
 }
 
