@@ -571,11 +571,13 @@ def _rocprof_rows(directory, pattern):
 
 
 def _per_dispatch(directory, kernel_sub, counter):
-    """Mean per-dispatch value of `counter` for kernels whose name contains
-    kernel_sub, from rocprofv3 counter_collection.csv files."""
+    """Mean per-dispatch value of `counter` for the kernel named kernel_sub
+    (template arguments aside), from rocprofv3 counter_collection.csv files."""
+    import re
+    pat = re.compile(r"\b" + re.escape(kernel_sub) + r"[<(]")  # the kernel itself, not a longer name
     vals = {}
     for r in _rocprof_rows(directory, "*counter_collection.csv"):
-        if kernel_sub in r.get("Kernel_Name", "") and r.get("Counter_Name") == counter:
+        if pat.search(r.get("Kernel_Name", "")) and r.get("Counter_Name") == counter:
             key = r.get("Dispatch_Id") or r.get("Correlation_Id")
             vals[key] = vals.get(key, 0.0) + float(r["Counter_Value"])
     if not vals:
@@ -593,8 +595,8 @@ def pmc_summary(args):
     copy_bytes = args.copy_calibration_mb * 2 ** 20
     kf, nk = _per_dispatch(kdir_f, kern, "FETCH_SIZE")
     kw, _ = _per_dispatch(kdir_w, kern, "WRITE_SIZE")
-    cf, _ = _per_dispatch(cdir_f, "k_copy", "FETCH_SIZE")
-    cw, _ = _per_dispatch(cdir_w, "k_copy", "WRITE_SIZE")
+    cf, _ = _per_dispatch(cdir_f, "k_copy_u32x4", "FETCH_SIZE")
+    cw, _ = _per_dispatch(cdir_w, "k_copy_u32x4", "WRITE_SIZE")
     f_scale = copy_bytes / (cf * 1024.0)  # true bytes per counted byte
     w_scale = copy_bytes / (cw * 1024.0)
     fetch = kf * 1024.0 * f_scale
@@ -609,6 +611,41 @@ def pmc_summary(args):
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
+
+
+def trace_summary(args):
+    """Markdown summary of a rocprofv3 --kernel-trace --stats -f csv run: per
+    kernel calls, average and total duration (from the kernel trace, so one
+    row per dispatched kernel name), written to --summary-out with the
+    command that made the trace."""
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(args.trace_summary, "**", "*kernel_trace.csv"), recursive=True):
+        with open(path, newline="") as f:
+            rows.extend(csv.DictReader(f))
+    if not rows:
+        raise SystemExit(f"no kernel_trace.csv under {args.trace_summary}")
+    agg = {}
+    for r in rows:
+        name = r["Kernel_Name"]
+        dur = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
+        a = agg.setdefault(name, [0, 0.0, float("inf"), 0.0])
+        a[0] += 1
+        a[1] += dur
+        a[2] = min(a[2], dur)
+        a[3] = max(a[3], dur)
+    total = sum(a[1] for a in agg.values())
+    lines = [f"# {args.summary_title}", "", "Command (from the repository root, on the MI355X):", "",
+             f"    {args.summary_cmd}", "",
+             "| kernel | calls | avg ms | min ms | max ms | total ms | share |", "|---|---|---|---|---|---|---|"]
+    for name, (k, tot, lo, hi) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        short = name.split("(")[0].replace("void ", "")
+        lines.append(f"| `{short}` | {k} | {tot / k:.4f} | {lo:.4f} | {hi:.4f} | {tot:.3f} | {100 * tot / total:.1f}% |")
+    text = "\n".join(lines) + "\n"
+    with open(args.summary_out, "w") as f:
+        f.write(text)
+    print(text)
 
 
 def load_golden_rmat(scale, edge_factor, seed):
@@ -685,6 +722,10 @@ def main():
                     help="summarise rocprofv3 --pmc output dirs into profiles/ (no GPU use)")
     ap.add_argument("--pmc-workload", default="rmat24")
     ap.add_argument("--pmc-kernel", default="k_split_rows", help="kernel name the PMC summary covers")
+    ap.add_argument("--trace-summary", default="", help="summarise a rocprofv3 kernel-trace CSV dir (no GPU use)")
+    ap.add_argument("--summary-out", default="", help="markdown file --trace-summary writes")
+    ap.add_argument("--summary-title", default="rocprofv3 kernel trace")
+    ap.add_argument("--summary-cmd", default="", help="the profiled command, quoted in the summary")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="batched vs plain on this many graphs (0: skip)")
     ap.add_argument("--no-extras", action="store_true",
                     help="only the RMAT count: no batched-vs-plain and no serving legs (profiler runs)")
@@ -718,6 +759,9 @@ def main():
 
     if args.pmc_summary:
         pmc_summary(args)
+        return
+    if args.trace_summary:
+        trace_summary(args)
         return
     if args.ingest_only:
         print(json.dumps(ingest_bench(args.scale, args.seed)), flush=True)
