@@ -96,17 +96,29 @@ void Slot::release_device() {
     sp.release();
 }
 
+int Slot::ensure_side() {
+    if (side) return WTC_OK;
+    WTC_HIPC(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    WTC_HIPC(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    WTC_HIPC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    return WTC_OK;
+}
+
 void Slot::destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
     release_device();
     h_desc.release();
     h_out.release();
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     if (ev_ready) (void)hipEventDestroy(ev_ready);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
-    stream = nullptr;
-    ev_ready = nullptr;
+    stream = side = nullptr;
+    ev_ready = ev_fork = ev_join = nullptr;
     for (auto &e : ev) e = nullptr;
 }
 
@@ -210,9 +222,12 @@ void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t ro
     sd->ie = base + L.ie;
 }
 
+// A bitmap length whose row does not fit one block's LDS is counted by the
+// row walk against the bitmap in HBM, as the reference does.
 bool split_eligible(const Tuning &t, const GraphDesc &d) {
     return t.kernel == kKernelSplit && d.mode == kModeTiles && d.a > 0 && d.bm_rows > 0 &&
-           d.bm_rows <= kSplitPartsMax * kSplitPart && (reinterpret_cast<uintptr_t>(d.colInd) & 15u) == 0;
+           d.bm_rows <= kSplitPartsMax * kSplitPart && split_rows_lds(d.bm_rows) <= t.lds_bytes &&
+           (reinterpret_cast<uintptr_t>(d.colInd) & 15u) == 0;
 }
 
 int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan, const uint64_t *m,
@@ -306,6 +321,8 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
         }
         split_layout(&sd, base, hd[0].n, m[0], split_rows, ob);
         WTC_HIPC(split_rows_grid(ob, split_rows, t.split_wpb, t.num_cus, &s.rows_grid));
+        int rc = s.ensure_side();
+        if (rc) return rc;
     }
     std::memcpy(s.h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
     if (split) std::memcpy(s.h_desc.as<char>() + split_off, &sd, sizeof(SplitDesc));
@@ -363,14 +380,21 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
         WTC_HIPC(launch_split_range(ddesc, sd, s.stream));
         WTC_HIPC(launch_setup(ddesc, 1, tree, meta, dcounts, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
+        // the list part's walk needs only the prefix, tree and tiles: it runs
+        // on the side stream while the main stream transposes (both add into
+        // the same count; they share no other device state)
+        WTC_HIPC(hipEventRecord(s.ev_fork, s.stream));
+        WTC_HIPC(hipStreamWaitEvent(s.side, s.ev_fork, 0));
+        WTC_HIPC(launch_count(p.kernel, ddesc, 1, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways,
+                              p.wpb, p.work, s.grid, s.side));
+        WTC_HIPC(hipEventRecord(s.ev_join, s.side));
         WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_rows, meta,
                                         s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
         WTC_HIPC(launch_split_rows(ddesc, sd, p.ob, p.split_rows, p.split_wpb, meta, dcounts, s.rows_grid,
                                    s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[4], s.stream));
-        WTC_HIPC(launch_count(p.kernel, ddesc, 1, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways,
-                              p.wpb, p.work, s.grid, s.stream));
+        WTC_HIPC(hipStreamWaitEvent(s.stream, s.ev_join, 0));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
         WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + p.G) * sizeof(uint64_t),
                                 hipMemcpyDeviceToHost, s.stream));
@@ -412,6 +436,7 @@ uint32_t default_small_words() {
 }
 
 int init_small_graphs(Tuning &t, const hipDeviceProp_t &prop, uint32_t want) {
+    t.lds_bytes = prop.sharedMemPerBlock;
     t.small_words_max = (uint32_t)std::min<size_t>(prop.sharedMemPerBlock / 4, kSmallWordsMax);
     t.small_words = std::min(want, t.small_words_max);
     const hipError_t e = allow_small_lds(t.small_words_max * 4u);

@@ -91,6 +91,11 @@ struct Slot {
     uint32_t grid = 0;    // counting grid of the last enqueue
     uint32_t rows_grid = 0;  // row-kernel grid of the last split count
     bool split = false;      // the last enqueue was a split count
+    // split count: the list part's walk runs on `side` while the main stream
+    // transposes the top rows' in-edges (created on the first split count)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int ensure_side();
     uint32_t small = 0;   // graphs of the last enqueue taken by the small-graph kernel
     int create();
     void release_device();  // frees the device workspace (keeps stream/events)
@@ -122,6 +127,8 @@ struct Tuning {
     // kSmallWordsMax).
     uint32_t small_words = 0;
     uint32_t small_words_max = 0;
+    // LDS per block in bytes: the split count's row bitmap (a bits) must fit
+    size_t lds_bytes = 0;
     uint32_t grid_cache[2][2][4][4][kChunkMax + 2] = {};  // by kernel, offset width, ways, waves, staged
 };
 

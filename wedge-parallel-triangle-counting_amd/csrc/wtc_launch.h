@@ -65,6 +65,8 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
                                   uint32_t c2cap, uint32_t rows, uint64_t *meta, hipStream_t st);
 // Waves per block of the row kernel: 4, 8 or 16.
 bool valid_split_waves(int wpb);
+// LDS bytes of the row kernel's bitmap for `rows` top rows.
+size_t split_rows_lds(uint32_t rows);
 // Persistent grid of the row kernel for `rows` top rows (its LDS bitmap).
 hipError_t split_rows_grid(int ob, uint32_t rows, int wpb, int num_cus, uint32_t *grid);
 hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t rows, int wpb,
