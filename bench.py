@@ -427,31 +427,42 @@ def tune_sweep(args):
     ctx = wtc.Context(0)
     rows = []
 
-    def sweep_alone(g, family_skew, workload):
-        n, m, W = g.n, g.m, g.wedges()
-        dg = ctx.upload(g)
-        alens = [0] + [x for x in (2048, 8192, 32768, 131072, 262144) if x <= ((n + 31) // 32) * 32 and W > 0]
+    def sweep_alone(g, family_skew, workload, dg=None, big=False):
+        # big: a graph built on the GPU, where only the split count is swept
+        # (the walk's top-a bitmap in HBM takes a*a/8 bytes)
+        if dg is None:
+            n, m, W = g.n, g.m, g.wedges()
+            dg = ctx.upload(g)
+        else:  # W from a count's own wedge total (no host copy)
+            n, m = dg.n, dg.m
+            ctx.use_tuned(wtc.Tuned(12, 262144, wtc.KERNEL_SPLIT, 4, 4, 8))
+            W = dg.count(12, min(262144, ((n + 31) // 32) * 32), stats=True)[1].wedges
+        n32 = ((n + 31) // 32) * 32
+        alens = [0] + [x for x in (2048, 8192, 32768, 131072, 262144) if x <= n32 and W > 0]
+        # bitmap rows past 2^18 only for the split count (one row in LDS, no a*a/8 bytes in HBM)
+        split_alens = [x for x in alens if x] + [x for x in (524288, 1048576) if x <= n32 and W > 0]
         best = None
         base = wtc.Tuned(0, 0, wtc.KERNEL_WALK, 4, 4, 8)
-        for kernel in (wtc.KERNEL_WALK, wtc.KERNEL_SPLIT):
-            for a in alens:
-                if kernel == wtc.KERNEL_SPLIT and a == 0:
+        for kernel in ((wtc.KERNEL_SPLIT,) if big else (wtc.KERNEL_WALK, wtc.KERNEL_SPLIT)):
+            for a in (split_alens if kernel == wtc.KERNEL_SPLIT else alens):
+                if kernel == wtc.KERNEL_SPLIT and (a == 0 or (big and a < 131072)):
                     continue
                 # (the split count's spread shapes only its list part's walk)
                 log(f"tune {workload}: kernel {kernel} a {a}")
-                for sp in ((2, 4, 8, 16, 32, 64, 128) if kernel == wtc.KERNEL_WALK else (8, 12, 16)):
+                for sp in ((2, 4, 8, 16, 32, 64, 128) if kernel == wtc.KERNEL_WALK else (12,) if big else (8, 12, 16)):
                     if sp > 8 and W < (1 << 20):
                         continue
-                    t = wtc.Tuned(sp, a, kernel, base.waves_per_block, base.probe_ways, base.split_waves)
-                    ctx.use_tuned(t)
-                    ms, _ = _time_count(dg, sp, a)
-                    if best is None or ms < best[0]:
-                        best = (ms, t)
+                    for sw0 in ((8, 16) if kernel == wtc.KERNEL_SPLIT and a > 262144 else (base.split_waves,)):
+                        t = wtc.Tuned(sp, a, kernel, base.waves_per_block, base.probe_ways, sw0)
+                        ctx.use_tuned(t)
+                        ms, _ = _time_count(dg, sp, a, reps=2 if big else 3)
+                        if best is None or ms < best[0]:
+                            best = (ms, t)
         ms0, t0 = best
         log(f"tune {workload}: stage 1 best {ms0:.3f} ms {t0.as_dict()}")
-        for waves in (2, 4, 8):
-            for ways in (2, 4, 8):
-                for sw in ((4, 8, 16) if t0.kernel == wtc.KERNEL_SPLIT else (8,)):
+        for waves in ((4,) if big else (2, 4, 8)):
+            for ways in ((4,) if big else (2, 4, 8)):
+                for sw in ((8, 16) if big else (4, 8, 16) if t0.kernel == wtc.KERNEL_SPLIT else (8,)):
                     t = wtc.Tuned(t0.spread, t0.adjacency_matrix_len, t0.kernel, waves, ways, sw)
                     ctx.use_tuned(t)
                     ms, _ = _time_count(dg, t.spread, t.adjacency_matrix_len)
@@ -468,6 +479,8 @@ def tune_sweep(args):
 
     for scale in [int(x) for x in args.tune_scales.split(",") if x]:
         sweep_alone(WL.rmat(scale), "skew", f"rmat{scale}")
+    for scale in [int(x) for x in args.tune_device_scales.split(",") if x]:
+        sweep_alone(None, "skew", f"rmat{scale}", dg=wtc.DeviceGraph.gen_rmat(ctx, scale), big=True)
     for lg in [int(x) for x in args.tune_flat.split(",") if x]:
         n = 1 << lg
         g = wtc.preprocess(wtc.gen_erdos_renyi(n, 16.0 / (n - 1), seed=lg))
@@ -500,7 +513,8 @@ def tune_sweep(args):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as f:
         f.write("# Autotuner table for MI355X, written by: python bench.py --tune --tune-scales "
-                f"{args.tune_scales} --tune-flat {args.tune_flat} --tune-batch {args.tune_batch}\n")
+                f"{args.tune_scales} --tune-device-scales {args.tune_device_scales} --tune-flat {args.tune_flat} "
+                f"--tune-batch {args.tune_batch}\n")
         f.write("# family skew log2w spread a_div a_cap kernel waves ways split_waves  measured_ms workload\n")
         by_fam = {}
         for r in rows:
@@ -689,6 +703,7 @@ def offsets_u64_leg(ctx, g, W, spread, a, steps, warmup, expect):
         rows_ms = statistics.median(s["rows_ms"] for s in stats)
         dom_bytes = 4.0 * stats[-1]["top_wedges"] + 16.0 * stats[-1]["in_edges"]
         out["kernel_ms"]["split_rows"] = rows_ms
+        out["kernel_ms"]["split_slabs"] = statistics.median(s["list_ms"] for s in stats)
         out["roofline"] = {"kernel": "k_split_rows", "achieved": dom_bytes / (rows_ms / 1e3) / 1e9, "unit": "GB/s",
                            "frac": dom_bytes / (rows_ms / 1e3) / HBM_PEAK, "algorithmic_bytes": dom_bytes}
     return out
@@ -751,6 +766,8 @@ def main():
     ap.add_argument("--tune", action="store_true",
                     help="run the autotuner's sweep and write the tuning table the library loads")
     ap.add_argument("--tune-scales", default="12,14,16,18,20,22,24", help="R-MAT scales of the skewed buckets")
+    ap.add_argument("--tune-device-scales", default="26,28",
+                    help="R-MAT scales of the skewed buckets generated on the GPU (split count only)")
     ap.add_argument("--tune-flat", default="14,18,22", help="log2 n of the Erdos-Renyi (flat) buckets")
     ap.add_argument("--tune-batch", type=int, default=4096, help="graphs of the batch bucket (0: skip)")
     ap.add_argument("--tune-out", default="", help="where --tune writes the table (default: tuning/mi355x.tsv "
@@ -902,6 +919,7 @@ def main():
                       "prefix": statistics.median(s["prefix_ms"] for s in stats),
                       "bitmap_or_transposition": statistics.median(s["bitmap_ms"] for s in stats),
                       "split_rows": statistics.median(s["rows_ms"] for s in stats) if split else None,
+                      "split_slabs": statistics.median(s["list_ms"] for s in stats) if split else None,
                       "grid": stats[0]["blocks"]},
         "wedge_checks_per_s_kernel": W / (count_ms / 1e3),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,

@@ -194,11 +194,11 @@ typedef struct wtc_ctx wtc_ctx;       /* HIP stream + reusable workspace */
 typedef struct wtc_dgraph wtc_dgraph; /* oriented CSR resident in HBM    */
 
 typedef struct {
-    double prefix_ms;  /* wedge prefix scan + setup (split count: its three
-                          prefixes, range map and setup)                      */
+    double prefix_ms;  /* wedge prefix scan + setup (split count: its two
+                          prefixes and range map)                             */
     double bitmap_ms;  /* bitmap clear + build (split count: the counting
-                          sort of the top rows' in-edges)                     */
-    double count_ms;   /* counting kernel(s) (split count: row kernel + walk) */
+                          sort of the in-edges by middle vertex)              */
+    double count_ms;   /* counting kernel(s) (split count: row + slab kernels) */
     double total_ms;   /* first launch to count back on the host             */
     uint64_t wedges;   /* W counted                                          */
     uint32_t blocks;   /* counting-kernel grid                               */
@@ -208,6 +208,8 @@ typedef struct {
     double rows_ms;    /* split count: its row kernel alone                  */
     uint64_t top_wedges; /* split count: wedges probed in LDS (w in the top rows) */
     uint64_t in_edges; /* split count: in-edge tails the row kernel streamed */
+    double list_ms;    /* split count: its slab kernel alone (w below the top rows) */
+    uint64_t list_in_edges; /* split count: in-edge tails the slab kernel streamed */
 } wtc_count_stats;
 
 int wtc_device_count(int *count);

@@ -1,8 +1,9 @@
-"""GPU parity of the split count (KERNEL_SPLIT): the top-a wedges counted by
-middle vertex against an LDS bitmap row after a per-count transposition of
-the top rows' in-edges, the list-probe wedges by the row walk over their own
-prefix.  Every count, whole graph or wedge range, must equal the oracle's
-count of the same wedges (tc.cu:173-293 over [w_begin, w_end))."""
+"""GPU parity of the split count (KERNEL_SPLIT): every in-edge grouped by its
+middle vertex in a per-count transposition, the top-a rows' wedges counted
+against an LDS bitmap row, the others against LDS windows of colInd (or HBM
+for rows longer than a window holds).  Every count, whole graph or wedge
+range, must equal the oracle's count of the same wedges (tc.cu:173-293 over
+[w_begin, w_end))."""
 import numpy as np
 import pytest
 
@@ -17,8 +18,8 @@ def _rmat(scale, ef=16, seed=1):
 
 
 def _complete(k):
-    return wtc.preprocess(wtc.graph_from_pairs(
-        np.array([[i, j] for i in range(k) for j in range(i + 1, k)], np.uint32)))
+    i, j = np.triu_indices(k, 1)
+    return wtc.preprocess(wtc.graph_from_pairs(np.stack([i, j], axis=1).astype(np.uint32)))
 
 
 @pytest.fixture
@@ -110,4 +111,49 @@ def test_split_stats_report_the_full_order(split_ctx):
     cnt, st = dg.count(6, 2048, W // 4, W // 2, stats=True)
     assert cnt == O.tc_wedge(g, 6, 2048, W // 4, W // 2)
     assert st.wedges == W // 2 - W // 4
+    dg.free()
+
+
+def _partition(g, a):
+    """(top wedges, top in-edges, list in-edges) of the whole order: in-edges
+    v -> w with a non-empty tail, split by w >= thr (thr = n - a, or 0 when
+    a >= n), from the CSR alone."""
+    rp = np.asarray(g.row_ptr, np.int64)
+    ci = np.asarray(g.col_ind, np.int64)
+    d = np.diff(rp)
+    src = np.repeat(np.arange(g.n), d)
+    tail = d[src] - (np.arange(g.m) - rp[src]) - 1
+    thr = g.n - a if a < g.n else 0
+    top = (ci >= thr) & (tail > 0)
+    lst = (ci < thr) & (tail > 0)
+    return int(tail[top].sum()), int(top.sum()), int(lst.sum())
+
+
+def test_split_stats_partition_the_wedges(split_ctx):
+    """The row kernel's wedges and both kernels' in-edges equal the CSR's own
+    split of the in-edges at thr, for several bitmap lengths."""
+    g = _rmat(13, seed=12)
+    expect = O.tc_merge(g)
+    W = g.wedges()
+    dg = split_ctx.upload(g)
+    for a in (32, 512, 2048, 8192):
+        cnt, st = dg.count(6, a, stats=True)
+        assert cnt == expect and st.split == 1 and st.wedges == W
+        assert (st.top_wedges, st.in_edges, st.list_in_edges) == _partition(g, a), a
+    dg.free()
+
+
+def test_split_long_list_rows_closed_form(split_ctx):
+    """K_4200 with a = 32 or 64: the list rows w < 103 have out-degree above
+    the slab window's row limit (4096), so their ids are searched in HBM;
+    every wedge closes, so any range [lo, hi) holds hi - lo triangles."""
+    g = _complete(4200)
+    W = g.wedges()
+    dg = split_ctx.upload(g)
+    rng = np.random.default_rng(17)
+    for a in (32, 64):
+        assert dg.count(8, a) == W
+        for lo, hi in _ranges(W, rng, k=4):
+            lo_c, hi_c = min(lo, W), min(hi, W)
+            assert dg.count(8, a, lo, hi) == max(0, hi_c - lo_c), (a, lo, hi)
     dg.free()

@@ -96,29 +96,17 @@ void Slot::release_device() {
     sp.release();
 }
 
-int Slot::ensure_side() {
-    if (side) return WTC_OK;
-    WTC_HIPC(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    WTC_HIPC(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    WTC_HIPC(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-    return WTC_OK;
-}
-
 void Slot::destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (side) (void)hipStreamSynchronize(side);
     release_device();
     h_desc.release();
     h_out.release();
     for (auto &e : ev)
         if (e) (void)hipEventDestroy(e);
     if (ev_ready) (void)hipEventDestroy(ev_ready);
-    if (ev_fork) (void)hipEventDestroy(ev_fork);
-    if (ev_join) (void)hipEventDestroy(ev_join);
-    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
-    stream = side = nullptr;
-    ev_ready = ev_fork = ev_join = nullptr;
+    stream = nullptr;
+    ev_ready = nullptr;
     for (auto &e : ev) e = nullptr;
 }
 
@@ -165,15 +153,20 @@ namespace {
 // Byte offsets of the pieces of a split count's workspace in one buffer
 // (each on a 256-byte boundary), and the buffer's size.
 struct SplitLayout {
-    size_t kpos, wsfull, hist1, part_base, c2base, hist2, tot, recid, rec, rpin, ie, total;
-    uint32_t chunks, parts, c2cap;
+    size_t wsfull, hist1, part_base, c2base, hist2, tot, recid, rec, recaux, rpin, ie, ieaux, total;
+    uint32_t chunks, slabs, keys, parts, c2cap;
     uint64_t cap;
 };
+
+// Slabs of the list rows of a graph of m edges.
+uint64_t split_slabs(uint64_t m) { return (m + kSlabSpan - 1) / kSlabSpan; }
 
 SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
     SplitLayout L{};
     L.chunks = std::min<uint32_t>(kSplitChunksMax, std::max<uint32_t>(1, n / kSplitChunkVerts));
-    L.parts = (rows + kSplitPart - 1) / kSplitPart;
+    L.slabs = (uint32_t)split_slabs(m);
+    L.keys = rows + L.slabs;
+    L.parts = (L.keys + kSplitPart - 1) / kSplitPart;
     L.cap = std::max<uint64_t>(m, 1);  // an in-edge is an edge
     L.c2cap = (uint32_t)((L.cap + kSplitChunk2 - 1) / kSplitChunk2) + L.parts;
     size_t off = 0;
@@ -182,17 +175,18 @@ SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
         off = (off + bytes + 255) & ~(size_t)255;
         return at;
     };
-    L.kpos = take((size_t)n * sizeof(uint32_t));
     L.wsfull = take(2 * ((size_t)n + 1) * sizeof(uint64_t));  // ws_full, then ie_pre
     L.hist1 = take((size_t)L.chunks * L.parts * sizeof(uint64_t));
     L.part_base = take(((size_t)L.parts + 1) * sizeof(uint64_t));
     L.c2base = take(((size_t)L.parts + 1) * sizeof(uint32_t));
     L.hist2 = take((size_t)L.c2cap * kSplitPart * sizeof(uint32_t));
-    L.tot = take((size_t)rows * sizeof(uint32_t));
+    L.tot = take((size_t)L.keys * sizeof(uint32_t));
     L.recid = take((size_t)L.cap * sizeof(uint32_t));
     L.rec = take((size_t)L.cap * 2 * (size_t)ob);
-    L.rpin = take(((size_t)rows + 1) * (size_t)ob);
+    L.recaux = take((size_t)L.cap * sizeof(uint32_t));
+    L.rpin = take(((size_t)L.keys + 1) * (size_t)ob);
     L.ie = take((size_t)L.cap * 2 * (size_t)ob);
+    L.ieaux = take((size_t)L.cap * sizeof(uint32_t));
     L.total = off;
     return L;
 }
@@ -204,30 +198,34 @@ size_t split_workspace_bytes(uint32_t n, uint64_t m, uint32_t rows, int ob) { re
 void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t rows, int ob) {
     const SplitLayout L = split_plan(n, m, rows, ob);
     sd->rows = rows;
+    sd->keys = L.keys;
+    sd->slabs = L.slabs;
     sd->chunks = L.chunks;
     sd->parts = L.parts;
     sd->cap = L.cap;
     sd->c2cap = L.c2cap;
-    sd->kpos = reinterpret_cast<uint32_t *>(base + L.kpos);
     sd->ws_full = reinterpret_cast<uint64_t *>(base + L.wsfull);
     sd->ie_pre = sd->ws_full + (size_t)n + 1;
     sd->hist1 = reinterpret_cast<uint64_t *>(base + L.hist1);
     sd->part_base = reinterpret_cast<uint64_t *>(base + L.part_base);
     sd->c2base = reinterpret_cast<uint32_t *>(base + L.c2base);
     sd->hist2 = reinterpret_cast<uint32_t *>(base + L.hist2);
-    sd->row_tot = reinterpret_cast<uint32_t *>(base + L.tot);
+    sd->key_tot = reinterpret_cast<uint32_t *>(base + L.tot);
     sd->rec_id = reinterpret_cast<uint32_t *>(base + L.recid);
     sd->rec = base + L.rec;
+    sd->rec_aux = reinterpret_cast<uint32_t *>(base + L.recaux);
     sd->rp_in = base + L.rpin;
     sd->ie = base + L.ie;
+    sd->ie_aux = reinterpret_cast<uint32_t *>(base + L.ieaux);
 }
 
-// A bitmap length whose row does not fit one block's LDS is counted by the
-// row walk against the bitmap in HBM, as the reference does.
-bool split_eligible(const Tuning &t, const GraphDesc &d) {
+// A bitmap length whose row does not fit one block's LDS (or a graph whose
+// keys overflow the transposition's parts) is counted by the row walk
+// against the bitmap in HBM, as the reference does.
+bool split_eligible(const Tuning &t, const GraphDesc &d, uint64_t m) {
     return t.kernel == kKernelSplit && d.mode == kModeTiles && d.a > 0 && d.bm_rows > 0 &&
-           d.bm_rows <= kSplitPartsMax * kSplitPart && split_rows_lds(d.bm_rows) <= t.lds_bytes &&
-           (reinterpret_cast<uintptr_t>(d.colInd) & 15u) == 0;
+           (uint64_t)d.bm_rows + split_slabs(m) <= (uint64_t)kSplitPartsMax * kSplitPart &&
+           split_rows_lds(d.bm_rows) <= t.lds_bytes && (reinterpret_cast<uintptr_t>(d.colInd) & 15u) == 0;
 }
 
 int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *plan, const uint64_t *m,
@@ -263,7 +261,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     // The split count takes a single graph with a bitmap length whose colInd
     // is 16-byte aligned (its row kernel loads four ids at a time); its top
     // rows then need no bitmap in HBM.
-    const bool split = G == 1 && m != nullptr && split_eligible(t, hd[0]) && (!t.split_needs_base || split_base);
+    const bool split = G == 1 && m != nullptr && split_eligible(t, hd[0], m[0]) && (!t.split_needs_base || split_base);
     const uint32_t split_rows = split ? hd[0].bm_rows : 0;
     if (split) {
         hd[0].bm_rows = 0;
@@ -275,7 +273,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     for (const auto &d : hd) {
         if (d.n == 0 || d.spread == 0) return WTC_ERR_ARG;
         if (d.mode == kModeTiles) max_spread = std::max(max_spread, d.spread);
-        if (!d.ws) ws_n += (uint64_t)d.n + 1;
+        if (!d.ws && !split) ws_n += (uint64_t)d.n + 1;  // the split count keeps its prefixes in its workspace
         if (d.bm_rows && !d.bitmap) bm_words += bm_words_for(d.a);
         rows += d.bm_rows;
         blocks += prefix_blocks(d.n);
@@ -283,7 +281,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     if (blocks > 0x7fffffffULL) return WTC_ERR_RANGE;
     const bool single = G == 1;
     if (ws_n) WTC_HIPC(s.ws.ensure(ws_n * sizeof(uint64_t)));
-    WTC_HIPC(s.partials.ensure((split ? 3 : 1) * blocks * sizeof(uint64_t)));
+    WTC_HIPC(s.partials.ensure((split ? 2 : 1) * blocks * sizeof(uint64_t)));
     if (single) WTC_HIPC(s.tree.ensure(kTreeSize * sizeof(uint64_t)));
     // descriptors, then the block list of the small-graph kernel
     if (small_parts > 0x7fffffffULL) return WTC_ERR_RANGE;
@@ -299,7 +297,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     for (auto &d : hd) {
         d.bm_clear_base = clear_off;
         clear_off += d.bm_rows ? bm_words_for(d.a) : 0;
-        if (!d.ws) {
+        if (!d.ws && !split) {
             d.ws = s.ws.as<uint64_t>() + ws_off;
             ws_off += (uint64_t)d.n + 1;
         }
@@ -321,8 +319,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
         }
         split_layout(&sd, base, hd[0].n, m[0], split_rows, ob);
         WTC_HIPC(split_rows_grid(ob, split_rows, t.split_wpb, t.num_cus, &s.rows_grid));
-        int rc = s.ensure_side();
-        if (rc) return rc;
+        WTC_HIPC(split_slabs_grid(ob, t.num_cus, &s.slabs_grid));
     }
     std::memcpy(s.h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
     if (split) std::memcpy(s.h_desc.as<char>() + split_off, &sd, sizeof(SplitDesc));
@@ -335,7 +332,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
 
     const int oi = ob == 8 ? 1 : 0;
     const uint32_t staged = std::min<uint32_t>(max_spread, t.chunk);
-    // the split count's list part, and a set the split count does not take, use the row walk
+    // a set the split count does not take uses the row walk
     const int kernel = t.kernel == kKernelStaged ? kKernelStaged : kKernelWalk;
     uint32_t &gc = t.grid_cache[kernel == kKernelWalk ? 0 : 1][oi][ways_index(t.ways)][wpb_index(t.wpb)][staged];
     if (!gc) WTC_HIPC(count_grid(kernel, ob, staged, t.ways, t.wpb, t.num_cus, &gc));
@@ -363,6 +360,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->split_off = split_off;
     plan->split_rows = split_rows;
     plan->split_chunks = sd.chunks;
+    plan->split_keys = sd.keys;
     plan->split_c2cap = sd.c2cap;
     plan->split_wpb = t.split_wpb;
     return WTC_OK;
@@ -377,24 +375,15 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
         SplitDesc *sd = reinterpret_cast<SplitDesc *>(s.desc.as<char>() + p.split_off);
         if (timed) WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
         WTC_HIPC(launch_split_prefix(ddesc, sd, p.ob, p.blocks, s.partials.as<uint64_t>(), s.stream));
-        WTC_HIPC(launch_split_range(ddesc, sd, s.stream));
-        WTC_HIPC(launch_setup(ddesc, 1, tree, meta, dcounts, s.stream));
+        WTC_HIPC(launch_split_range(ddesc, sd, p.ob, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
-        // the list part's walk needs only the prefix, tree and tiles: it runs
-        // on the side stream while the main stream transposes (both add into
-        // the same count; they share no other device state)
-        WTC_HIPC(hipEventRecord(s.ev_fork, s.stream));
-        WTC_HIPC(hipStreamWaitEvent(s.side, s.ev_fork, 0));
-        WTC_HIPC(launch_count(p.kernel, ddesc, 1, p.ob, tree, meta, dcounts, p.max_spread, p.chunk, p.ways,
-                              p.wpb, p.work, s.grid, s.side));
-        WTC_HIPC(hipEventRecord(s.ev_join, s.side));
-        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_rows, meta,
+        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_keys, meta, dcounts,
                                         s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
         WTC_HIPC(launch_split_rows(ddesc, sd, p.ob, p.split_rows, p.split_wpb, meta, dcounts, s.rows_grid,
                                    s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[4], s.stream));
-        WTC_HIPC(hipStreamWaitEvent(s.stream, s.ev_join, 0));
+        WTC_HIPC(launch_split_slabs(ddesc, sd, p.ob, meta, dcounts, s.slabs_grid, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
         WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + p.G) * sizeof(uint64_t),
                                 hipMemcpyDeviceToHost, s.stream));
@@ -457,8 +446,10 @@ int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt) {
         rt->split = s.split;
         if (s.split) {
             WTC_HIPC(hipEventElapsedTime(&rt->rows_ms, s.ev[2], s.ev[4]));
+            WTC_HIPC(hipEventElapsedTime(&rt->list_ms, s.ev[4], s.ev[3]));
             rt->top_wedges = h[kMetaTopWedges];
             rt->in_edges = h[kMetaInEdges];
+            rt->list_in_edges = h[kMetaAllEdges] - h[kMetaInEdges];
         }
     }
     return WTC_OK;
@@ -533,6 +524,8 @@ int run_set(wtc_ctx *c, int ob, std::vector<GraphDesc> &hd, uint64_t *counts, wt
         st->rows_ms = rt.rows_ms;
         st->top_wedges = rt.top_wedges;
         st->in_edges = rt.in_edges;
+        st->list_ms = rt.list_ms;
+        st->list_in_edges = rt.list_in_edges;
     }
     return WTC_OK;
 }

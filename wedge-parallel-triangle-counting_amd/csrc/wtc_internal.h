@@ -84,18 +84,14 @@ struct Slot {
     hipEvent_t ev[5] = {};  // prefix start, bitmap start, count start, count end, split row kernel end
     hipEvent_t ev_ready = nullptr;  // input copied (waited on by the slot's stream)
     DevBuf ws, partials, tree, bm, desc, out;
-    // split count: kpos, full prefix, transposition counts/totals/offsets, in-edge tails
+    // split count: prefixes, transposition counts/totals/offsets, in-edge records
     DevBuf sp;  // split count workspace (split_layout)
     PinBuf h_desc, h_out;
     uint32_t G = 0;       // graphs of the last enqueue
     uint32_t grid = 0;    // counting grid of the last enqueue
-    uint32_t rows_grid = 0;  // row-kernel grid of the last split count
-    bool split = false;      // the last enqueue was a split count
-    // split count: the list part's walk runs on `side` while the main stream
-    // transposes the top rows' in-edges (created on the first split count)
-    hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    int ensure_side();
+    uint32_t rows_grid = 0;   // row-kernel grid of the last split count
+    uint32_t slabs_grid = 0;  // slab-kernel grid of the last split count
+    bool split = false;       // the last enqueue was a split count
     uint32_t small = 0;   // graphs of the last enqueue taken by the small-graph kernel
     int create();
     void release_device();  // frees the device workspace (keeps stream/events)
@@ -142,9 +138,9 @@ int init_small_graphs(Tuning &t, const hipDeviceProp_t &prop, uint32_t want);
 
 struct RunTimes {
     float prefix_ms = 0, bitmap_ms = 0, count_ms = 0;
-    float rows_ms = 0;  // split count: its row kernel
+    float rows_ms = 0, list_ms = 0;  // split count: its row and slab kernels
     bool split = false;
-    uint64_t top_wedges = 0, in_edges = 0;
+    uint64_t top_wedges = 0, in_edges = 0, list_in_edges = 0;
 };
 
 uint64_t bm_words_for(uint32_t a);
@@ -172,7 +168,7 @@ struct SetPlan {
     // split count: SplitDesc at byte split_off of the descriptor buffer
     bool split = false;
     size_t split_off = 0;
-    uint32_t split_rows = 0, split_chunks = 0, split_c2cap = 0, split_wpb = 8;
+    uint32_t split_rows = 0, split_keys = 0, split_chunks = 0, split_c2cap = 0, split_wpb = 8;
 };
 // Sizes the slot's workspace for the set, assigns each graph its prefix and
 // bitmap (unless given), fills the pinned host descriptors and the plan.
@@ -195,7 +191,7 @@ int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, const ui
 // a graph of n vertices, m edges and `rows` top rows at offset width ob, and
 // the SplitDesc pointers into a workspace of that size at base.  stage_set
 // takes the workspace from split_base when given, else from the slot.
-bool split_eligible(const Tuning &t, const GraphDesc &d);
+bool split_eligible(const Tuning &t, const GraphDesc &d, uint64_t m);
 size_t split_workspace_bytes(uint32_t n, uint64_t m, uint32_t rows, int ob);
 void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t rows, int ob);
 // Waits for the slot (or, if it is done, does not wait) and reads results.

@@ -76,58 +76,86 @@ struct GraphDesc {
 static_assert(sizeof(GraphDesc) == 128, "GraphDesc layout");
 
 // meta[] slots written by the setup kernel
-// (kMetaNext..kMetaNext+7: per-XCD work counters of the counting kernel;
-// kMetaRows: tile counter of the split count's row kernel; kMetaTopWedges,
-// kMetaInEdges: the split count's wedges probed in LDS and in-edge tails)
+// (kMetaNext..kMetaNext+7: per-XCD work counters of the counting kernel).
+// The split count: kMetaRows and kMetaSlabs, tile counters of its row and
+// slab kernels; kMetaTopWedges and kMetaListWedges, the wedges each of them
+// checked; kMetaInEdges, the top rows' in-edges; kMetaAllEdges, every
+// grouped in-edge.
 enum : int {
     kMetaTiles = 0,
     kMetaWedges = 1,
     kMetaRows = 2,
     kMetaTopWedges = 3,
     kMetaInEdges = 4,
+    kMetaSlabs = 5,
+    kMetaListWedges = 6,
+    kMetaAllEdges = 7,
     kMetaNext = 8,
     kMetaSlots = 16
 };
 
-// Split count of one graph (wtc_split.hip).  Top rows per part of the
+// Split count of one graph (wtc_split.hip).  Keys per part of the
 // transposition's second level (its LDS counters), the most parts, records
 // per second-level block, threads per transposition block, and in-edges per
 // tile of the row kernel.
 constexpr uint32_t kSplitPart = 32768;
-constexpr uint32_t kSplitPartsMax = 64;
+constexpr uint32_t kSplitPartsMax = 256;
 constexpr uint32_t kSplitChunk2 = 262144;
 constexpr int kSplitBlock = 1024;
 constexpr uint32_t kSplitTile = 1024;
+// Slabs of the list rows (middle vertex below thr): slab s holds the rows
+// whose adjacency starts in colInd[s*kSlabSpan, (s+1)*kSlabSpan).  The slab
+// kernel keeps colInd[s*kSlabSpan, +kSlabWin) in LDS, which holds every such
+// row of out-degree <= kSlabRowMax (longer rows are searched in HBM).
+constexpr uint32_t kSlabSpan = 8192;
+constexpr uint32_t kSlabRowMax = 4096;
+constexpr uint32_t kSlabWin = kSlabSpan + kSlabRowMax;
+constexpr uint32_t kSlabTile = 1024;
 
 // Workspace and device-filled state of a split count.  The top rows are the
 // ranks [thr, n) of the graph's GraphDesc; OffT arrays use the set's offset
-// width.  The in-edges of the top rows are grouped by row in two levels:
-// first into one bucket per part of kSplitPart rows (records: id and tail),
-// then each part's records by row (the tails only, in ie).
+// width.  Every in-edge (v, j) (j a position of adj+(v) other than its last)
+// is grouped by a key: w - thr for w = colInd[j] >= thr (a top row), else
+// rows + the slab of w.  Two levels: first into one bucket per part of
+// kSplitPart keys (records: key and tail), then each part's records by key
+// (the tails only, in ie).  A top record is the tail [lo, hi) of colInd; a
+// list record is (j, hi), its tail [j + 1, hi) (the one in-edge whose tail
+// the range cuts at the front: [clip_lo, hi)).
 struct SplitDesc {
-    uint32_t *kpos;       // n: entries of adj+(v) below thr
-    uint64_t *ws_full;    // n+1: the full wedge prefix (ws of the GraphDesc holds the list part's)
-    uint64_t *ie_pre;     // n+1: prefix of the top in-edges (entries >= thr, not last) of each row
+    uint64_t *ws_full;    // n+1: the full wedge prefix
+    uint64_t *ie_pre;     // n+1: prefix of the in-edges (positions of adj+(v) but the last) of each row
     uint64_t *hist1;      // chunks x parts: first-level counts, then record offsets
     uint64_t *part_base;  // parts+1: first record of each part; part_base[parts] = the in-edges
     uint32_t *c2base;     // parts+1: first second-level block of each part
     uint32_t *hist2;      // c2cap x kSplitPart: second-level counts, then column prefixes
-    uint32_t *row_tot;    // rows: in-edges of each top row
-    uint32_t *rec_id;     // cap: first-level records, the in-edge's id
-    void *rec;            // cap pairs of OffT: first-level records, the in-edge's tail
-    void *rp_in;          // rows+1 OffT: first in-edge of each top row
-    void *ie;             // cap pairs of OffT: the tail [lo, hi) of colInd of each in-edge, by row
-    uint64_t cap;         // in-edges the record and ie arrays hold (>= the top in-edges)
+    uint32_t *key_tot;    // keys: in-edges of each key
+    uint32_t *rec_id;     // cap: first-level records, the in-edge's key
+    void *rec;            // cap pairs of OffT: first-level records
+    uint32_t *rec_aux;    // cap: first-level list records, where adj+(w) lies (slab_aux)
+    void *rp_in;          // keys+1 OffT: first in-edge of each key
+    void *ie;             // cap pairs of OffT: the records grouped by key
+    uint32_t *ie_aux;     // cap: the list records' slab_aux words, beside ie
+    uint64_t cap;         // in-edges the record and ie arrays hold (>= the in-edges)
     uint32_t rows;        // top rows: n - thr
+    uint32_t keys;        // rows + slabs
     uint32_t chunks;      // vertex chunks of the first level (about equal shares of the in-edges)
-    uint32_t parts;       // row parts of kSplitPart rows (<= kSplitPartsMax)
+    uint32_t parts;       // key parts of kSplitPart keys (<= kSplitPartsMax)
     uint32_t c2cap;       // second-level blocks launched (>= those the records need)
+    uint32_t slabs;       // slabs of the list rows: ceil(m / kSlabSpan)
     // device-filled by the range kernel
     uint64_t w_lo, w_hi;  // the counted wedges of the full order, clamped to [0, W]
+    uint64_t clip_j;      // the in-edge whose tail the range cuts at the front (~0: none)
+    uint64_t clip_lo;     // that tail's first position
     uint32_t v_lo, v_hi;  // vertices holding w_lo and w_hi - 1 (v_lo > v_hi: none)
     uint32_t clip;        // 1 unless [w_lo, w_hi) is the whole order
     uint32_t pad_;
 };
-static_assert(sizeof(SplitDesc) == 152, "SplitDesc layout");
+static_assert(sizeof(SplitDesc) == 184, "SplitDesc layout");
+
+// Where adj+(w) of a list row lies in its slab's window: bits 0-12 its start
+// rowPtr[w] - slab*kSlabSpan, bits 13-25 its length (kSlabAuxLong: longer
+// than kSlabRowMax, searched in HBM).
+constexpr uint32_t kSlabAuxLong = 0x1fffu;
+static_assert(kSlabSpan <= 8192 && kSlabRowMax < kSlabAuxLong, "slab_aux fields");
 
 }  // namespace wtc

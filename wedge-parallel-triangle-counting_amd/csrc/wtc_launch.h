@@ -39,9 +39,9 @@ bool valid_count_waves(int wpb);
 // Counting kernels: kKernelWalk (k_walk: waves walk the wedge order in steps
 // of 64 consecutive wedges) and kKernelStaged (k_count: per-lane spreads
 // staged through LDS).  Both count the same wedges of the same tiles.
-// kKernelSplit: a single graph with a > 0 counted by the split count (the
-// top-a part grouped by middle vertex against one LDS bitmap row, the rest
-// by k_walk over the list part's own prefix; wtc_split.hip); any other set
+// kKernelSplit: a single graph with a > 0 counted by the split count (its
+// in-edges grouped by middle vertex: the top-a rows against one LDS bitmap
+// row, the rest against LDS windows of colInd; wtc_split.hip); any other set
 // with it by k_walk.
 enum : int { kKernelWalk = 0, kKernelStaged = 1, kKernelSplit = 2 };
 // `ways`: independent probes each lane advances together (1, 2, 4 or 8; for
@@ -54,15 +54,17 @@ hipError_t count_grid(int kernel, int ob, uint32_t staged, int ways, int wpb, in
 hipError_t launch_count(int kernel, const GraphDesc *desc, uint32_t G, int ob, const uint64_t *tree, uint64_t *meta,
                         unsigned long long *counts, uint32_t max_spread, uint32_t chunk, int ways, int wpb,
                         uint32_t work, uint32_t grid, hipStream_t st);
-// Split count (wtc_split.hip).  partials holds 3*nblocks entries.
+// Split count (wtc_split.hip).  partials holds 2*nblocks entries.
 hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t nblocks,
                                uint64_t *partials, hipStream_t st);
-// Maps the GraphDesc's range onto the list part (before launch_setup).
-hipError_t launch_split_range(GraphDesc *desc, SplitDesc *sd, hipStream_t st);
-// The two-level counting sort of the top rows' in-edges (chunks first-level
-// blocks, c2cap second-level blocks); zeroes meta[kMetaRows].
+// The counted range's ends and the in-edge it cuts at the front.
+hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipStream_t st);
+// The two-level counting sort of the in-edges by key (chunks first-level
+// blocks, c2cap second-level blocks); zeroes the tile counters, the wedge
+// tallies and counts[0].
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
-                                  uint32_t c2cap, uint32_t rows, uint64_t *meta, hipStream_t st);
+                                  uint32_t c2cap, uint32_t keys, uint64_t *meta, unsigned long long *counts,
+                                  hipStream_t st);
 // Waves per block of the row kernel: 4, 8 or 16.
 bool valid_split_waves(int wpb);
 // LDS bytes of the row kernel's bitmap for `rows` top rows.
@@ -71,6 +73,10 @@ size_t split_rows_lds(uint32_t rows);
 hipError_t split_rows_grid(int ob, uint32_t rows, int wpb, int num_cus, uint32_t *grid);
 hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t rows, int wpb,
                              uint64_t *meta, unsigned long long *counts, uint32_t grid, hipStream_t st);
+// Persistent grid of the slab kernel (the list rows).
+hipError_t split_slabs_grid(int ob, int num_cus, uint32_t *grid);
+hipError_t launch_split_slabs(const GraphDesc *desc, const SplitDesc *sd, int ob, uint64_t *meta,
+                              unsigned long long *counts, uint32_t grid, hipStream_t st);
 // Vertex chunks of the transposition (equal shares of the in-edges): one
 // per kSplitChunkVerts vertices, at least 1 and at most kSplitChunksMax.
 constexpr uint32_t kSplitChunksMax = 1024;

@@ -1,35 +1,35 @@
 // wtc_split.hip -- the split count of one graph (counting kernel choice
-// kKernelSplit).  Every wedge (v, w, u) of a degree-oriented CSR is counted
-// by one of two paths, chosen by its middle vertex w:
+// kKernelSplit).  Every wedge (v, w, u) of a degree-oriented CSR belongs to
+// one in-edge v -> w: w = colInd[j] at a position j of adj+(v) that is not
+// the row's last, and u runs over the row's tail colInd[j+1 .. e).  The
+// in-edge closes |tail ∩ adj+(w)| triangles.  The in-edges are grouped by
+// their middle vertex w with a counting sort built per count, and counted in
+// two kernels chosen by w:
 //
-//  * w among the top-a ranks (w >= thr = n - a): the wedges whose probe the
-//    reference answers from its top-a bitmap (tc.cu:262-272).  They are
-//    counted grouped by w.  The in-edges v -> w (j the position of w in
-//    adj+(v), e the row end, tail colInd[j+1 .. e)) are grouped by w with a
-//    counting sort of the top-a columns, built per count.  A block of the
-//    row kernel holds adj+(w) as a bitmap of a bits in LDS, and its waves
-//    stream the tails of w's in-edges through it, four ids per lane per
-//    16-byte load.  Each probe is one LDS read, and the top-a bitmap never
-//    exists in HBM: one row of it at a time in LDS instead of a*a/8 bytes.
-//  * w below thr: the sorted-list probes of tc.cu:273-286.  For a vertex v
-//    with k entries of adj+(v) below thr these are the first
-//    C(d,2) - C(d-k,2) wedges of its row-major order, so the list part is a
-//    wedge order of its own -- a prefix of every vertex's -- with its own
-//    prefix (the GraphDesc's ws), counted by the row walk of k_walk.
+//  * w among the top-a ranks (w >= thr): the wedges whose probe the
+//    reference answers from its top-a bitmap (tc.cu:262-272).  One key per
+//    row.  A block of the row kernel holds adj+(w) as a bitmap of a bits in
+//    LDS, and its waves stream the tails of w's in-edges through it, four
+//    ids per lane per 16-byte load.  Each probe is one LDS read, and the
+//    top-a bitmap never exists in HBM: one row of it at a time in LDS instead
+//    of a*a/8 bytes.
+//  * w below thr: the sorted-list probes of tc.cu:273-286.  One key per slab
+//    of colInd (kSlabSpan positions; a row belongs to the slab its adjacency
+//    starts in).  A block of the slab kernel copies the slab's window of
+//    colInd into LDS once, and its waves stream the tails of the slab's
+//    in-edges, searching each id in adj+(w) inside that window.
 //
 // A wedge range [w_begin, w_end) of the full order (the reference's wedge
-// numbering, tc.cu:203-212) maps to one contiguous range of the list part's
-// order and to the in-edges of the vertices holding it, with the tails of
-// the two end vertices clipped.
+// numbering, tc.cu:203-212) holds the in-edges of the vertices between its
+// ends, with the tails of the two end vertices clipped.
 //
-// Pipeline on one stream: k_split_partials, k_scan_partials (three times)
-// and k_split_scan (the two wedge prefixes and the top in-edge prefix);
-// k_split_range; k_setup (tiles and search tree of the list part); the
-// transposition in two counting-sort levels, k_split_l1<count>,
-// k_split_l1scan, k_split_l1<scatter> (records bucketed by part of
-// kSplitPart rows), k_split_l2<count>, k_split_l2cols, k_split_rowscan,
-// k_split_l2<scatter> (each part's records grouped by row); k_split_rows
-// (top-a part); k_walk (list part).
+// Pipeline on one stream: k_split_partials, k_scan_partials (twice) and
+// k_split_scan (the full wedge prefix and the in-edge prefix);
+// k_split_range; the transposition in two counting-sort levels,
+// k_split_l1<count>, k_split_l1scan, k_split_l1<scatter> (records bucketed
+// by part of kSplitPart keys), k_split_l2<count>, k_split_l2cols,
+// k_split_rowscan, k_split_l2<scatter> (each part's records grouped by key);
+// k_split_rows (top rows); k_split_slabs (list rows).
 #include "wtc_dev.h"
 #include "wtc_kernels.h"
 #include "wtc_launch.h"
@@ -38,178 +38,145 @@ namespace wtc {
 
 namespace {
 
-// Entries of the ascending row colInd[b, e) below x.
-template <typename OffT>
-__device__ __forceinline__ uint32_t entries_below(const uint32_t *__restrict__ ci, OffT b, OffT e, uint32_t x) {
-    if (b == e || ci[b] >= x) return 0u;
-    if (ci[e - 1] < x) return (uint32_t)(e - b);
-    OffT lo = b + 1, hi = e - 1;  // the first entry >= x lies in [lo, hi]
-    while (lo < hi) {
-        const OffT mid = lo + ((hi - lo) >> 1);
-        if (ci[mid] < x) lo = mid + 1;
-        else hi = mid;
-    }
-    return (uint32_t)(lo - b);
-}
-
-// List-part wedges of a vertex with out-degree d and k entries below thr:
-// the pairs (i, t), i < t < d, with i < k.
-__device__ __forceinline__ uint64_t list_wedges(uint64_t d, uint64_t k) { return binom2_d(d) - binom2_d(d - k); }
-
-// Position in the list part's order of position x of the full order.
-__device__ uint64_t list_pos(const uint64_t *ws, const uint64_t *wf, uint32_t n, uint64_t x) {
-    if (x >= wf[n]) return ws[n];
-    const uint32_t v = rightmost_le(wf, 0u, n - 1, x);
-    const uint64_t off = x - wf[v], cnt = ws[v + 1] - ws[v];
-    return ws[v] + (off < cnt ? off : cnt);
-}
-
 // Four ids loaded by one 16-byte load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <typename OffT>
 struct TailPair {
-    OffT lo, hi;  // colInd positions [lo, hi) of one in-edge's tail
+    OffT lo, hi;  // a top record: colInd positions [lo, hi) of one in-edge's tail;
+                  // a list record: (j, hi), the tail [j + 1, hi)
 };
+
+// Wedges before row i of a vertex of out-degree d in its row-major order
+// (row r holds d - 1 - r wedges).
+__device__ __forceinline__ uint64_t row_base(uint64_t i, uint64_t d) { return i * (d - 1) - binom2_d(i); }
 
 }  // namespace
 
 // --------------------------------------------------------------- prefixes
 
-// Top in-edges of a row of out-degree d with k entries below thr: the
-// positions j >= k of its entries that are not the row's last.
-__device__ __forceinline__ uint64_t top_in_edges(uint64_t d, uint64_t k) { return d > k + 1 ? d - k - 1 : 0; }
+// In-edges of a row of out-degree d: every position but the last.
+__device__ __forceinline__ uint64_t in_edges_of(uint64_t d) { return d ? d - 1 : 0; }
 
-// Block b: vertices [b*kScanChunk, (b+1)*kScanChunk): kpos, and the sums of
-// the list part's wedges, the full order's wedges and the top in-edges.
+// Block b: vertices [b*kScanChunk, (b+1)*kScanChunk): the sums of the full
+// order's wedges and of the in-edges.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void k_split_partials(const GraphDesc *__restrict__ desc,
-                                                           const SplitDesc *__restrict__ sd,
-                                                           uint64_t *__restrict__ part_list,
                                                            uint64_t *__restrict__ part_full,
                                                            uint64_t *__restrict__ part_in) {
-    __shared__ uint64_t s_l[kWavesPerBlock], s_f[kWavesPerBlock], s_i[kWavesPerBlock];
+    __shared__ uint64_t s_f[kWavesPerBlock], s_i[kWavesPerBlock];
     const GraphDesc &D = desc[0];
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
-    const uint32_t *__restrict__ colInd = D.colInd;
-    uint32_t *__restrict__ kpos = sd->kpos;
-    const uint32_t n = D.n, thr = D.thr;
+    const uint32_t n = D.n;
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
-    uint64_t al = 0, af = 0, ai = 0;
+    uint64_t af = 0, ai = 0;
     for (int k = 0; k < kScanItems; k++) {
         const uint64_t v = base + (uint64_t)k * kBlock + threadIdx.x;
         if (v < n) {
-            const OffT b = rowPtr[v], e = rowPtr[v + 1];
-            const uint32_t kb = entries_below<OffT>(colInd, b, e, thr);
-            kpos[v] = kb;
-            const uint64_t d = (uint64_t)(e - b);
-            al += list_wedges(d, kb);
+            const uint64_t d = (uint64_t)(rowPtr[v + 1] - rowPtr[v]);
             af += binom2_d(d);
-            ai += top_in_edges(d, kb);
+            ai += in_edges_of(d);
         }
     }
-    al = wave_sum(al);
     af = wave_sum(af);
     ai = wave_sum(ai);
     if ((threadIdx.x & (kWave - 1)) == 0) {
-        s_l[threadIdx.x >> 6] = al;
         s_f[threadIdx.x >> 6] = af;
         s_i[threadIdx.x >> 6] = ai;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        part_list[blockIdx.x] = s_l[0] + s_l[1] + s_l[2] + s_l[3];
         part_full[blockIdx.x] = s_f[0] + s_f[1] + s_f[2] + s_f[3];
         part_in[blockIdx.x] = s_i[0] + s_i[1] + s_i[2] + s_i[3];
     }
 }
 
-// ws (list part), ws_full and ie_pre from the scanned block partials.
+// ws_full and ie_pre from the scanned block partials.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void k_split_scan(const GraphDesc *__restrict__ desc,
                                                        const SplitDesc *__restrict__ sd,
-                                                       const uint64_t *__restrict__ part_list,
                                                        const uint64_t *__restrict__ part_full,
                                                        const uint64_t *__restrict__ part_in) {
     __shared__ uint64_t s_tot[kWavesPerBlock];
     const GraphDesc &D = desc[0];
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
-    const uint32_t *__restrict__ kpos = sd->kpos;
-    uint64_t *__restrict__ ws = D.ws;
     uint64_t *__restrict__ wf = sd->ws_full;
     uint64_t *__restrict__ wi = sd->ie_pre;
     const uint32_t n = D.n;
-    uint64_t cl = part_list[blockIdx.x], cf = part_full[blockIdx.x], ci = part_in[blockIdx.x];
+    uint64_t cf = part_full[blockIdx.x], ci = part_in[blockIdx.x];
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
     for (int k = 0; k < kScanItems; k++) {
         if (base + (uint64_t)k * kBlock >= n) break;  // block-uniform
         const uint64_t v = base + (uint64_t)k * kBlock + threadIdx.x;
-        uint64_t xl = 0, xf = 0, xi = 0;
+        uint64_t xf = 0, xi = 0;
         if (v < n) {
             const uint64_t d = (uint64_t)(rowPtr[v + 1] - rowPtr[v]);
-            const uint32_t kb = kpos[v];
-            xl = list_wedges(d, kb);
             xf = binom2_d(d);
-            xi = top_in_edges(d, kb);
+            xi = in_edges_of(d);
         }
-        uint64_t tl, tf, ti;
-        const uint64_t el = block_excl_scan(xl, s_tot, tl);
+        uint64_t tf, ti;
         const uint64_t ef = block_excl_scan(xf, s_tot, tf);
         const uint64_t ei = block_excl_scan(xi, s_tot, ti);
         if (v < n) {
-            ws[v] = cl + el;
             wf[v] = cf + ef;
             wi[v] = ci + ei;
         }
-        cl += tl;
         cf += tf;
         ci += ti;
     }
     if (blockIdx.x == (n - 1) / kScanChunk && threadIdx.x == 0) {
-        ws[n] = cl;
         wf[n] = cf;
         wi[n] = ci;
     }
 }
 
-// One thread: the counted range of the full order, its vertices and its
-// image in the list part's order (which k_setup then tiles).
-__global__ void k_split_range(GraphDesc *__restrict__ desc, SplitDesc *__restrict__ sd) {
+// One thread: the counted range of the full order, its end vertices, and
+// the one in-edge whose tail the range cuts at the front (row i0 of v_lo
+// holding wedge w_lo: its tail starts w_lo - base(i0) ids in).
+template <typename OffT>
+__global__ void k_split_range(const GraphDesc *__restrict__ desc, SplitDesc *__restrict__ sd) {
     if (threadIdx.x != 0) return;
-    GraphDesc &D = desc[0];
+    const GraphDesc &D = desc[0];
     SplitDesc &S = sd[0];
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
     const uint32_t n = D.n;
-    const uint64_t *ws = D.ws, *wf = S.ws_full;
+    const uint64_t *wf = S.ws_full;
     const uint64_t W = wf[n];
     const uint64_t lo = D.w_begin < W ? D.w_begin : W, hi = D.w_end < W ? D.w_end : W;
     S.w_lo = lo;
     S.w_hi = hi;
+    S.clip_j = ~0ull;
+    S.clip_lo = 0;
     if (lo == 0 && hi == W) {  // the whole order: nothing clipped
         S.clip = 0;
         S.v_lo = 0;
         S.v_hi = n - 1;
-        D.w_begin = 0;
-        D.w_end = ws[n];
         return;
     }
     S.clip = 1;
     if (lo >= hi) {  // nothing to count
         S.v_lo = 1;
         S.v_hi = 0;
-        D.w_begin = 0;
-        D.w_end = 0;
         return;
     }
-    S.v_lo = rightmost_le(wf, 0u, n - 1, lo);
+    const uint32_t v = rightmost_le(wf, 0u, n - 1, lo);
+    S.v_lo = v;
     S.v_hi = rightmost_le(wf, 0u, n - 1, hi - 1);
-    D.w_begin = list_pos(ws, wf, n, lo);
-    D.w_end = list_pos(ws, wf, n, hi);
+    const uint64_t rb = (uint64_t)rowPtr[v], d = (uint64_t)rowPtr[v + 1] - rb, off = lo - wf[v];
+    uint64_t s = 0, e = d - 1;  // the last i in [0, d - 2] with row_base(i) <= off
+    while (e - s > 1) {
+        const uint64_t mid = s + ((e - s) >> 1);
+        if (row_base(mid, d) <= off) s = mid;
+        else e = mid;
+    }
+    S.clip_j = rb + s;
+    S.clip_lo = rb + s + 1 + (off - row_base(s, d));
 }
 
 // ------------------------------------------------------------ transposition
 
-// First vertex of chunk c of `chunks`: the vertex holding top in-edge
-// floor(c*K/chunks) (K = all top in-edges), so that chunks hold about equal
+// First vertex of chunk c of `chunks`: the vertex holding in-edge
+// floor(c*K/chunks) (K = all in-edges), so that chunks hold about equal
 // shares of the in-edges; chunk `chunks` starts at n.  Wave-uniform.
 __device__ __forceinline__ uint32_t chunk_start(cptr<uint64_t> ie_pre, uint32_t n, uint32_t c, uint32_t chunks) {
     if (c >= chunks) return n;
@@ -227,7 +194,7 @@ template <typename OffT>
 __device__ __forceinline__ void clip_tail(OffT j, OffT e, OffT rb, uint64_t ws, uint64_t w_lo, uint64_t w_hi,
                                           OffT &lo, OffT &hi) {
     const uint64_t d = (uint64_t)(e - rb), i = (uint64_t)(j - rb);
-    const uint64_t bse = ws + i * (d - 1) - binom2_d(i), L = (uint64_t)(e - j - 1);
+    const uint64_t bse = ws + row_base(i, d), L = (uint64_t)(e - j - 1);
     const uint64_t a = bse > w_lo ? bse : w_lo;
     const uint64_t z = bse + L < w_hi ? bse + L : w_hi;
     lo = j + 1;
@@ -238,23 +205,25 @@ __device__ __forceinline__ void clip_tail(OffT j, OffT e, OffT rb, uint64_t ws, 
     }
 }
 
-// Vertices with at least this many top in-edges are taken by a whole wave.
+// Vertices with at least this many in-edges are taken by a whole wave.
 constexpr uint32_t kSplitBigVertex = kWave;
 
 // First level of the transposition.  Block b takes vertex chunk b (clipped
 // to the counted vertices); a wave takes 64 consecutive vertices:
-//  * a vertex with kSplitBigVertex or more top in-edges is taken by the
-//    whole wave in turn: the lanes load its in-edges' ids 64 consecutive
-//    positions at a time, four loads in flight;
+//  * a vertex with kSplitBigVertex or more in-edges is taken by the whole
+//    wave in turn: the lanes load its ids 64 consecutive positions at a
+//    time, four loads in flight;
 //  * the others' in-edges are scanned and dealt to the lanes 64 at a time
 //    (a lane finds its vertex by a binary lifting over the scanned counts).
 // Every in-edge (its tail clipped at the ends of the counted range; empty
-// ones dropped) belongs to the part (id - thr) / kSplitPart.  kScatter =
-// false: the block counts its in-edges per part into hist1[b][part];
-// kScatter = true (hist1 then holding the record offset of each chunk and
-// part): each in-edge takes the next record slot of its part through an LDS
-// cursor, and its id and tail are stored there.  Both passes see the same
-// in-edges in the same chunks.
+// ones dropped) gets its key -- w - thr for a top row w, else rows +
+// rowPtr[w] / kSlabSpan -- and belongs to the part key / kSplitPart.
+// kScatter = false: the block counts its in-edges per part into
+// hist1[b][part]; kScatter = true (hist1 then holding the record offset of
+// each chunk and part): the in-edges of a wave step take the next record
+// slots of their parts (one LDS atomic per part present), and their keys and
+// records are stored there.  Both passes see the same in-edges in the same
+// chunks.
 template <typename OffT, bool kScatter>
 __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__restrict__ desc,
                                                           const SplitDesc *__restrict__ sd) {
@@ -264,12 +233,12 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     const SplitDesc &S = sd[0];
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
     const gptr<uint32_t> colInd = to_global(D.colInd);
-    const uint32_t *__restrict__ kpos = S.kpos;
     const uint64_t *__restrict__ wf = S.ws_full;
     uint64_t *__restrict__ hist1 = S.hist1;
     uint32_t *__restrict__ rec_id = S.rec_id;
     TailPair<OffT> *__restrict__ rec = static_cast<TailPair<OffT> *>(S.rec);
-    const uint32_t n = D.n, thr = D.thr, parts = S.parts;
+    uint32_t *__restrict__ rec_aux = S.rec_aux;
+    const uint32_t n = D.n, thr = D.thr, rows = S.rows, parts = S.parts;
     const uint32_t b = blockIdx.x;
     for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) {
         s_cnt[i] = 0u;
@@ -277,19 +246,47 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     }
     __syncthreads();
 
-    // one in-edge: count it in its part, or store its record
-    auto take = [&](uint32_t id, OffT lo, OffT hi) {
-        const uint32_t q = (id - thr) / kSplitPart;
-        if (!kScatter) {
-            atomicAdd(&s_cnt[q], 1u);
-        } else {
-            const uint64_t p = s_base[q] + atomicAdd(&s_cnt[q], 1u);
-            rec_id[p] = id;
-            rec[p] = TailPair<OffT>{lo, hi};
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+    const uint64_t below = lane ? ~0ull >> (kWave - lane) : 0ull;  // lanes under this one
+
+    // The wave's in-edges of one step (act lanes): counted in their parts,
+    // or stored at their parts' next record slots.  One LDS atomic per part
+    // present in the wave, so a part's records of one step are consecutive.
+    // Called by the whole wave.
+    auto take = [&](bool act, uint32_t key, OffT first, OffT hi, uint32_t aux) {
+        const uint32_t q = act ? key / kSplitPart : kSplitPartsMax;
+        uint64_t pend = __ballot(act);
+        while (pend) {
+            const int src = __builtin_ctzll(pend);
+            const uint32_t q0 = (uint32_t)__builtin_amdgcn_readlane((int)q, src);
+            const uint64_t same = __ballot(q == q0);
+            pend &= ~same;
+            const uint32_t cnt = (uint32_t)__popcll(same);
+            if (!kScatter) {
+                if ((int)lane == src) atomicAdd(&s_cnt[q0], cnt);
+            } else {
+                uint32_t at = 0;
+                if ((int)lane == src) at = atomicAdd(&s_cnt[q0], cnt);
+                at = (uint32_t)__builtin_amdgcn_readlane((int)at, src);
+                if (q == q0) {
+                    const uint64_t p = s_base[q0] + at + (uint32_t)__popcll(same & below);
+                    rec_id[p] = key;
+                    rec[p] = TailPair<OffT>{first, hi};
+                    if (key >= rows) rec_aux[p] = aux;
+                }
+            }
         }
     };
+    // key of middle vertex id, given rowPtr[id] for a list row
+    auto key_of = [&](uint32_t id, OffT rp) -> uint32_t {
+        return id >= thr ? id - thr : rows + (uint32_t)((uint64_t)rp / kSlabSpan);
+    };
+    // slab_aux of a list row: adj+(id) = colInd[rp, rp1)
+    auto aux_of = [&](OffT rp, OffT rp1) -> uint32_t {
+        const uint64_t len = (uint64_t)(rp1 - rp);
+        return (uint32_t)((uint64_t)rp % kSlabSpan) | ((len <= kSlabRowMax ? (uint32_t)len : kSlabAuxLong) << 13);
+    };
 
-    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
     const bool clip = S.clip != 0;
     const uint32_t v_lo = S.v_lo, v_hi = S.v_hi;
     const uint64_t w_lo = S.w_lo, w_hi = S.w_hi;
@@ -302,15 +299,14 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     constexpr uint32_t kWaves = kSplitBlock / kWave;
     for (uint64_t g = c0 + (uint64_t)wv * kWave; g < c1; g += (uint64_t)kWaves * kWave) {
         const uint64_t v = g + lane;
-        OffT j0 = 0, e = 0, rb = 0;
+        OffT e = 0, rb = 0;
         uint32_t cnt = 0;
         bool edge = false;
         uint64_t wsv = 0;
         if (v < c1) {
             rb = rowPtr[v];
             e = rowPtr[v + 1];
-            j0 = rb + kpos[v];  // the row's first id >= thr
-            cnt = (uint32_t)top_in_edges((uint64_t)(e - rb), kpos[v]);
+            cnt = (uint32_t)in_edges_of((uint64_t)(e - rb));
             edge = clip && (v == v_lo || v == v_hi);
             if (edge) wsv = wf[v];
         }
@@ -319,14 +315,14 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
         while (big) {
             const uint32_t src = (uint32_t)__builtin_ctzll(big);
             big &= big - 1;
-            const OffT vj0 = (OffT)readlane_u64((uint64_t)j0, src);
-            const OffT vend = vj0 + (OffT)(uint32_t)__builtin_amdgcn_readlane((int)cnt, (int)src);
+            const OffT vrb = (OffT)readlane_u64((uint64_t)rb, src);
+            const OffT vend = vrb + (OffT)(uint32_t)__builtin_amdgcn_readlane((int)cnt, (int)src);
             const OffT ve = (OffT)readlane_u64((uint64_t)e, src);
             const bool vedge = __builtin_amdgcn_readlane((int)edge, (int)src) != 0;
-            const OffT vrb = (OffT)readlane_u64((uint64_t)rb, src);
             const uint64_t vws = readlane_u64(wsv, src);
-            for (OffT jb = vj0; jb < vend; jb += 4 * kWave) {
+            for (OffT jb = vrb; jb < vend; jb += 4 * kWave) {
                 uint32_t ids[4];
+                OffT rp[4], rp1[4];
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const OffT j = jb + (OffT)(u * kWave) + lane;
@@ -335,11 +331,19 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
                     const OffT j = jb + (OffT)(u * kWave) + lane;
-                    if (j < vend) {
-                        OffT lo = j + 1, hi = ve;
-                        if (vedge) clip_tail<OffT>(j, ve, vrb, vws, w_lo, w_hi, lo, hi);
-                        if (lo < hi) take(ids[u], lo, hi);
+                    rp[u] = rp1[u] = 0;
+                    if (j < vend && ids[u] < thr) {
+                        rp[u] = rowPtr[ids[u]];
+                        rp1[u] = rowPtr[ids[u] + 1];
                     }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const OffT j = jb + (OffT)(u * kWave) + lane;
+                    OffT lo = j + 1, hi = ve;
+                    if (vedge && j < vend) clip_tail<OffT>(j, ve, vrb, vws, w_lo, w_hi, lo, hi);
+                    const bool top = ids[u] >= thr;
+                    take(j < vend && lo < hi, key_of(ids[u], rp[u]), top ? lo : j, hi, aux_of(rp[u], rp1[u]));
                 }
             }
         }
@@ -364,16 +368,23 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
                 const uint32_t x = (uint32_t)__shfl((int)excl, (int)(c & (kWave - 1)), kWave);
                 if (c < (uint32_t)kWave && x <= idx) pos = c;
             }
-            const OffT j = shfl_off(j0, pos) + (OffT)(idx - (uint32_t)__shfl((int)excl, (int)pos, kWave));
+            const OffT rbv = shfl_off(rb, pos);
+            const OffT j = rbv + (OffT)(idx - (uint32_t)__shfl((int)excl, (int)pos, kWave));
             const OffT ev = shfl_off(e, pos);
             OffT lo = j + 1, hi = ev;
             if (any_edge) {
                 const bool ed = __shfl((int)edge, (int)pos, kWave) != 0;
-                const OffT rbv = shfl_off(rb, pos);
                 const uint64_t wv0 = shfl_u64(wsv, pos);
                 if (ed) clip_tail<OffT>(j, ev, rbv, wv0, w_lo, w_hi, lo, hi);
             }
-            if (idx < tot && lo < hi) take(colInd[j], lo, hi);
+            const bool act = idx < tot && lo < hi;
+            const uint32_t id = act ? colInd[j] : 0u;
+            OffT rp = 0, rp1 = 0;
+            if (act && id < thr) {
+                rp = rowPtr[id];
+                rp1 = rowPtr[id + 1];
+            }
+            take(act, key_of(id, rp), id >= thr ? lo : j, hi, aux_of(rp, rp1));
         }
     }
     if (!kScatter) {
@@ -423,21 +434,22 @@ __global__ __launch_bounds__(kSplitPartsMax) void k_split_l1scan(const SplitDesc
 
 // Second level.  Block c2 takes records [part_base[q] + k*kSplitChunk2, ...)
 // of part q (c2 = c2base[q] + k; blocks past c2base[parts] do nothing) and
-// groups them by row of the part: kScatter = false counts them per row into
-// hist2[c2][row - q*kSplitPart]; kScatter = true (hist2 then holding, per
-// row, the part's records of earlier blocks) places each record's tail at
-// the next slot of its row in ie.
+// groups them by key of the part: kScatter = false counts them per key into
+// hist2[c2][key - q*kSplitPart]; kScatter = true (hist2 then holding, per
+// key, the part's records of earlier blocks) places each record at the next
+// slot of its key in ie.
+constexpr int kL2Records = 4;
+
 template <typename OffT, bool kScatter>
-__global__ __launch_bounds__(kSplitBlock) void k_split_l2(const GraphDesc *__restrict__ desc,
-                                                          const SplitDesc *__restrict__ sd) {
+__global__ __launch_bounds__(kSplitBlock) void k_split_l2(const SplitDesc *__restrict__ sd) {
     __shared__ uint32_t s_cnt[kSplitPart];
     const SplitDesc &S = sd[0];
-    const uint32_t parts = S.parts, rows = S.rows, thr = desc[0].thr;
+    const uint32_t parts = S.parts, keys = S.keys, rows = S.rows;
     const uint32_t c2 = blockIdx.x;
     if (c2 >= S.c2base[parts]) return;  // block-uniform
     uint32_t q = 0;
     while (q + 1 < parts && S.c2base[q + 1] <= c2) q++;
-    const uint32_t r0 = q * kSplitPart, r1 = r0 + kSplitPart < rows ? r0 + kSplitPart : rows;
+    const uint32_t r0 = q * kSplitPart, r1 = r0 + kSplitPart < keys ? r0 + kSplitPart : keys;
     const uint64_t k0 = S.part_base[q] + (uint64_t)(c2 - S.c2base[q]) * kSplitChunk2;
     const uint64_t kq = S.part_base[q + 1], k1 = k0 + kSplitChunk2 < kq ? k0 + kSplitChunk2 : kq;
     uint32_t *__restrict__ hist2 = S.hist2 + (uint64_t)c2 * kSplitPart;
@@ -447,13 +459,46 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l2(const GraphDesc *__res
     const TailPair<OffT> *__restrict__ rec = static_cast<const TailPair<OffT> *>(S.rec);
     const OffT *__restrict__ rp_in = static_cast<const OffT *>(S.rp_in);
     TailPair<OffT> *__restrict__ ie = static_cast<TailPair<OffT> *>(S.ie);
-    for (uint64_t k = k0 + threadIdx.x; k < k1; k += kSplitBlock) {
-        const uint32_t r = rec_id[k] - thr;
+    const uint32_t *__restrict__ rec_aux = S.rec_aux;
+    uint32_t *__restrict__ ie_aux = S.ie_aux;
+    // kL2Records records per thread per step, their loads issued together
+    // (the scatter's key base and slot depend on the record's key)
+    constexpr int kU = kL2Records;
+    for (uint64_t kb = k0 + threadIdx.x; kb < k1; kb += (uint64_t)kU * kSplitBlock) {
+        uint32_t r[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const uint64_t k = kb + (uint64_t)u * kSplitBlock;
+            r[u] = k < k1 ? rec_id[k] : kSentinel;
+        }
         if (!kScatter) {
-            atomicAdd(&s_cnt[r - r0], 1u);
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (r[u] != kSentinel) atomicAdd(&s_cnt[r[u] - r0], 1u);
         } else {
-            const OffT p = rp_in[r] + (OffT)atomicAdd(&s_cnt[r - r0], 1u);
-            ie[p] = rec[k];
+            TailPair<OffT> t[kU];
+            OffT base[kU];
+            uint32_t aux[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                const uint64_t k = kb + (uint64_t)u * kSplitBlock;
+                if (r[u] != kSentinel) {
+                    t[u] = rec[k];
+                    base[u] = rp_in[r[u]];
+                    if (r[u] >= rows) aux[u] = rec_aux[k];
+                }
+            }
+            uint32_t slot[kU];
+#pragma unroll
+            for (int u = 0; u < kU; u++)
+                if (r[u] != kSentinel) slot[u] = atomicAdd(&s_cnt[r[u] - r0], 1u);
+#pragma unroll
+            for (int u = 0; u < kU; u++) {
+                if (r[u] == kSentinel) continue;
+                const OffT p = base[u] + (OffT)slot[u];
+                ie[p] = t[u];
+                if (r[u] >= rows) ie_aux[p] = aux[u];
+            }
         }
     }
     if (!kScatter) {
@@ -462,13 +507,13 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l2(const GraphDesc *__res
     }
 }
 
-// Thread per top row r (part q): the column prefix of hist2 over the part's
-// second-level blocks (hist2[c2][r - q*kSplitPart] becomes the row's records
-// in earlier blocks) and the row's total.
+// Thread per key r (part q): the column prefix of hist2 over the part's
+// second-level blocks (hist2[c2][r - q*kSplitPart] becomes the key's records
+// in earlier blocks) and the key's total.
 __global__ __launch_bounds__(kBlock) void k_split_l2cols(const SplitDesc *__restrict__ sd) {
     const SplitDesc &S = sd[0];
     const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r >= S.rows) return;
+    if (r >= S.keys) return;
     const uint32_t q = r / kSplitPart, i = r - q * kSplitPart;
     uint32_t *__restrict__ hist2 = S.hist2;
     uint32_t s = 0;
@@ -477,27 +522,28 @@ __global__ __launch_bounds__(kBlock) void k_split_l2cols(const SplitDesc *__rest
         hist2[(uint64_t)c2 * kSplitPart + i] = s;
         s += h;
     }
-    S.row_tot[r] = s;
+    S.key_tot[r] = s;
 }
 
-// One block: rp_in = exclusive prefix of the row totals; rp_in[rows] = the
-// in-edges; the row kernel's tile counter zeroed; the counted wedges of the
-// full order, those of the top rows and the in-edges into meta.
+// One block: rp_in = exclusive prefix of the key totals; rp_in[keys] = the
+// in-edges; the tile counters and wedge tallies zeroed; the count zeroed;
+// the counted wedges of the full order, the top rows' in-edges and all
+// in-edges into meta.
 template <typename OffT>
-__global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const GraphDesc *__restrict__ desc,
-                                                               const SplitDesc *__restrict__ sd,
-                                                               uint64_t *__restrict__ meta) {
+__global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const SplitDesc *__restrict__ sd,
+                                                               uint64_t *__restrict__ meta,
+                                                               unsigned long long *__restrict__ counts) {
     constexpr int kWaves = kSplitBlock / kWave;
     __shared__ uint64_t s_w[kWaves];
     const SplitDesc &S = sd[0];
-    const uint32_t rows = S.rows;
-    const uint32_t *__restrict__ tot = S.row_tot;
+    const uint32_t keys = S.keys, rows = S.rows;
+    const uint32_t *__restrict__ tot = S.key_tot;
     OffT *__restrict__ rp = static_cast<OffT *>(S.rp_in);
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
     uint64_t carry = 0;
-    for (uint32_t base = 0; base < rows; base += kSplitBlock) {
+    for (uint32_t base = 0; base < keys; base += kSplitBlock) {
         const uint32_t i = base + threadIdx.x;
-        const uint64_t x = i < rows ? tot[i] : 0;
+        const uint64_t x = i < keys ? tot[i] : 0;
         const uint64_t inc = wave_incl_scan(x);
         if (lane == kWave - 1) s_w[wv] = inc;
         __syncthreads();
@@ -509,15 +555,21 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const GraphDesc *
             all += t;
         }
         __syncthreads();
-        if (i < rows) rp[i] = (OffT)(carry + off + inc - x);
+        const uint64_t at = carry + off + inc - x;
+        if (i < keys) rp[i] = (OffT)at;
+        if (i == rows) meta[kMetaInEdges] = at;
         carry += all;
     }
     if (threadIdx.x == 0) {
-        rp[rows] = (OffT)carry;
+        rp[keys] = (OffT)carry;
+        if (rows >= keys) meta[kMetaInEdges] = carry;
         meta[kMetaRows] = 0;
-        meta[kMetaWedges] = S.w_hi - S.w_lo;  // k_setup counted the list part's only
-        meta[kMetaTopWedges] = S.w_hi - S.w_lo - desc[0].wcount;
-        meta[kMetaInEdges] = carry;
+        meta[kMetaSlabs] = 0;
+        meta[kMetaTopWedges] = 0;
+        meta[kMetaListWedges] = 0;
+        meta[kMetaWedges] = S.w_hi - S.w_lo;
+        meta[kMetaAllEdges] = carry;
+        counts[0] = 0;
     }
 }
 
@@ -559,7 +611,7 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = uni32(threadIdx.x >> 6);
     const uint32_t sub = lane / kLanes, sl = lane % kLanes;
     unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaRows);
-    uint64_t hits = 0;
+    uint64_t hits = 0, checked = 0;
     for (;;) {
         __syncthreads();  // the previous tile's s_tile reads and bit clears are done
         if (threadIdx.x == 0) s_tile = atomicAdd(ctr, 1ull);
@@ -590,6 +642,7 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
                     lo = ieG[2 * (x0 + sub)];
                     hi = ieG[2 * (x0 + sub) + 1];
                 }
+                if (sl == 0) checked += (uint64_t)(hi - lo);
                 for (OffT q = (lo & ~(OffT)3) + 4 * sl; q < hi; q += 8 * kLanes) {
                     const OffT q1 = q + 4 * kLanes;
                     const u32x4 a = *reinterpret_cast<gptr<u32x4>>(colInd + q);
@@ -614,7 +667,151 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
         }
     }
     hits = wave_sum(hits);
+    checked = wave_sum(checked);
     if (lane == 0 && hits) atomicAdd(&counts[0], (unsigned long long)hits);
+    if (lane == 0 && checked) atomicAdd(reinterpret_cast<unsigned long long *>(meta + kMetaTopWedges), checked);
+}
+
+// ---------------------------------------------------------------- list part
+
+// Lower bound of u in the ascending s[a, e): the first position whose id is
+// >= u (e if none), galloping from a (the bound of a smaller id) and then
+// halving.
+__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *s, uint32_t a, uint32_t e, uint32_t u) {
+    if (a >= e || s[a] >= u) return a;
+    uint32_t step = 1, b = a + 1;  // s[a] < u
+    while (b < e && s[b] < u) {
+        a = b;
+        step <<= 1;
+        b = a + step;
+    }
+    // the bound lies in (a, min(b, e)]
+    a += 1;
+    if (b > e) b = e;
+    while (a < b) {
+        const uint32_t mid = (a + b) >> 1;
+        if (s[mid] < u) a = mid + 1;
+        else b = mid;
+    }
+    return a;
+}
+
+// Persistent blocks take tiles of kSlabTile consecutive list in-edges (in
+// slab order) from one counter.  Per slab of a tile: colInd[s*kSlabSpan,
+// +kSlabWin) copied into LDS (kept while the next tile stays in the slab),
+// the tile's in-edges of the slab dealt to the waves (kLanes lanes per
+// in-edge), and each tail streamed by 16-byte loads.  A record's slab_aux
+// word places adj+(w) in the window, so the tail loads follow the record
+// load directly; each id u is looked up there by a galloping lower bound
+// from the lane's last bound (a lane's ids only grow along a tail), or in
+// HBM for a row longer than kSlabRowMax.  Same alignment and slack needs as
+// the row kernel.
+template <typename OffT, int kWaves, int kLanes>
+__global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc *__restrict__ desc,
+                                                                const SplitDesc *__restrict__ sd,
+                                                                uint64_t *__restrict__ meta,
+                                                                unsigned long long *__restrict__ counts) {
+    __shared__ uint32_t s_win[kSlabWin];
+    __shared__ uint64_t s_tile;
+    const GraphDesc &D = desc[0];
+    const SplitDesc &S = sd[0];
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
+    const gptr<uint32_t> colInd = to_global(D.colInd);
+    const cptr<OffT> rp_in = to_const(static_cast<const OffT *>(S.rp_in));
+    const gptr<OffT> ieG = to_global(static_cast<const OffT *>(S.ie));
+    const gptr<uint32_t> auxG = to_global(static_cast<const uint32_t *>(S.ie_aux));
+    const uint32_t rows = S.rows, keys = S.keys;
+    const uint64_t L0 = (uint64_t)rp_in[rows], L1 = (uint64_t)rp_in[keys];
+    const uint64_t mEnd = (uint64_t)to_const(rowPtr)[D.n];
+    const OffT clip_j = (OffT)S.clip_j, clip_lo = (OffT)S.clip_lo;
+    const bool clipped = S.clip_j != ~0ull;
+    constexpr uint32_t kPer = kWave / kLanes;  // in-edges a wave takes per step
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = uni32(threadIdx.x >> 6);
+    const uint32_t sub = lane / kLanes, sl = lane % kLanes;
+    unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaSlabs);
+    uint32_t cur = kSentinel;  // slab in s_win (block-uniform)
+    uint64_t hits = 0, checked = 0;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_tile = atomicAdd(ctr, 1ull);
+        __syncthreads();
+        const uint64_t k0 = L0 + uni64(s_tile) * kSlabTile;
+        if (k0 >= L1) break;  // block-uniform
+        const uint64_t k1 = k0 + kSlabTile < L1 ? k0 + kSlabTile : L1;
+        uint64_t k = k0;
+        while (k < k1) {
+            const uint32_t key = uni32(rightmost_le(rp_in, rows, keys - 1, k));  // the key holding in-edge k
+            const uint64_t re = (uint64_t)rp_in[key + 1];
+            const uint64_t kend = re < k1 ? re : k1;
+            const uint32_t slab = key - rows;
+            const uint64_t wb = (uint64_t)slab * kSlabSpan;  // window start in colInd
+            if (slab != cur) {
+                __syncthreads();  // the previous window's searches are done
+                for (uint32_t i = threadIdx.x; i < kSlabWin; i += kWave * kWaves) {
+                    const uint64_t p = wb + i;
+                    s_win[i] = p < mEnd ? colInd[p] : kSentinel;
+                }
+                cur = slab;
+                __syncthreads();
+            }
+            uint32_t h = 0;
+            for (uint64_t x0 = k + (uint64_t)wv * kPer; x0 < kend; x0 += (uint64_t)kWaves * kPer) {
+                const uint64_t x = x0 + sub;
+                OffT j = 0, hi = 0;
+                uint32_t aux = 0;
+                if (x < kend) {
+                    j = ieG[2 * x];
+                    hi = ieG[2 * x + 1];
+                    aux = auxG[x];
+                }
+                OffT lo = j + 1;
+                if (clipped && j == clip_j) lo = clip_lo;
+                if (x >= kend) hi = lo;  // nothing to stream
+                if (sl == 0) checked += (uint64_t)(hi - lo);
+                uint32_t lb = aux & kSlabAuxLong, len = aux >> 13;
+                const bool inwin = len != kSlabAuxLong;
+                OffT gb = 0, ge = 0;  // adj+(w) in HBM for a long row
+                if (!inwin && x < kend) {
+                    const uint32_t w = colInd[j];
+                    gb = rowPtr[w];
+                    ge = rowPtr[w + 1];
+                }
+                const uint32_t le = inwin ? lb + len : 0u;
+                for (OffT q = (lo & ~(OffT)3) + 4 * sl; q < hi; q += 4 * kLanes) {
+                    const u32x4 a = *reinterpret_cast<gptr<u32x4>>(colInd + q);
+                    const uint32_t ua[4] = {a[0], a[1], a[2], a[3]};
+#pragma unroll
+                    for (int z = 0; z < 4; z++) {
+                        const OffT t = q + (OffT)z;
+                        if (t < lo || t >= hi) continue;
+                        const uint32_t u = ua[z];
+                        if (inwin) {
+                            lb = lds_lower_bound(s_win, lb, le, u);
+                            h += lb < le && s_win[lb] == u ? 1u : 0u;
+                        } else {
+                            OffT n_ = ge - gb;
+                            while (n_ > 0) {
+                                const OffT half = n_ >> 1;
+                                if (colInd[gb + half] < u) {
+                                    gb += half + 1;
+                                    n_ -= half + 1;
+                                } else {
+                                    n_ = half;
+                                }
+                            }
+                            h += gb < ge && colInd[gb] == u ? 1u : 0u;
+                        }
+                    }
+                }
+            }
+            hits += h;
+            k = kend;
+        }
+    }
+    hits = wave_sum(hits);
+    checked = wave_sum(checked);
+    if (lane == 0 && hits) atomicAdd(&counts[0], (unsigned long long)hits);
+    if (lane == 0 && checked) atomicAdd(reinterpret_cast<unsigned long long *>(meta + kMetaListWedges), checked);
 }
 
 // ---------------------------------------------------------------- launches
@@ -622,24 +819,23 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
 hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t nblocks,
                                uint64_t *partials, hipStream_t st) {
     if (nblocks == 0) return hipSuccess;
-    uint64_t *pl = partials, *pf = partials + nblocks, *pi = partials + 2 * (size_t)nblocks;
-    hipError_t e = WTC_LAUNCH_OFF(ob, k_split_partials, dim3(nblocks), dim3(kBlock), 0, st, desc, sd, pl, pf, pi);
+    uint64_t *pf = partials, *pi = partials + nblocks;
+    hipError_t e = WTC_LAUNCH_OFF(ob, k_split_partials, dim3(nblocks), dim3(kBlock), 0, st, desc, pf, pi);
     if (e != hipSuccess) return e;
-    if ((e = launch_scan_partials(pl, nblocks, st)) != hipSuccess) return e;
     if ((e = launch_scan_partials(pf, nblocks, st)) != hipSuccess) return e;
     if ((e = launch_scan_partials(pi, nblocks, st)) != hipSuccess) return e;
-    return WTC_LAUNCH_OFF(ob, k_split_scan, dim3(nblocks), dim3(kBlock), 0, st, desc, sd, (const uint64_t *)pl,
-                          (const uint64_t *)pf, (const uint64_t *)pi);
+    return WTC_LAUNCH_OFF(ob, k_split_scan, dim3(nblocks), dim3(kBlock), 0, st, desc, sd, (const uint64_t *)pf,
+                          (const uint64_t *)pi);
 }
 
-hipError_t launch_split_range(GraphDesc *desc, SplitDesc *sd, hipStream_t st) {
-    k_split_range<<<dim3(1), dim3(kWave), 0, st>>>(desc, sd);
-    return hipGetLastError();
+hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipStream_t st) {
+    return WTC_LAUNCH_OFF(ob, k_split_range, dim3(1), dim3(kWave), 0, st, desc, sd);
 }
 
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
-                                  uint32_t c2cap, uint32_t rows, uint64_t *meta, hipStream_t st) {
-    if (rows == 0 || chunks == 0 || c2cap == 0) return hipErrorInvalidValue;
+                                  uint32_t c2cap, uint32_t keys, uint64_t *meta, unsigned long long *counts,
+                                  hipStream_t st) {
+    if (keys == 0 || chunks == 0 || c2cap == 0) return hipErrorInvalidValue;
     const dim3 blk(kSplitBlock), g1(chunks), g2(c2cap);
     hipError_t e;
     if (ob == 8) k_split_l1<uint64_t, false><<<g1, blk, 0, st>>>(desc, sd);
@@ -650,16 +846,16 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
     if (ob == 8) k_split_l1<uint64_t, true><<<g1, blk, 0, st>>>(desc, sd);
     else k_split_l1<uint32_t, true><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_l2<uint64_t, false><<<g2, blk, 0, st>>>(desc, sd);
-    else k_split_l2<uint32_t, false><<<g2, blk, 0, st>>>(desc, sd);
+    if (ob == 8) k_split_l2<uint64_t, false><<<g2, blk, 0, st>>>(sd);
+    else k_split_l2<uint32_t, false><<<g2, blk, 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_split_l2cols<<<dim3((rows + kBlock - 1) / kBlock), dim3(kBlock), 0, st>>>(sd);
+    k_split_l2cols<<<dim3((keys + kBlock - 1) / kBlock), dim3(kBlock), 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_rowscan<uint64_t><<<dim3(1), blk, 0, st>>>(desc, sd, meta);
-    else k_split_rowscan<uint32_t><<<dim3(1), blk, 0, st>>>(desc, sd, meta);
+    if (ob == 8) k_split_rowscan<uint64_t><<<dim3(1), blk, 0, st>>>(sd, meta, counts);
+    else k_split_rowscan<uint32_t><<<dim3(1), blk, 0, st>>>(sd, meta, counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_l2<uint64_t, true><<<g2, blk, 0, st>>>(desc, sd);
-    else k_split_l2<uint32_t, true><<<g2, blk, 0, st>>>(desc, sd);
+    if (ob == 8) k_split_l2<uint64_t, true><<<g2, blk, 0, st>>>(sd);
+    else k_split_l2<uint32_t, true><<<g2, blk, 0, st>>>(sd);
     return hipGetLastError();
 }
 
@@ -702,6 +898,33 @@ hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob,
     const void *fn = split_rows_kernel(ob, wpb);
     void *args[] = {&desc, &sd, &meta, &counts};
     return hipLaunchKernel(fn, dim3(grid), dim3(wpb * kWave), args, split_rows_lds(rows), st);
+}
+
+// The slab kernel's block: 8 waves, 8 lanes per in-edge tail (list tails
+// are short: ~50 ids on R-MAT-24).
+constexpr int kSlabWaves = 8;
+constexpr int kSlabLanes = 8;
+
+static const void *split_slabs_kernel(int ob) {
+    return ob == 8 ? reinterpret_cast<const void *>(&k_split_slabs<uint64_t, kSlabWaves, kSlabLanes>)
+                   : reinterpret_cast<const void *>(&k_split_slabs<uint32_t, kSlabWaves, kSlabLanes>);
+}
+
+hipError_t split_slabs_grid(int ob, int num_cus, uint32_t *grid) {
+    int per_cu = 0;
+    hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, split_slabs_kernel(ob), kSlabWaves * kWave, 0);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) return hipErrorInvalidConfiguration;
+    *grid = (uint32_t)per_cu * (uint32_t)(num_cus > 0 ? num_cus : 1);
+    return hipSuccess;
+}
+
+hipError_t launch_split_slabs(const GraphDesc *desc, const SplitDesc *sd, int ob, uint64_t *meta,
+                              unsigned long long *counts, uint32_t grid, hipStream_t st) {
+    if (grid == 0) return hipErrorInvalidValue;
+    void *args[] = {&desc, &sd, &meta, &counts};
+    return hipLaunchKernel(split_slabs_kernel(ob), dim3(grid), dim3(kSlabWaves * kWave), args, 0, st);
 }
 
 }  // namespace wtc

@@ -64,7 +64,8 @@ class CountStats(C.Structure):
     _fields_ = [("prefix_ms", C.c_double), ("bitmap_ms", C.c_double), ("count_ms", C.c_double),
                 ("total_ms", C.c_double), ("wedges", C.c_uint64), ("blocks", C.c_uint32),
                 ("launches", C.c_uint32), ("small_graphs", C.c_uint32), ("split", C.c_uint32),
-                ("rows_ms", C.c_double), ("top_wedges", C.c_uint64), ("in_edges", C.c_uint64)]
+                ("rows_ms", C.c_double), ("top_wedges", C.c_uint64), ("in_edges", C.c_uint64),
+                ("list_ms", C.c_double), ("list_in_edges", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
