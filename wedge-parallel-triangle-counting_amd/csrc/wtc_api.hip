@@ -153,8 +153,9 @@ namespace {
 // Byte offsets of the pieces of a split count's workspace in one buffer
 // (each on a 256-byte boundary), and the buffer's size.
 struct SplitLayout {
-    size_t wsfull, hist1, part_base, c2base, hist2, tot, recid, rec, recaux, rpin, ie, ieaux, total;
-    uint32_t chunks, slabs, keys, parts, c2cap;
+    size_t wsfull, hist1, part_base, c2base, hist2, sub_base, c3base, hist3, sub_tot, key_tot;
+    size_t recid, rec, recaux, recid2, rec2, recaux2, rpin, total;
+    uint32_t chunks, slabs, keys, parts, subs, c2cap, c3cap;
     uint64_t cap;
 };
 
@@ -167,26 +168,34 @@ SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
     L.slabs = (uint32_t)split_slabs(m);
     L.keys = rows + L.slabs;
     L.parts = (L.keys + kSplitPart - 1) / kSplitPart;
+    L.subs = L.parts * kSplitFan2;
     L.cap = std::max<uint64_t>(m, 1);  // an in-edge is an edge
     L.c2cap = (uint32_t)((L.cap + kSplitChunk2 - 1) / kSplitChunk2) + L.parts;
+    L.c3cap = (uint32_t)((L.cap + kSplitChunk3 - 1) / kSplitChunk3) + L.subs;
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t at = off;
         off = (off + bytes + 255) & ~(size_t)255;
         return at;
     };
+    const size_t pair = 2 * (size_t)ob;
     L.wsfull = take(2 * ((size_t)n + 1) * sizeof(uint64_t));  // ws_full, then ie_pre
     L.hist1 = take((size_t)L.chunks * L.parts * sizeof(uint64_t));
     L.part_base = take(((size_t)L.parts + 1) * sizeof(uint64_t));
     L.c2base = take(((size_t)L.parts + 1) * sizeof(uint32_t));
-    L.hist2 = take((size_t)L.c2cap * kSplitPart * sizeof(uint32_t));
-    L.tot = take((size_t)L.keys * sizeof(uint32_t));
+    L.hist2 = take((size_t)L.c2cap * kSplitFan2 * sizeof(uint32_t));
+    L.sub_base = take(((size_t)L.subs + 1) * sizeof(uint64_t));
+    L.c3base = take(((size_t)L.subs + 1) * sizeof(uint32_t));
+    L.hist3 = take((size_t)L.c3cap * kSplitSub * sizeof(uint32_t));
+    L.sub_tot = take((size_t)L.subs * sizeof(uint32_t));
+    L.key_tot = take((size_t)L.keys * sizeof(uint32_t));
     L.recid = take((size_t)L.cap * sizeof(uint32_t));
-    L.rec = take((size_t)L.cap * 2 * (size_t)ob);
+    L.rec = take((size_t)L.cap * pair);
     L.recaux = take((size_t)L.cap * sizeof(uint32_t));
+    L.recid2 = take((size_t)L.cap * sizeof(uint32_t));
+    L.rec2 = take((size_t)L.cap * pair);
+    L.recaux2 = take((size_t)L.cap * sizeof(uint32_t));
     L.rpin = take(((size_t)L.keys + 1) * (size_t)ob);
-    L.ie = take((size_t)L.cap * 2 * (size_t)ob);
-    L.ieaux = take((size_t)L.cap * sizeof(uint32_t));
     L.total = off;
     return L;
 }
@@ -202,21 +211,30 @@ void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t ro
     sd->slabs = L.slabs;
     sd->chunks = L.chunks;
     sd->parts = L.parts;
+    sd->subs = L.subs;
     sd->cap = L.cap;
     sd->c2cap = L.c2cap;
+    sd->c3cap = L.c3cap;
     sd->ws_full = reinterpret_cast<uint64_t *>(base + L.wsfull);
     sd->ie_pre = sd->ws_full + (size_t)n + 1;
     sd->hist1 = reinterpret_cast<uint64_t *>(base + L.hist1);
     sd->part_base = reinterpret_cast<uint64_t *>(base + L.part_base);
     sd->c2base = reinterpret_cast<uint32_t *>(base + L.c2base);
     sd->hist2 = reinterpret_cast<uint32_t *>(base + L.hist2);
-    sd->key_tot = reinterpret_cast<uint32_t *>(base + L.tot);
+    sd->sub_base = reinterpret_cast<uint64_t *>(base + L.sub_base);
+    sd->c3base = reinterpret_cast<uint32_t *>(base + L.c3base);
+    sd->hist3 = reinterpret_cast<uint32_t *>(base + L.hist3);
+    sd->sub_tot = reinterpret_cast<uint32_t *>(base + L.sub_tot);
+    sd->key_tot = reinterpret_cast<uint32_t *>(base + L.key_tot);
     sd->rec_id = reinterpret_cast<uint32_t *>(base + L.recid);
     sd->rec = base + L.rec;
     sd->rec_aux = reinterpret_cast<uint32_t *>(base + L.recaux);
+    sd->rec_id2 = reinterpret_cast<uint32_t *>(base + L.recid2);
+    sd->rec2 = base + L.rec2;
+    sd->rec_aux2 = reinterpret_cast<uint32_t *>(base + L.recaux2);
     sd->rp_in = base + L.rpin;
-    sd->ie = base + L.ie;
-    sd->ie_aux = reinterpret_cast<uint32_t *>(base + L.ieaux);
+    sd->ie = sd->rec;  // the third level writes the pairs over A's
+    sd->ie_aux = sd->rec_aux;
 }
 
 // A bitmap length whose row does not fit one block's LDS (or a graph whose
@@ -362,6 +380,8 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->split_chunks = sd.chunks;
     plan->split_keys = sd.keys;
     plan->split_c2cap = sd.c2cap;
+    plan->split_c3cap = sd.c3cap;
+    plan->split_subs = sd.subs;
     plan->split_wpb = t.split_wpb;
     return WTC_OK;
 }
@@ -377,8 +397,8 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
         WTC_HIPC(launch_split_prefix(ddesc, sd, p.ob, p.blocks, s.partials.as<uint64_t>(), s.stream));
         WTC_HIPC(launch_split_range(ddesc, sd, p.ob, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
-        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_keys, meta, dcounts,
-                                        s.stream));
+        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_c3cap, p.split_subs,
+                                        p.split_keys, meta, dcounts, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
         WTC_HIPC(launch_split_rows(ddesc, sd, p.ob, p.split_rows, p.split_wpb, meta, dcounts, s.rows_grid,
                                    s.stream));

@@ -94,13 +94,22 @@ enum : int {
     kMetaSlots = 16
 };
 
-// Split count of one graph (wtc_split.hip).  Keys per part of the
-// transposition's second level (its LDS counters), the most parts, records
-// per second-level block, threads per transposition block, and in-edges per
+// Split count of one graph (wtc_split.hip).  The in-edges are grouped by key
+// in three counting-sort levels: by part of kSplitPart keys, by sub-part of
+// kSplitSub keys (kSplitFan2 per part), by key.  Each level's scatter keeps
+// at most kSplitFan2 or kSplitSub output runs open per block, so that the
+// lines of a run fill in the L2 cache before they are written back.
+// kSplitChunk2/3: records per block of the second and third levels;
+// kSplitBlock: threads per transposition block; kSplitTile: in-edges per
 // tile of the row kernel.
 constexpr uint32_t kSplitPart = 32768;
 constexpr uint32_t kSplitPartsMax = 256;
-constexpr uint32_t kSplitChunk2 = 262144;
+constexpr uint32_t kSplitSub = 256;
+constexpr uint32_t kSplitSubShift = 8;
+constexpr uint32_t kSplitFan2 = kSplitPart / kSplitSub;
+static_assert(kSplitSub == 1u << kSplitSubShift && kSplitFan2 * kSplitSub == kSplitPart, "key levels");
+constexpr uint32_t kSplitChunk2 = 65536;
+constexpr uint32_t kSplitChunk3 = 65536;
 constexpr int kSplitBlock = 1024;
 constexpr uint32_t kSplitTile = 1024;
 // Slabs of the list rows (middle vertex below thr): slab s holds the rows
@@ -116,32 +125,41 @@ constexpr uint32_t kSlabTile = 1024;
 // ranks [thr, n) of the graph's GraphDesc; OffT arrays use the set's offset
 // width.  Every in-edge (v, j) (j a position of adj+(v) other than its last)
 // is grouped by a key: w - thr for w = colInd[j] >= thr (a top row), else
-// rows + the slab of w.  Two levels: first into one bucket per part of
-// kSplitPart keys (records: key and tail), then each part's records by key
-// (the tails only, in ie).  A top record is the tail [lo, hi) of colInd; a
-// list record is (j, hi), its tail [j + 1, hi) (the one in-edge whose tail
-// the range cuts at the front: [clip_lo, hi)).
+// rows + the slab of w.  Records (key, pair, slab_aux) go through the
+// levels A -> B -> ie: A bucketed by part, B by sub-part, ie by key (the
+// pairs only, in A's pair array; a list record's slab_aux in A's aux
+// array).  A top record's pair is the tail [lo, hi) of colInd; a list
+// record's is (j, hi), its tail [j + 1, hi) (the one in-edge whose tail the
+// range cuts at the front: [clip_lo, hi)).
 struct SplitDesc {
     uint64_t *ws_full;    // n+1: the full wedge prefix
     uint64_t *ie_pre;     // n+1: prefix of the in-edges (positions of adj+(v) but the last) of each row
     uint64_t *hist1;      // chunks x parts: first-level counts, then record offsets
     uint64_t *part_base;  // parts+1: first record of each part; part_base[parts] = the in-edges
     uint32_t *c2base;     // parts+1: first second-level block of each part
-    uint32_t *hist2;      // c2cap x kSplitPart: second-level counts, then column prefixes
+    uint32_t *hist2;      // c2cap x kSplitFan2: second-level counts, then column prefixes
+    uint64_t *sub_base;   // subs+1: first record of each sub-part
+    uint32_t *c3base;     // subs+1: first third-level block of each sub-part
+    uint32_t *hist3;      // c3cap x kSplitSub: third-level counts, then column prefixes
+    uint32_t *sub_tot;    // subs: records of each sub-part
     uint32_t *key_tot;    // keys: in-edges of each key
-    uint32_t *rec_id;     // cap: first-level records, the in-edge's key
-    void *rec;            // cap pairs of OffT: first-level records
-    uint32_t *rec_aux;    // cap: first-level list records, where adj+(w) lies (slab_aux)
+    uint32_t *rec_id;     // cap: A keys
+    void *rec;            // cap pairs of OffT: A pairs, then ie
+    uint32_t *rec_aux;    // cap: A slab_aux words (list records), then ie_aux
+    uint32_t *rec_id2;    // cap: B keys
+    void *rec2;           // cap pairs of OffT: B pairs
+    uint32_t *rec_aux2;   // cap: B slab_aux words
     void *rp_in;          // keys+1 OffT: first in-edge of each key
-    void *ie;             // cap pairs of OffT: the records grouped by key
-    uint32_t *ie_aux;     // cap: the list records' slab_aux words, beside ie
-    uint64_t cap;         // in-edges the record and ie arrays hold (>= the in-edges)
+    void *ie;             // = rec: the records' pairs grouped by key
+    uint32_t *ie_aux;     // = rec_aux: the list records' slab_aux words, beside ie
+    uint64_t cap;         // records each array holds (>= the in-edges)
     uint32_t rows;        // top rows: n - thr
     uint32_t keys;        // rows + slabs
     uint32_t chunks;      // vertex chunks of the first level (about equal shares of the in-edges)
-    uint32_t parts;       // key parts of kSplitPart keys (<= kSplitPartsMax)
-    uint32_t c2cap;       // second-level blocks launched (>= those the records need)
+    uint32_t parts;       // parts of kSplitPart keys (<= kSplitPartsMax)
+    uint32_t subs;        // sub-parts: parts * kSplitFan2
     uint32_t slabs;       // slabs of the list rows: ceil(m / kSlabSpan)
+    uint32_t c2cap, c3cap;  // second- and third-level blocks the hists hold (>= those the records need)
     // device-filled by the range kernel
     uint64_t w_lo, w_hi;  // the counted wedges of the full order, clamped to [0, W]
     uint64_t clip_j;      // the in-edge whose tail the range cuts at the front (~0: none)
@@ -150,7 +168,7 @@ struct SplitDesc {
     uint32_t clip;        // 1 unless [w_lo, w_hi) is the whole order
     uint32_t pad_;
 };
-static_assert(sizeof(SplitDesc) == 184, "SplitDesc layout");
+static_assert(sizeof(SplitDesc) == 248, "SplitDesc layout");
 
 // Where adj+(w) of a list row lies in its slab's window: bits 0-12 its start
 // rowPtr[w] - slab*kSlabSpan, bits 13-25 its length (kSlabAuxLong: longer

@@ -59,12 +59,13 @@ hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int o
                                uint64_t *partials, hipStream_t st);
 // The counted range's ends and the in-edge it cuts at the front.
 hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipStream_t st);
-// The two-level counting sort of the in-edges by key (chunks first-level
-// blocks, c2cap second-level blocks); zeroes the tile counters, the wedge
-// tallies and counts[0].
+// The three-level counting sort of the in-edges by key (chunks first-level
+// blocks; c2cap and c3cap bound the second- and third-level blocks; subs
+// sub-parts, keys keys); zeroes the tile counters, the wedge tallies and
+// counts[0].
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
-                                  uint32_t c2cap, uint32_t keys, uint64_t *meta, unsigned long long *counts,
-                                  hipStream_t st);
+                                  uint32_t c2cap, uint32_t c3cap, uint32_t subs, uint32_t keys, uint64_t *meta,
+                                  unsigned long long *counts, hipStream_t st);
 // Waves per block of the row kernel: 4, 8 or 16.
 bool valid_split_waves(int wpb);
 // LDS bytes of the row kernel's bitmap for `rows` top rows.

@@ -25,10 +25,13 @@
 //
 // Pipeline on one stream: k_split_partials, k_scan_partials (twice) and
 // k_split_scan (the full wedge prefix and the in-edge prefix);
-// k_split_range; the transposition in two counting-sort levels,
+// k_split_range; the transposition in three counting-sort levels,
 // k_split_l1<count>, k_split_l1scan, k_split_l1<scatter> (records bucketed
-// by part of kSplitPart keys), k_split_l2<count>, k_split_l2cols,
-// k_split_rowscan, k_split_l2<scatter> (each part's records grouped by key);
+// by part of kSplitPart keys), k_split_level_count<2>,
+// k_split_levelcols<2>, k_split_subscan, k_split_level_scatter<2> (each
+// part's records bucketed by sub-part of kSplitSub keys),
+// k_split_level_count<3>, k_split_levelcols<3>, k_split_rowscan,
+// k_split_level_scatter<3> (each sub-part's records grouped by key);
 // k_split_rows (top rows); k_split_slabs (list rows).
 #include "wtc_dev.h"
 #include "wtc_kernels.h"
@@ -432,97 +435,263 @@ __global__ __launch_bounds__(kSplitPartsMax) void k_split_l1scan(const SplitDesc
     }
 }
 
-// Second level.  Block c2 takes records [part_base[q] + k*kSplitChunk2, ...)
-// of part q (c2 = c2base[q] + k; blocks past c2base[parts] do nothing) and
-// groups them by key of the part: kScatter = false counts them per key into
-// hist2[c2][key - q*kSplitPart]; kScatter = true (hist2 then holding, per
-// key, the part's records of earlier blocks) places each record at the next
-// slot of its key in ie.
-constexpr int kL2Records = 4;
+// Second and third levels.  Level 2 groups each part's A records by
+// sub-part into B; level 3 groups each sub-part's B records by key into ie.
+// A level's input bucket b (a part, or a sub-part) holds records
+// [base_in[b], base_in[b+1]) and is split into blocks of kChunk records
+// (its blocks start at cbase[b]); an output bucket is one of the kF
+// sub-parts of a part, or one of the kSplitSub keys of a sub-part.
+// k_split_level_count counts a block's records per output bucket into
+// hist[c][0..kF); k_split_level_scatter (hist then holding, per output
+// bucket, the records of the input bucket's earlier blocks) places each
+// record at the next slot of its output bucket.  Blocks stride over the
+// level's blocks.
+constexpr int kLevelRecords = 4;  // records per thread per step, loads issued together
+// Most blocks a level launches (they stride over its blocks of records).
+constexpr uint32_t kLevelGridMax = 2048;
 
-template <typename OffT, bool kScatter>
-__global__ __launch_bounds__(kSplitBlock) void k_split_l2(const SplitDesc *__restrict__ sd) {
-    __shared__ uint32_t s_cnt[kSplitPart];
+template <int kLevel>
+struct SplitLevel {
+    static constexpr uint32_t kF = kLevel == 2 ? kSplitFan2 : kSplitSub;
+    static constexpr uint32_t kShift = kLevel == 2 ? kSplitSubShift : 0u;
+    static constexpr uint32_t kChunk = kLevel == 2 ? kSplitChunk2 : kSplitChunk3;
+};
+
+// The last b in [0, nb) with cbase[b] <= c (cbase[0] = 0 <= c).
+__device__ __forceinline__ uint32_t bucket_of_block(const uint32_t *cbase, uint32_t nb, uint32_t c) {
+    uint32_t lo = 0, hi = nb;  // cbase[lo] <= c < cbase[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (cbase[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename OffT, int kLevel>
+__global__ __launch_bounds__(kSplitBlock) void k_split_level_count(const SplitDesc *__restrict__ sd) {
+    using L = SplitLevel<kLevel>;
+    constexpr uint32_t kF = L::kF;
+    __shared__ uint32_t s_cnt[kF];
     const SplitDesc &S = sd[0];
-    const uint32_t parts = S.parts, keys = S.keys, rows = S.rows;
-    const uint32_t c2 = blockIdx.x;
-    if (c2 >= S.c2base[parts]) return;  // block-uniform
-    uint32_t q = 0;
-    while (q + 1 < parts && S.c2base[q + 1] <= c2) q++;
-    const uint32_t r0 = q * kSplitPart, r1 = r0 + kSplitPart < keys ? r0 + kSplitPart : keys;
-    const uint64_t k0 = S.part_base[q] + (uint64_t)(c2 - S.c2base[q]) * kSplitChunk2;
-    const uint64_t kq = S.part_base[q + 1], k1 = k0 + kSplitChunk2 < kq ? k0 + kSplitChunk2 : kq;
-    uint32_t *__restrict__ hist2 = S.hist2 + (uint64_t)c2 * kSplitPart;
-    for (uint32_t i = threadIdx.x; i < r1 - r0; i += kSplitBlock) s_cnt[i] = kScatter ? hist2[i] : 0u;
-    __syncthreads();
-    const uint32_t *__restrict__ rec_id = S.rec_id;
-    const TailPair<OffT> *__restrict__ rec = static_cast<const TailPair<OffT> *>(S.rec);
-    const OffT *__restrict__ rp_in = static_cast<const OffT *>(S.rp_in);
-    TailPair<OffT> *__restrict__ ie = static_cast<TailPair<OffT> *>(S.ie);
-    const uint32_t *__restrict__ rec_aux = S.rec_aux;
-    uint32_t *__restrict__ ie_aux = S.ie_aux;
-    // kL2Records records per thread per step, their loads issued together
-    // (the scatter's key base and slot depend on the record's key)
-    constexpr int kU = kL2Records;
-    for (uint64_t kb = k0 + threadIdx.x; kb < k1; kb += (uint64_t)kU * kSplitBlock) {
-        uint32_t r[kU];
-#pragma unroll
-        for (int u = 0; u < kU; u++) {
-            const uint64_t k = kb + (uint64_t)u * kSplitBlock;
-            r[u] = k < k1 ? rec_id[k] : kSentinel;
-        }
-        if (!kScatter) {
-#pragma unroll
-            for (int u = 0; u < kU; u++)
-                if (r[u] != kSentinel) atomicAdd(&s_cnt[r[u] - r0], 1u);
-        } else {
-            TailPair<OffT> t[kU];
-            OffT base[kU];
-            uint32_t aux[kU];
+    const uint32_t nb = kLevel == 2 ? S.parts : S.subs;
+    const uint64_t *__restrict__ base_in = kLevel == 2 ? S.part_base : S.sub_base;
+    const uint32_t *__restrict__ cbase = kLevel == 2 ? S.c2base : S.c3base;
+    uint32_t *__restrict__ hist = kLevel == 2 ? S.hist2 : S.hist3;
+    const uint32_t *__restrict__ key_in = kLevel == 2 ? S.rec_id : S.rec_id2;
+    const uint32_t nblk = cbase[nb];
+    for (uint32_t c = blockIdx.x; c < nblk; c += gridDim.x) {
+        const uint32_t b = bucket_of_block(cbase, nb, c);
+        const uint64_t k0 = base_in[b] + (uint64_t)(c - cbase[b]) * L::kChunk;
+        const uint64_t kq = base_in[b + 1], k1 = k0 + L::kChunk < kq ? k0 + L::kChunk : kq;
+        __syncthreads();  // the previous block's LDS reads are done
+        for (uint32_t i = threadIdx.x; i < kF; i += kSplitBlock) s_cnt[i] = 0u;
+        __syncthreads();
+        constexpr int kU = kLevelRecords;
+        for (uint64_t kb = k0 + threadIdx.x; kb < k1; kb += (uint64_t)kU * kSplitBlock) {
+            uint32_t key[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++) {
                 const uint64_t k = kb + (uint64_t)u * kSplitBlock;
-                if (r[u] != kSentinel) {
-                    t[u] = rec[k];
-                    base[u] = rp_in[r[u]];
-                    if (r[u] >= rows) aux[u] = rec_aux[k];
-                }
+                key[u] = k < k1 ? key_in[k] : kSentinel;
             }
-            uint32_t slot[kU];
 #pragma unroll
             for (int u = 0; u < kU; u++)
-                if (r[u] != kSentinel) slot[u] = atomicAdd(&s_cnt[r[u] - r0], 1u);
-#pragma unroll
-            for (int u = 0; u < kU; u++) {
-                if (r[u] == kSentinel) continue;
-                const OffT p = base[u] + (OffT)slot[u];
-                ie[p] = t[u];
-                if (r[u] >= rows) ie_aux[p] = aux[u];
-            }
+                if (key[u] != kSentinel) atomicAdd(&s_cnt[(key[u] >> L::kShift) & (kF - 1)], 1u);
         }
-    }
-    if (!kScatter) {
         __syncthreads();
-        for (uint32_t i = threadIdx.x; i < r1 - r0; i += kSplitBlock) hist2[i] = s_cnt[i];
+        uint32_t *__restrict__ h = hist + (uint64_t)c * kF;
+        for (uint32_t i = threadIdx.x; i < kF; i += kSplitBlock) h[i] = s_cnt[i];
     }
 }
 
-// Thread per key r (part q): the column prefix of hist2 over the part's
-// second-level blocks (hist2[c2][r - q*kSplitPart] becomes the key's records
-// in earlier blocks) and the key's total.
-__global__ __launch_bounds__(kBlock) void k_split_l2cols(const SplitDesc *__restrict__ sd) {
+// Records a level's scatter stages in LDS per tile: 4096 with 4-byte
+// offsets, 2048 with 8-byte ones (32 + 8 KB of pairs and words either way).
+template <typename OffT>
+struct LevelTile {
+    static constexpr uint32_t kT = sizeof(OffT) == 4 ? 4096 : 2048;
+};
+
+// The scatter of a level.  hist[c] holds, per output bucket, the input
+// bucket's records in blocks before c.  The block takes its records in tiles
+// of kT: each tile is ranked per output bucket (LDS atomics), staged in LDS
+// sorted by output bucket, and written out in that order, so that the
+// records of one output bucket in a tile -- kT / kF of them on average --
+// leave as consecutive lanes of one store to consecutive addresses.
+template <typename OffT, int kLevel>
+__global__ __launch_bounds__(kSplitBlock) void k_split_level_scatter(const SplitDesc *__restrict__ sd) {
+    using L = SplitLevel<kLevel>;
+    constexpr uint32_t kF = L::kF;
+    constexpr uint32_t kT = LevelTile<OffT>::kT;
+    constexpr uint32_t kPerThread = kT / kSplitBlock;
+    __shared__ uint64_t s_base[kF];  // next output slot of each output bucket
+    __shared__ uint32_t s_tcnt[kF];  // the tile's records per output bucket
+    __shared__ uint32_t s_toff[kF];  // their exclusive prefix: each bucket's first staged slot
+    __shared__ uint32_t st_key[kT];
+    __shared__ TailPair<OffT> st_rec[kT];
+    __shared__ uint32_t st_aux[kT];
     const SplitDesc &S = sd[0];
-    const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
-    if (r >= S.keys) return;
-    const uint32_t q = r / kSplitPart, i = r - q * kSplitPart;
-    uint32_t *__restrict__ hist2 = S.hist2;
-    uint32_t s = 0;
-    for (uint32_t c2 = S.c2base[q]; c2 < S.c2base[q + 1]; c2++) {
-        const uint32_t h = hist2[(uint64_t)c2 * kSplitPart + i];
-        hist2[(uint64_t)c2 * kSplitPart + i] = s;
-        s += h;
+    const uint32_t rows = S.rows, keys = S.keys;
+    const uint32_t nb = kLevel == 2 ? S.parts : S.subs;
+    const uint64_t *__restrict__ base_in = kLevel == 2 ? S.part_base : S.sub_base;
+    const uint32_t *__restrict__ cbase = kLevel == 2 ? S.c2base : S.c3base;
+    const uint32_t *__restrict__ hist = kLevel == 2 ? S.hist2 : S.hist3;
+    const uint32_t *__restrict__ key_in = kLevel == 2 ? S.rec_id : S.rec_id2;
+    const TailPair<OffT> *__restrict__ rec_in = static_cast<const TailPair<OffT> *>(kLevel == 2 ? S.rec : S.rec2);
+    const uint32_t *__restrict__ aux_in = kLevel == 2 ? S.rec_aux : S.rec_aux2;
+    uint32_t *__restrict__ key_out = S.rec_id2;  // level 2 only
+    TailPair<OffT> *__restrict__ rec_out = static_cast<TailPair<OffT> *>(kLevel == 2 ? S.rec2 : S.ie);
+    uint32_t *__restrict__ aux_out = kLevel == 2 ? S.rec_aux2 : S.ie_aux;
+    const OffT *__restrict__ rp_in = static_cast<const OffT *>(S.rp_in);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    auto bucket = [](uint32_t key) { return (key >> L::kShift) & (kF - 1); };
+    const uint32_t nblk = cbase[nb];
+    for (uint32_t c = blockIdx.x; c < nblk; c += gridDim.x) {
+        const uint32_t b = bucket_of_block(cbase, nb, c);
+        const uint64_t k0 = base_in[b] + (uint64_t)(c - cbase[b]) * L::kChunk;
+        const uint64_t kq = base_in[b + 1], k1 = k0 + L::kChunk < kq ? k0 + L::kChunk : kq;
+        const uint32_t *__restrict__ h = hist + (uint64_t)c * kF;
+        __syncthreads();  // the previous block's LDS reads are done
+        for (uint32_t i = threadIdx.x; i < kF; i += kSplitBlock) {
+            // first slot of output bucket i: a sub-part's first record, or a key's first in-edge
+            const uint32_t o = b * kF + i;
+            const uint64_t ob = kLevel == 2 ? S.sub_base[o] : o < keys ? (uint64_t)rp_in[o] : 0;
+            s_base[i] = ob + h[i];
+        }
+        for (uint64_t t0 = k0; t0 < k1; t0 += kT) {  // block-uniform
+            for (uint32_t i = threadIdx.x; i < kF; i += kSplitBlock) s_tcnt[i] = 0u;
+            __syncthreads();
+            uint32_t key[kPerThread], rank[kPerThread], aux[kPerThread];
+            TailPair<OffT> t[kPerThread];
+#pragma unroll
+            for (uint32_t u = 0; u < kPerThread; u++) {
+                const uint64_t k = t0 + threadIdx.x + (uint64_t)u * kSplitBlock;
+                key[u] = k < k1 ? key_in[k] : kSentinel;
+                aux[u] = 0u;
+                if (key[u] != kSentinel) {
+                    t[u] = rec_in[k];
+                    if (key[u] >= rows) aux[u] = aux_in[k];
+                }
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kPerThread; u++)
+                if (key[u] != kSentinel) rank[u] = atomicAdd(&s_tcnt[bucket(key[u])], 1u);
+            __syncthreads();
+            if (threadIdx.x < kWave) {  // exclusive prefix of the tile's bucket counts, kF / 64 per lane
+                constexpr uint32_t kPer = kF / kWave;
+                uint32_t v[kPer], sum = 0;
+#pragma unroll
+                for (uint32_t z = 0; z < kPer; z++) {
+                    v[z] = s_tcnt[lane * kPer + z];
+                    sum += v[z];
+                }
+                uint32_t incl = sum;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)incl, o, kWave);
+                    if ((int)lane >= o) incl += y;
+                }
+                uint32_t run = incl - sum;
+#pragma unroll
+                for (uint32_t z = 0; z < kPer; z++) {
+                    s_toff[lane * kPer + z] = run;
+                    run += v[z];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (uint32_t u = 0; u < kPerThread; u++) {
+                if (key[u] == kSentinel) continue;
+                const uint32_t at = s_toff[bucket(key[u])] + rank[u];
+                st_key[at] = key[u];
+                st_rec[at] = t[u];
+                st_aux[at] = aux[u];
+            }
+            __syncthreads();
+            const uint32_t nt = (uint32_t)(k1 - t0 < kT ? k1 - t0 : kT);
+#pragma unroll
+            for (uint32_t u = 0; u < kPerThread; u++) {
+                const uint32_t i = threadIdx.x + u * kSplitBlock;
+                if (i >= nt) continue;
+                const uint32_t k = st_key[i], q = bucket(k);
+                const uint64_t p = s_base[q] + (i - s_toff[q]);
+                if (kLevel == 2) key_out[p] = k;
+                rec_out[p] = st_rec[i];
+                if (k >= rows) aux_out[p] = st_aux[i];
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < kF; i += kSplitBlock) s_base[i] += s_tcnt[i];
+        }
     }
-    S.key_tot[r] = s;
+}
+
+// Thread per output bucket o = b*kF + i of a level: the column prefix of
+// hist over input bucket b's blocks (hist[c][i] becomes o's records in b's
+// earlier blocks) and o's total (sub_tot, or key_tot at level 3).
+template <int kLevel>
+__global__ __launch_bounds__(kBlock) void k_split_levelcols(const SplitDesc *__restrict__ sd) {
+    constexpr uint32_t kF = SplitLevel<kLevel>::kF;
+    const SplitDesc &S = sd[0];
+    const uint32_t o = blockIdx.x * kBlock + threadIdx.x;
+    if (o >= (kLevel == 2 ? S.subs : S.keys)) return;
+    const uint32_t b = o / kF, i = o % kF;
+    const uint32_t *__restrict__ cbase = kLevel == 2 ? S.c2base : S.c3base;
+    uint32_t *__restrict__ hist = kLevel == 2 ? S.hist2 : S.hist3;
+    uint32_t s = 0;
+    const uint32_t c1 = cbase[b + 1];
+    for (uint32_t c = cbase[b]; c < c1; c += 8) {  // eight loads in flight
+        uint32_t x[8];
+#pragma unroll
+        for (uint32_t z = 0; z < 8; z++) x[z] = c + z < c1 ? hist[(uint64_t)(c + z) * kF + i] : 0u;
+#pragma unroll
+        for (uint32_t z = 0; z < 8; z++) {
+            if (c + z < c1) hist[(uint64_t)(c + z) * kF + i] = s;
+            s += x[z];
+        }
+    }
+    (kLevel == 2 ? S.sub_tot : S.key_tot)[o] = s;
+}
+
+// One block: sub_base = exclusive prefix of the sub-part totals (the
+// sub-parts in key order, so each part's sub-parts start at its own base)
+// and c3base = exclusive prefix of their third-level blocks.
+__global__ __launch_bounds__(kSplitBlock) void k_split_subscan(const SplitDesc *__restrict__ sd) {
+    constexpr int kWaves = kSplitBlock / kWave;
+    __shared__ uint64_t s_r[kWaves], s_b[kWaves];
+    const SplitDesc &S = sd[0];
+    const uint32_t subs = S.subs;
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+    uint64_t cr = 0, cb = 0;
+    for (uint32_t base = 0; base < subs; base += kSplitBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint64_t x = i < subs ? S.sub_tot[i] : 0;
+        const uint64_t y = (x + kSplitChunk3 - 1) / kSplitChunk3;
+        const uint64_t ix = wave_incl_scan(x), iy = wave_incl_scan(y);
+        if (lane == kWave - 1) {
+            s_r[wv] = ix;
+            s_b[wv] = iy;
+        }
+        __syncthreads();
+        uint64_t ox = 0, oy = 0, ax = 0, ay = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; w++) {
+            ox += w < (int)wv ? s_r[w] : 0;
+            oy += w < (int)wv ? s_b[w] : 0;
+            ax += s_r[w];
+            ay += s_b[w];
+        }
+        __syncthreads();
+        if (i < subs) {
+            S.sub_base[i] = cr + ox + ix - x;
+            S.c3base[i] = (uint32_t)(cb + oy + iy - y);
+        }
+        cr += ax;
+        cb += ay;
+    }
+    if (threadIdx.x == 0) {
+        S.sub_base[subs] = cr;
+        S.c3base[subs] = (uint32_t)cb;
+    }
 }
 
 // One block: rp_in = exclusive prefix of the key totals; rp_in[keys] = the
@@ -833,11 +1002,19 @@ hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipS
 }
 
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
-                                  uint32_t c2cap, uint32_t keys, uint64_t *meta, unsigned long long *counts,
-                                  hipStream_t st) {
-    if (keys == 0 || chunks == 0 || c2cap == 0) return hipErrorInvalidValue;
-    const dim3 blk(kSplitBlock), g1(chunks), g2(c2cap);
+                                  uint32_t c2cap, uint32_t c3cap, uint32_t subs, uint32_t keys, uint64_t *meta,
+                                  unsigned long long *counts, hipStream_t st) {
+    if (keys == 0 || chunks == 0 || c2cap == 0 || c3cap == 0 || subs == 0) return hipErrorInvalidValue;
+    const dim3 blk(kSplitBlock), g1(chunks);
+    const dim3 g2(c2cap < kLevelGridMax ? c2cap : kLevelGridMax), g3(c3cap < kLevelGridMax ? c3cap : kLevelGridMax);
+    const dim3 gs((subs + kBlock - 1) / kBlock), gk((keys + kBlock - 1) / kBlock);
     hipError_t e;
+#define WTC_SPLIT_LAUNCH(KERNEL, GRID, BLOCK, ...)                                             \
+    do {                                                                                       \
+        if (ob == 8) KERNEL<uint64_t, __VA_ARGS__><<<GRID, BLOCK, 0, st>>>(sd);                \
+        else KERNEL<uint32_t, __VA_ARGS__><<<GRID, BLOCK, 0, st>>>(sd);                        \
+        if ((e = hipGetLastError()) != hipSuccess) return e;                                   \
+    } while (0)
     if (ob == 8) k_split_l1<uint64_t, false><<<g1, blk, 0, st>>>(desc, sd);
     else k_split_l1<uint32_t, false><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -846,17 +1023,21 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
     if (ob == 8) k_split_l1<uint64_t, true><<<g1, blk, 0, st>>>(desc, sd);
     else k_split_l1<uint32_t, true><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_l2<uint64_t, false><<<g2, blk, 0, st>>>(sd);
-    else k_split_l2<uint32_t, false><<<g2, blk, 0, st>>>(sd);
+    WTC_SPLIT_LAUNCH(k_split_level_count, g2, blk, 2);
+    k_split_levelcols<2><<<gs, dim3(kBlock), 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_split_l2cols<<<dim3((keys + kBlock - 1) / kBlock), dim3(kBlock), 0, st>>>(sd);
+    k_split_subscan<<<dim3(1), blk, 0, st>>>(sd);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    WTC_SPLIT_LAUNCH(k_split_level_scatter, g2, blk, 2);
+    WTC_SPLIT_LAUNCH(k_split_level_count, g3, blk, 3);
+    k_split_levelcols<3><<<gk, dim3(kBlock), 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ob == 8) k_split_rowscan<uint64_t><<<dim3(1), blk, 0, st>>>(sd, meta, counts);
     else k_split_rowscan<uint32_t><<<dim3(1), blk, 0, st>>>(sd, meta, counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_l2<uint64_t, true><<<g2, blk, 0, st>>>(sd);
-    else k_split_l2<uint32_t, true><<<g2, blk, 0, st>>>(sd);
-    return hipGetLastError();
+    WTC_SPLIT_LAUNCH(k_split_level_scatter, g3, blk, 3);
+#undef WTC_SPLIT_LAUNCH
+    return hipSuccess;
 }
 
 size_t split_rows_lds(uint32_t rows) { return (size_t)((rows + 31) / 32) * sizeof(uint32_t); }
