@@ -153,7 +153,7 @@ namespace {
 // Byte offsets of the pieces of a split count's workspace in one buffer
 // (each on a 256-byte boundary), and the buffer's size.
 struct SplitLayout {
-    size_t wsfull, hist1, part_base, c2base, hist2, sub_base, c3base, hist3, sub_tot, key_tot;
+    size_t wsfull, hist1, part_base, c2base, hist2, sub_base, c3base, hist3, sub_tot, list_bid, slab_row, key_tot;
     size_t recid, rec, recaux, recid2, rec2, recaux2, rpin, total;
     uint32_t chunks, slabs, keys, parts, subs, c2cap, c3cap;
     uint64_t cap;
@@ -188,6 +188,8 @@ SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
     L.c3base = take(((size_t)L.subs + 1) * sizeof(uint32_t));
     L.hist3 = take((size_t)L.c3cap * kSplitSub * sizeof(uint32_t));
     L.sub_tot = take((size_t)L.subs * sizeof(uint32_t));
+    L.list_bid = take((size_t)L.parts * sizeof(uint32_t));
+    L.slab_row = take(((size_t)L.slabs + 1) * sizeof(uint32_t));
     L.key_tot = take((size_t)L.keys * sizeof(uint32_t));
     L.recid = take((size_t)L.cap * sizeof(uint32_t));
     L.rec = take((size_t)L.cap * pair);
@@ -225,6 +227,8 @@ void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t ro
     sd->c3base = reinterpret_cast<uint32_t *>(base + L.c3base);
     sd->hist3 = reinterpret_cast<uint32_t *>(base + L.hist3);
     sd->sub_tot = reinterpret_cast<uint32_t *>(base + L.sub_tot);
+    sd->list_bid = reinterpret_cast<uint32_t *>(base + L.list_bid);
+    sd->slab_row = reinterpret_cast<uint32_t *>(base + L.slab_row);
     sd->key_tot = reinterpret_cast<uint32_t *>(base + L.key_tot);
     sd->rec_id = reinterpret_cast<uint32_t *>(base + L.recid);
     sd->rec = base + L.rec;

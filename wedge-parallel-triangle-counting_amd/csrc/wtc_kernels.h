@@ -114,11 +114,13 @@ constexpr int kSplitBlock = 1024;
 constexpr uint32_t kSplitTile = 1024;
 // Slabs of the list rows (middle vertex below thr): slab s holds the rows
 // whose adjacency starts in colInd[s*kSlabSpan, (s+1)*kSlabSpan).  The slab
-// kernel keeps colInd[s*kSlabSpan, +kSlabWin) in LDS, which holds every such
-// row of out-degree <= kSlabRowMax (longer rows are searched in HBM).
-constexpr uint32_t kSlabSpan = 8192;
-constexpr uint32_t kSlabRowMax = 4096;
+// kernel hashes colInd[s*kSlabSpan, +kSlabWin) into LDS, one bucket of
+// kSlabWays slots per position, which covers every such row of out-degree
+// <= kSlabRowMax (longer rows are searched in HBM).
+constexpr uint32_t kSlabSpan = 2048;
+constexpr uint32_t kSlabRowMax = 1024;
 constexpr uint32_t kSlabWin = kSlabSpan + kSlabRowMax;
+constexpr uint32_t kSlabWays = 4;
 constexpr uint32_t kSlabTile = 1024;
 
 // Workspace and device-filled state of a split count.  The top rows are the
@@ -142,6 +144,8 @@ struct SplitDesc {
     uint32_t *c3base;     // subs+1: first third-level block of each sub-part
     uint32_t *hist3;      // c3cap x kSplitSub: third-level counts, then column prefixes
     uint32_t *sub_tot;    // subs: records of each sub-part
+    uint32_t *list_bid;   // parts: the first list vertex w whose key lies in each part (0 up to the first list part)
+    uint32_t *slab_row;   // slabs+1: the first vertex w with rowPtr[w] >= s*kSlabSpan (set for s*kSlabSpan <= rowPtr[thr])
     uint32_t *key_tot;    // keys: in-edges of each key
     uint32_t *rec_id;     // cap: A keys
     void *rec;            // cap pairs of OffT: A pairs, then ie
@@ -168,12 +172,13 @@ struct SplitDesc {
     uint32_t clip;        // 1 unless [w_lo, w_hi) is the whole order
     uint32_t pad_;
 };
-static_assert(sizeof(SplitDesc) == 248, "SplitDesc layout");
+static_assert(sizeof(SplitDesc) == 264, "SplitDesc layout");
 
 // Where adj+(w) of a list row lies in its slab's window: bits 0-12 its start
 // rowPtr[w] - slab*kSlabSpan, bits 13-25 its length (kSlabAuxLong: longer
 // than kSlabRowMax, searched in HBM).
 constexpr uint32_t kSlabAuxLong = 0x1fffu;
 static_assert(kSlabSpan <= 8192 && kSlabRowMax < kSlabAuxLong, "slab_aux fields");
+static_assert(sizeof(SplitDesc) % 8 == 0, "SplitDesc alignment");
 
 }  // namespace wtc

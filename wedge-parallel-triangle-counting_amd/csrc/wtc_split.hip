@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void k_split_partials(const GraphDesc *__re
     }
 }
 
-// ws_full and ie_pre from the scanned block partials.
+// ws_full and ie_pre from the scanned block partials, and slab_row.
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void k_split_scan(const GraphDesc *__restrict__ desc,
                                                        const SplitDesc *__restrict__ sd,
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(kBlock) void k_split_scan(const GraphDesc *__restri
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
     uint64_t *__restrict__ wf = sd->ws_full;
     uint64_t *__restrict__ wi = sd->ie_pre;
-    const uint32_t n = D.n;
+    uint32_t *__restrict__ slab_row = sd->slab_row;
+    const uint32_t n = D.n, thr = D.thr, slabs = sd->slabs;
     uint64_t cf = part_full[blockIdx.x], ci = part_in[blockIdx.x];
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
     for (int k = 0; k < kScanItems; k++) {
@@ -123,6 +124,11 @@ __global__ __launch_bounds__(kBlock) void k_split_scan(const GraphDesc *__restri
         if (v < n) {
             wf[v] = cf + ef;
             wi[v] = ci + ei;
+            if (v <= thr) {  // the slabs whose first row is v: rowPtr[v - 1] < s*kSlabSpan <= rowPtr[v]
+                const uint64_t s_hi = (uint64_t)rowPtr[v] / kSlabSpan;
+                for (uint64_t s = v ? (uint64_t)rowPtr[v - 1] / kSlabSpan + 1 : 0; s <= s_hi && s <= slabs; s++)
+                    slab_row[s] = (uint32_t)v;
+            }
         }
         cf += tf;
         ci += ti;
@@ -133,15 +139,34 @@ __global__ __launch_bounds__(kBlock) void k_split_scan(const GraphDesc *__restri
     }
 }
 
-// One thread: the counted range of the full order, its end vertices, and
-// the one in-edge whose tail the range cuts at the front (row i0 of v_lo
-// holding wedge w_lo: its tail starts w_lo - base(i0) ids in).
+// Thread 0: the counted range of the full order, its end vertices, and the
+// one in-edge whose tail the range cuts at the front (row i0 of v_lo holding
+// wedge w_lo: its tail starts w_lo - base(i0) ids in).  Thread q < parts:
+// list_bid[q], the first list vertex w (below thr) whose key rows +
+// rowPtr[w] / kSlabSpan lies in part q or later (keys grow with w).
 template <typename OffT>
-__global__ void k_split_range(const GraphDesc *__restrict__ desc, SplitDesc *__restrict__ sd) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(kSplitPartsMax) void k_split_range(const GraphDesc *__restrict__ desc,
+                                                                SplitDesc *__restrict__ sd) {
     const GraphDesc &D = desc[0];
     SplitDesc &S = sd[0];
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
+    const uint32_t q = threadIdx.x;
+    if (q < S.parts) {
+        const uint64_t first = S.rows / kSplitPart;  // the part of the first list key
+        uint32_t bid = 0;
+        if (q > first) {
+            const uint64_t at = ((uint64_t)q * kSplitPart - S.rows) * kSlabSpan;  // colInd position
+            uint32_t lo = 0, hi = D.thr;  // the first w in [0, thr] with rowPtr[w] >= at (thr: none below)
+            while (lo < hi) {
+                const uint32_t mid = lo + ((hi - lo) >> 1);
+                if ((uint64_t)rowPtr[mid] < at) lo = mid + 1;
+                else hi = mid;
+            }
+            bid = lo;
+        }
+        S.list_bid[q] = bid;
+    }
+    if (threadIdx.x != 0) return;
     const uint32_t n = D.n;
     const uint64_t *wf = S.ws_full;
     const uint64_t W = wf[n];
@@ -222,16 +247,17 @@ constexpr uint32_t kSplitBigVertex = kWave;
 // ones dropped) gets its key -- w - thr for a top row w, else rows +
 // rowPtr[w] / kSlabSpan -- and belongs to the part key / kSplitPart.
 // kScatter = false: the block counts its in-edges per part into
-// hist1[b][part]; kScatter = true (hist1 then holding the record offset of
-// each chunk and part): the in-edges of a wave step take the next record
-// slots of their parts (one LDS atomic per part present), and their keys and
-// records are stored there.  Both passes see the same in-edges in the same
-// chunks.
+// hist1[b][part] (a list vertex's part from list_bid, without reading its
+// row); kScatter = true (hist1 then holding the record offset of each chunk
+// and part): the in-edges of a wave step take the next record slots of
+// their parts (one LDS atomic per part present), and their keys and records
+// are stored there.  Both passes see the same in-edges in the same chunks.
 template <typename OffT, bool kScatter>
 __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__restrict__ desc,
                                                           const SplitDesc *__restrict__ sd) {
     __shared__ uint32_t s_cnt[kSplitPartsMax];
     __shared__ uint64_t s_base[kSplitPartsMax];
+    __shared__ uint32_t s_bid[kSplitPartsMax];
     const GraphDesc &D = desc[0];
     const SplitDesc &S = sd[0];
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
@@ -246,16 +272,31 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) {
         s_cnt[i] = 0u;
         s_base[i] = kScatter ? hist1[b * parts + i] : 0u;
+        s_bid[i] = S.list_bid[i];
     }
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
     const uint64_t below = lane ? ~0ull >> (kWave - lane) : 0ull;  // lanes under this one
 
-    // The wave's in-edges of one step (act lanes): counted in their parts,
-    // or stored at their parts' next record slots.  One LDS atomic per part
-    // present in the wave, so a part's records of one step are consecutive.
-    // Called by the whole wave.
+    // The part of a list vertex's key, from list_bid (no rowPtr read).
+    auto list_part = [&](uint32_t id) -> uint32_t {
+        uint32_t lo = 0, hi = parts;  // s_bid[lo] <= id < s_bid[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_bid[mid] <= id) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
+    // The count pass: the in-edge of an act lane counted in its part.
+    auto count = [&](bool act, uint32_t id) {
+        if (act) atomicAdd(&s_cnt[id >= thr ? (id - thr) / kSplitPart : list_part(id)], 1u);
+    };
+    // The scatter pass: the wave's in-edges of one step (act lanes) stored at
+    // their parts' next record slots.  One LDS atomic per part present in
+    // the wave, so a part's records of one step are consecutive.  Called by
+    // the whole wave.
     auto take = [&](bool act, uint32_t key, OffT first, OffT hi, uint32_t aux) {
         const uint32_t q = act ? key / kSplitPart : kSplitPartsMax;
         uint64_t pend = __ballot(act);
@@ -265,18 +306,14 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
             const uint64_t same = __ballot(q == q0);
             pend &= ~same;
             const uint32_t cnt = (uint32_t)__popcll(same);
-            if (!kScatter) {
-                if ((int)lane == src) atomicAdd(&s_cnt[q0], cnt);
-            } else {
-                uint32_t at = 0;
-                if ((int)lane == src) at = atomicAdd(&s_cnt[q0], cnt);
-                at = (uint32_t)__builtin_amdgcn_readlane((int)at, src);
-                if (q == q0) {
-                    const uint64_t p = s_base[q0] + at + (uint32_t)__popcll(same & below);
-                    rec_id[p] = key;
-                    rec[p] = TailPair<OffT>{first, hi};
-                    if (key >= rows) rec_aux[p] = aux;
-                }
+            uint32_t at = 0;
+            if ((int)lane == src) at = atomicAdd(&s_cnt[q0], cnt);
+            at = (uint32_t)__builtin_amdgcn_readlane((int)at, src);
+            if (q == q0) {
+                const uint64_t p = s_base[q0] + at + (uint32_t)__popcll(same & below);
+                rec_id[p] = key;
+                rec[p] = TailPair<OffT>{first, hi};
+                if (key >= rows) rec_aux[p] = aux;
             }
         }
     };
@@ -335,7 +372,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
                 for (int u = 0; u < 4; u++) {
                     const OffT j = jb + (OffT)(u * kWave) + lane;
                     rp[u] = rp1[u] = 0;
-                    if (j < vend && ids[u] < thr) {
+                    if (kScatter && j < vend && ids[u] < thr) {
                         rp[u] = rowPtr[ids[u]];
                         rp1[u] = rowPtr[ids[u] + 1];
                     }
@@ -345,8 +382,12 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
                     const OffT j = jb + (OffT)(u * kWave) + lane;
                     OffT lo = j + 1, hi = ve;
                     if (vedge && j < vend) clip_tail<OffT>(j, ve, vrb, vws, w_lo, w_hi, lo, hi);
-                    const bool top = ids[u] >= thr;
-                    take(j < vend && lo < hi, key_of(ids[u], rp[u]), top ? lo : j, hi, aux_of(rp[u], rp1[u]));
+                    if (!kScatter) {
+                        count(j < vend && lo < hi, ids[u]);
+                    } else {
+                        const bool top = ids[u] >= thr;
+                        take(j < vend && lo < hi, key_of(ids[u], rp[u]), top ? lo : j, hi, aux_of(rp[u], rp1[u]));
+                    }
                 }
             }
         }
@@ -382,12 +423,16 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
             }
             const bool act = idx < tot && lo < hi;
             const uint32_t id = act ? colInd[j] : 0u;
-            OffT rp = 0, rp1 = 0;
-            if (act && id < thr) {
-                rp = rowPtr[id];
-                rp1 = rowPtr[id + 1];
+            if (!kScatter) {
+                count(act, id);
+            } else {
+                OffT rp = 0, rp1 = 0;
+                if (act && id < thr) {
+                    rp = rowPtr[id];
+                    rp1 = rowPtr[id + 1];
+                }
+                take(act, key_of(id, rp), id >= thr ? lo : j, hi, aux_of(rp, rp1));
             }
-            take(act, key_of(id, rp), id >= thr ? lo : j, hi, aux_of(rp, rp1));
         }
     }
     if (!kScatter) {
@@ -843,44 +888,34 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
 
 // ---------------------------------------------------------------- list part
 
-// Lower bound of u in the ascending s[a, e): the first position whose id is
-// >= u (e if none), galloping from a (the bound of a smaller id) and then
-// halving.
-__device__ __forceinline__ uint32_t lds_lower_bound(const uint32_t *s, uint32_t a, uint32_t e, uint32_t u) {
-    if (a >= e || s[a] >= u) return a;
-    uint32_t step = 1, b = a + 1;  // s[a] < u
-    while (b < e && s[b] < u) {
-        a = b;
-        step <<= 1;
-        b = a + step;
-    }
-    // the bound lies in (a, min(b, e)]
-    a += 1;
-    if (b > e) b = e;
-    while (a < b) {
-        const uint32_t mid = (a + b) >> 1;
-        if (s[mid] < u) a = mid + 1;
-        else b = mid;
-    }
-    return a;
+// Bucket of id u in a row's region of len buckets (multiplicative hash).
+__device__ __forceinline__ uint32_t slab_bucket(uint32_t u, uint32_t len) {
+    return __umulhi(u * 0x9e3779b1u, len);
 }
 
 // Persistent blocks take tiles of kSlabTile consecutive list in-edges (in
-// slab order) from one counter.  Per slab of a tile: colInd[s*kSlabSpan,
-// +kSlabWin) copied into LDS (kept while the next tile stays in the slab),
-// the tile's in-edges of the slab dealt to the waves (kLanes lanes per
-// in-edge), and each tail streamed by 16-byte loads.  A record's slab_aux
-// word places adj+(w) in the window, so the tail loads follow the record
-// load directly; each id u is looked up there by a galloping lower bound
-// from the lane's last bound (a lane's ids only grow along a tail), or in
-// HBM for a row longer than kSlabRowMax.  Same alignment and slack needs as
-// the row kernel.
+// slab order) from one counter.  A tile's in-edges of one slab form a
+// segment.  Per segment, the block hashes into LDS every row of the slab the
+// segment's in-edges name that is not there yet (the table is kept while
+// the next segment stays in the slab; a wave per row clears its buckets and
+// inserts its ids): the row starting at window position st with len
+// entries owns the buckets [st, st + len) of kSlabWays slots each, so a
+// quarter of its slots are filled and a lookup nearly always ends in its
+// first bucket (one 16-byte LDS read).  Then the segment's in-edges are
+// dealt to the waves, kLanes lanes each; each tail is streamed by 16-byte
+// loads and each id u looked up in adj+(w) by linear probing from its hash
+// bucket, or by a binary search in HBM for a row longer than kSlabRowMax.
+// Same alignment and slack needs as the row kernel.
 template <typename OffT, int kWaves, int kLanes>
 __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc *__restrict__ desc,
                                                                 const SplitDesc *__restrict__ sd,
                                                                 uint64_t *__restrict__ meta,
                                                                 unsigned long long *__restrict__ counts) {
-    __shared__ uint32_t s_win[kSlabWin];
+    constexpr uint32_t kRefWords = kSlabSpan / 32;
+    __shared__ __align__(16) uint32_t s_tab[kSlabWin * kSlabWays];
+    __shared__ uint32_t s_ref[kRefWords];  // rows (by start position) hashed so far
+    __shared__ uint32_t s_new[kSlabTile];  // slab_aux words of the rows a segment adds
+    __shared__ uint32_t s_nnew;
     __shared__ uint64_t s_tile;
     const GraphDesc &D = desc[0];
     const SplitDesc &S = sd[0];
@@ -891,14 +926,12 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc 
     const gptr<uint32_t> auxG = to_global(static_cast<const uint32_t *>(S.ie_aux));
     const uint32_t rows = S.rows, keys = S.keys;
     const uint64_t L0 = (uint64_t)rp_in[rows], L1 = (uint64_t)rp_in[keys];
-    const uint64_t mEnd = (uint64_t)to_const(rowPtr)[D.n];
     const OffT clip_j = (OffT)S.clip_j, clip_lo = (OffT)S.clip_lo;
     const bool clipped = S.clip_j != ~0ull;
-    constexpr uint32_t kPer = kWave / kLanes;  // in-edges a wave takes per step
+    constexpr uint32_t kThreads = kWave * kWaves;
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = uni32(threadIdx.x >> 6);
-    const uint32_t sub = lane / kLanes, sl = lane % kLanes;
     unsigned long long *const ctr = reinterpret_cast<unsigned long long *>(meta + kMetaSlabs);
-    uint32_t cur = kSentinel;  // slab in s_win (block-uniform)
+    uint32_t cur = kSentinel;  // slab in s_tab (block-uniform)
     uint64_t hits = 0, checked = 0;
     for (;;) {
         __syncthreads();
@@ -914,15 +947,41 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc 
             const uint64_t kend = re < k1 ? re : k1;
             const uint32_t slab = key - rows;
             const uint64_t wb = (uint64_t)slab * kSlabSpan;  // window start in colInd
-            if (slab != cur) {
-                __syncthreads();  // the previous window's searches are done
-                for (uint32_t i = threadIdx.x; i < kSlabWin; i += kWave * kWaves) {
-                    const uint64_t p = wb + i;
-                    s_win[i] = p < mEnd ? colInd[p] : kSentinel;
-                }
-                cur = slab;
-                __syncthreads();
+            // the rows this segment names that the table lacks
+            __syncthreads();  // the previous segment's lookups are done
+            if (slab != cur)
+                for (uint32_t i = threadIdx.x; i < kRefWords; i += kThreads) s_ref[i] = 0u;
+            if (threadIdx.x == 0) s_nnew = 0u;
+            cur = slab;
+            __syncthreads();
+            for (uint64_t x = k + threadIdx.x; x < kend; x += kThreads) {
+                const uint32_t aux = auxG[x], st = aux & kSlabAuxLong, len = aux >> 13;
+                if (len == kSlabAuxLong || len == 0) continue;  // searched in HBM, or empty
+                const uint32_t bit = 1u << (st & 31u);
+                if ((atomicOr(&s_ref[st >> 5], bit) & bit) == 0u) s_new[atomicAdd(&s_nnew, 1u)] = aux;
             }
+            __syncthreads();
+            // a wave per new row: its buckets cleared, then its ids inserted
+            // (one wave's LDS operations stay in order)
+            const uint32_t nnew = s_nnew;
+            for (uint32_t r = wv; r < nnew; r += kWaves) {
+                const uint32_t aux = s_new[r], st = aux & kSlabAuxLong, len = aux >> 13;
+                for (uint32_t i = lane; i < len; i += kWave)
+                    reinterpret_cast<uint4 *>(s_tab)[st + i] = make_uint4(kSentinel, kSentinel, kSentinel, kSentinel);
+                for (uint32_t i = lane; i < len; i += kWave) {
+                    const uint32_t u = colInd[wb + st + i];
+                    uint32_t bk = st + slab_bucket(u, len);
+                    for (bool done = false; !done;) {
+                        for (uint32_t z = 0; z < kSlabWays && !done; z++)
+                            done = atomicCAS(&s_tab[bk * kSlabWays + z], kSentinel, u) == kSentinel;
+                        bk = bk + 1 == st + len ? st : bk + 1;
+                    }
+                }
+            }
+            __syncthreads();
+            // the segment's in-edges dealt to the waves, kLanes lanes each
+            constexpr uint32_t kPer = kWave / kLanes;
+            const uint32_t sub = lane / kLanes, sl = lane % kLanes;
             uint32_t h = 0;
             for (uint64_t x0 = k + (uint64_t)wv * kPer; x0 < kend; x0 += (uint64_t)kWaves * kPer) {
                 const uint64_t x = x0 + sub;
@@ -937,15 +996,14 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc 
                 if (clipped && j == clip_j) lo = clip_lo;
                 if (x >= kend) hi = lo;  // nothing to stream
                 if (sl == 0) checked += (uint64_t)(hi - lo);
-                uint32_t lb = aux & kSlabAuxLong, len = aux >> 13;
-                const bool inwin = len != kSlabAuxLong;
+                const uint32_t st = aux & kSlabAuxLong, len = aux >> 13;
+                if (len == 0) continue;  // adj+(w) is empty: no triangle
                 OffT gb = 0, ge = 0;  // adj+(w) in HBM for a long row
-                if (!inwin && x < kend) {
+                if (len == kSlabAuxLong) {
                     const uint32_t w = colInd[j];
                     gb = rowPtr[w];
                     ge = rowPtr[w + 1];
                 }
-                const uint32_t le = inwin ? lb + len : 0u;
                 for (OffT q = (lo & ~(OffT)3) + 4 * sl; q < hi; q += 4 * kLanes) {
                     const u32x4 a = *reinterpret_cast<gptr<u32x4>>(colInd + q);
                     const uint32_t ua[4] = {a[0], a[1], a[2], a[3]};
@@ -954,21 +1012,30 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc 
                         const OffT t = q + (OffT)z;
                         if (t < lo || t >= hi) continue;
                         const uint32_t u = ua[z];
-                        if (inwin) {
-                            lb = lds_lower_bound(s_win, lb, le, u);
-                            h += lb < le && s_win[lb] == u ? 1u : 0u;
+                        if (len != kSlabAuxLong) {
+                            uint32_t bk = st + slab_bucket(u, len);
+                            for (;;) {
+                                const uint4 v = reinterpret_cast<const uint4 *>(s_tab)[bk];
+                                if (v.x == u || v.y == u || v.z == u || v.w == u) {
+                                    h++;
+                                    break;
+                                }
+                                if (v.w == kSentinel) break;  // slots fill in order: a free one ends the chain
+                                bk = bk + 1 == st + len ? st : bk + 1;
+                            }
                         } else {
-                            OffT n_ = ge - gb;
+                            OffT lb = gb, n_ = ge - gb;
                             while (n_ > 0) {
                                 const OffT half = n_ >> 1;
-                                if (colInd[gb + half] < u) {
-                                    gb += half + 1;
+                                if (colInd[lb + half] < u) {
+                                    lb += half + 1;
                                     n_ -= half + 1;
                                 } else {
                                     n_ = half;
                                 }
                             }
-                            h += gb < ge && colInd[gb] == u ? 1u : 0u;
+                            gb = lb;  // the lane's next id is larger
+                            h += lb < ge && colInd[lb] == u ? 1u : 0u;
                         }
                     }
                 }
@@ -998,7 +1065,7 @@ hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int o
 }
 
 hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipStream_t st) {
-    return WTC_LAUNCH_OFF(ob, k_split_range, dim3(1), dim3(kWave), 0, st, desc, sd);
+    return WTC_LAUNCH_OFF(ob, k_split_range, dim3(1), dim3(kSplitPartsMax), 0, st, desc, sd);
 }
 
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
