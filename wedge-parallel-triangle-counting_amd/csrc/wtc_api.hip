@@ -154,6 +154,7 @@ namespace {
 // (each on a 256-byte boundary), and the buffer's size.
 struct SplitLayout {
     size_t wsfull, hist1, part_base, c2base, hist2, sub_base, c3base, hist3, sub_tot, list_bid, slab_row, key_tot;
+    size_t part_tot, key_part;
     size_t recid, rec, recaux, recid2, rec2, recaux2, rpin, total;
     uint32_t chunks, slabs, keys, parts, subs, c2cap, c3cap;
     uint64_t cap;
@@ -190,6 +191,8 @@ SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
     L.sub_tot = take((size_t)L.subs * sizeof(uint32_t));
     L.list_bid = take((size_t)L.parts * sizeof(uint32_t));
     L.slab_row = take(((size_t)L.slabs + 1) * sizeof(uint32_t));
+    L.part_tot = take((size_t)L.parts * sizeof(uint64_t));
+    L.key_part = take(((size_t)L.keys / kKeyChunk + 2) * sizeof(uint64_t));
     L.key_tot = take((size_t)L.keys * sizeof(uint32_t));
     L.recid = take((size_t)L.cap * sizeof(uint32_t));
     L.rec = take((size_t)L.cap * pair);
@@ -229,6 +232,8 @@ void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t ro
     sd->sub_tot = reinterpret_cast<uint32_t *>(base + L.sub_tot);
     sd->list_bid = reinterpret_cast<uint32_t *>(base + L.list_bid);
     sd->slab_row = reinterpret_cast<uint32_t *>(base + L.slab_row);
+    sd->part_tot = reinterpret_cast<uint64_t *>(base + L.part_tot);
+    sd->key_part = reinterpret_cast<uint64_t *>(base + L.key_part);
     sd->key_tot = reinterpret_cast<uint32_t *>(base + L.key_tot);
     sd->rec_id = reinterpret_cast<uint32_t *>(base + L.recid);
     sd->rec = base + L.rec;
@@ -385,7 +390,8 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->split_keys = sd.keys;
     plan->split_c2cap = sd.c2cap;
     plan->split_c3cap = sd.c3cap;
-    plan->split_subs = sd.subs;
+    plan->split_parts = sd.parts;
+    plan->split_key_part = sd.key_part;
     plan->split_wpb = t.split_wpb;
     return WTC_OK;
 }
@@ -401,8 +407,8 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
         WTC_HIPC(launch_split_prefix(ddesc, sd, p.ob, p.blocks, s.partials.as<uint64_t>(), s.stream));
         WTC_HIPC(launch_split_range(ddesc, sd, p.ob, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
-        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_c3cap, p.split_subs,
-                                        p.split_keys, meta, dcounts, s.stream));
+        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_c3cap, p.split_parts,
+                                        p.split_keys, p.split_key_part, meta, dcounts, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
         WTC_HIPC(launch_split_rows(ddesc, sd, p.ob, p.split_rows, p.split_wpb, meta, dcounts, s.rows_grid,
                                    s.stream));

@@ -168,8 +168,9 @@ struct SetPlan {
     // split count: SplitDesc at byte split_off of the descriptor buffer
     bool split = false;
     size_t split_off = 0;
-    uint32_t split_rows = 0, split_keys = 0, split_chunks = 0, split_c2cap = 0, split_c3cap = 0, split_subs = 0;
+    uint32_t split_rows = 0, split_keys = 0, split_chunks = 0, split_c2cap = 0, split_c3cap = 0, split_parts = 0;
     uint32_t split_wpb = 8;
+    uint64_t *split_key_part = nullptr;  // device: the key-block prefix of the transposition
 };
 // Sizes the slot's workspace for the set, assigns each graph its prefix and
 // bitmap (unless given), fills the pinned host descriptors and the plan.

@@ -112,6 +112,8 @@ constexpr uint32_t kSplitChunk2 = 65536;
 constexpr uint32_t kSplitChunk3 = 65536;
 constexpr int kSplitBlock = 1024;
 constexpr uint32_t kSplitTile = 1024;
+// Keys per block of the key-offset scan (four per thread).
+constexpr uint32_t kKeyChunk = 4 * kSplitBlock;
 // Slabs of the list rows (middle vertex below thr): slab s holds the rows
 // whose adjacency starts in colInd[s*kSlabSpan, (s+1)*kSlabSpan).  The slab
 // kernel hashes colInd[s*kSlabSpan, +kSlabWin) into LDS, one bucket of
@@ -146,6 +148,8 @@ struct SplitDesc {
     uint32_t *sub_tot;    // subs: records of each sub-part
     uint32_t *list_bid;   // parts: the first list vertex w whose key lies in each part (0 up to the first list part)
     uint32_t *slab_row;   // slabs+1: the first vertex w with rowPtr[w] >= s*kSlabSpan (set for s*kSlabSpan <= rowPtr[thr])
+    uint64_t *part_tot;   // parts: records of each part
+    uint64_t *key_part;   // key blocks: in-edges of each block of kKeyChunk keys, then their prefix
     uint32_t *key_tot;    // keys: in-edges of each key
     uint32_t *rec_id;     // cap: A keys
     void *rec;            // cap pairs of OffT: A pairs, then ie
@@ -172,7 +176,7 @@ struct SplitDesc {
     uint32_t clip;        // 1 unless [w_lo, w_hi) is the whole order
     uint32_t pad_;
 };
-static_assert(sizeof(SplitDesc) == 264, "SplitDesc layout");
+static_assert(sizeof(SplitDesc) == 280, "SplitDesc layout");
 
 // Where adj+(w) of a list row lies in its slab's window: bits 0-12 its start
 // rowPtr[w] - slab*kSlabSpan, bits 13-25 its length (kSlabAuxLong: longer

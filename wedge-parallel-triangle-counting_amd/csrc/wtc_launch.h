@@ -60,12 +60,12 @@ hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int o
 // The counted range's ends and the in-edge it cuts at the front.
 hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipStream_t st);
 // The three-level counting sort of the in-edges by key (chunks first-level
-// blocks; c2cap and c3cap bound the second- and third-level blocks; subs
-// sub-parts, keys keys); zeroes the tile counters, the wedge tallies and
-// counts[0].
+// blocks; c2cap and c3cap bound the second- and third-level blocks; parts
+// parts, keys keys; key_part = the SplitDesc's key_part, on the device);
+// zeroes the tile counters, the wedge tallies and counts[0].
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
-                                  uint32_t c2cap, uint32_t c3cap, uint32_t subs, uint32_t keys, uint64_t *meta,
-                                  unsigned long long *counts, hipStream_t st);
+                                  uint32_t c2cap, uint32_t c3cap, uint32_t parts, uint32_t keys, uint64_t *key_part,
+                                  uint64_t *meta, unsigned long long *counts, hipStream_t st);
 // Waves per block of the row kernel: 4, 8 or 16.
 bool valid_split_waves(int wpb);
 // LDS bytes of the row kernel's bitmap for `rows` top rows.
@@ -81,6 +81,7 @@ hipError_t launch_split_slabs(const GraphDesc *desc, const SplitDesc *sd, int ob
 // Vertex chunks of the transposition (equal shares of the in-edges): one
 // per kSplitChunkVerts vertices, at least 1 and at most kSplitChunksMax.
 constexpr uint32_t kSplitChunksMax = 1024;
+static_assert(kSplitChunksMax <= (uint32_t)kSplitBlock, "one first-level chunk per thread of the part scan");
 constexpr uint32_t kSplitChunkVerts = 2048;
 
 // The `work` word of a grab size and XCD dealing.

@@ -26,12 +26,13 @@
 // Pipeline on one stream: k_split_partials, k_scan_partials (twice) and
 // k_split_scan (the full wedge prefix and the in-edge prefix);
 // k_split_range; the transposition in three counting-sort levels,
-// k_split_l1<count>, k_split_l1scan, k_split_l1<scatter> (records bucketed
-// by part of kSplitPart keys), k_split_level_count<2>,
+// k_split_l1<count>, k_split_l1cols, k_split_l1bases, k_split_l1<scatter>
+// (records bucketed by part of kSplitPart keys), k_split_level_count<2>,
 // k_split_levelcols<2>, k_split_subscan, k_split_level_scatter<2> (each
 // part's records bucketed by sub-part of kSplitSub keys),
-// k_split_level_count<3>, k_split_levelcols<3>, k_split_rowscan,
-// k_split_level_scatter<3> (each sub-part's records grouped by key);
+// k_split_level_count<3>, k_split_levelcols<3>, k_split_keysum,
+// k_scan_partials, k_split_keyscan, k_split_level_scatter<3> (each
+// sub-part's records grouped by key);
 // k_split_rows (top rows); k_split_slabs (list rows).
 #include "wtc_dev.h"
 #include "wtc_kernels.h"
@@ -248,8 +249,8 @@ constexpr uint32_t kSplitBigVertex = kWave;
 // rowPtr[w] / kSlabSpan -- and belongs to the part key / kSplitPart.
 // kScatter = false: the block counts its in-edges per part into
 // hist1[b][part] (a list vertex's part from list_bid, without reading its
-// row); kScatter = true (hist1 then holding the record offset of each chunk
-// and part): the in-edges of a wave step take the next record slots of
+// row); kScatter = true (hist1 then holding each part's records in earlier
+// chunks, and part_base its first record): the in-edges of a wave step take the next record slots of
 // their parts (one LDS atomic per part present), and their keys and records
 // are stored there.  Both passes see the same in-edges in the same chunks.
 template <typename OffT, bool kScatter>
@@ -271,7 +272,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     const uint32_t b = blockIdx.x;
     for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) {
         s_cnt[i] = 0u;
-        s_base[i] = kScatter ? hist1[b * parts + i] : 0u;
+        s_base[i] = kScatter ? S.part_base[i] + hist1[b * parts + i] : 0u;
         s_bid[i] = S.list_bid[i];
     }
     __syncthreads();
@@ -441,42 +442,58 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     }
 }
 
-// One thread per part (one block): the records of each part (its column of
-// hist1 summed), the parts' first records (part_base), each chunk's record
-// offset in each part (hist1 rewritten), and the first second-level block
-// of each part (c2base: kSplitChunk2 records per block).
-__global__ __launch_bounds__(kSplitPartsMax) void k_split_l1scan(const SplitDesc *__restrict__ sd) {
-    __shared__ uint64_t s_tot[kSplitPartsMax];
-    const SplitDesc &S = sd[0];
-    const uint32_t parts = S.parts, chunks = S.chunks, q = threadIdx.x;
-    uint64_t *__restrict__ hist1 = S.hist1;
-    uint64_t tot = 0;
-    if (q < parts)
-        for (uint32_t c = 0; c < chunks; c++) tot += hist1[c * parts + q];
-    s_tot[q] = q < parts ? tot : 0;
+// Exclusive scan over the kSplitBlock threads of a block; the block's sum
+// into total.  s_w holds a word per wave.
+__device__ __forceinline__ uint64_t split_block_scan(uint64_t x, uint64_t *s_w, uint64_t &total) {
+    constexpr int kWaves = kSplitBlock / kWave;
+    const uint64_t inc = wave_incl_scan(x);
+    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
+    if (lane == kWave - 1) s_w[wv] = inc;
     __syncthreads();
-    if (q == 0) {
-        uint64_t base = 0;
-        uint32_t c2 = 0;
-        for (uint32_t i = 0; i < parts; i++) {
-            const uint64_t t = s_tot[i];
-            S.part_base[i] = base;
-            S.c2base[i] = c2;
-            s_tot[i] = base;
-            base += t;
-            c2 += (uint32_t)((t + kSplitChunk2 - 1) / kSplitChunk2);
-        }
-        S.part_base[parts] = base;
-        S.c2base[parts] = c2;
+    uint64_t off = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        const uint64_t t = s_w[w];
+        off += w < (int)wv ? t : 0;
+        all += t;
     }
     __syncthreads();
+    total = all;
+    return off + inc - x;
+}
+
+// Block q (a part), thread c (a chunk): hist1[c][q] becomes the part's
+// records in earlier chunks, and part_tot[q] its total.
+__global__ __launch_bounds__(kSplitBlock) void k_split_l1cols(const SplitDesc *__restrict__ sd) {
+    __shared__ uint64_t s_w[kSplitBlock / kWave];
+    const SplitDesc &S = sd[0];
+    const uint32_t q = blockIdx.x, c = threadIdx.x, parts = S.parts;
+    uint64_t *__restrict__ hist1 = S.hist1;
+    const uint64_t x = c < S.chunks ? hist1[(uint64_t)c * parts + q] : 0;
+    uint64_t tot;
+    const uint64_t ex = split_block_scan(x, s_w, tot);
+    if (c < S.chunks) hist1[(uint64_t)c * parts + q] = ex;
+    if (c == 0) S.part_tot[q] = tot;
+}
+
+// One block, thread per part: part_base and c2base (kSplitChunk2 records per
+// second-level block) as prefixes of the part totals.
+__global__ __launch_bounds__(kBlock) void k_split_l1bases(const SplitDesc *__restrict__ sd) {
+    __shared__ uint64_t s_tot[kWavesPerBlock];
+    static_assert(kSplitPartsMax <= (uint32_t)kBlock, "a part per thread");
+    const SplitDesc &S = sd[0];
+    const uint32_t q = threadIdx.x, parts = S.parts;
+    const uint64_t t = q < parts ? S.part_tot[q] : 0, nb = (t + kSplitChunk2 - 1) / kSplitChunk2;
+    uint64_t tt, tb;
+    const uint64_t et = block_excl_scan(t, s_tot, tt);
+    const uint64_t eb = block_excl_scan(nb, s_tot, tb);
     if (q < parts) {
-        uint64_t off = s_tot[q];
-        for (uint32_t c = 0; c < chunks; c++) {
-            const uint64_t h = hist1[c * parts + q];
-            hist1[c * parts + q] = off;
-            off += h;
-        }
+        S.part_base[q] = et;
+        S.c2base[q] = (uint32_t)eb;
+    }
+    if (q == 0) {
+        S.part_base[parts] = tt;
+        S.c2base[parts] = (uint32_t)tb;
     }
 }
 
@@ -739,50 +756,65 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_subscan(const SplitDesc *
     }
 }
 
-// One block: rp_in = exclusive prefix of the key totals; rp_in[keys] = the
-// in-edges; the tile counters and wedge tallies zeroed; the count zeroed;
-// the counted wedges of the full order, the top rows' in-edges and all
-// in-edges into meta.
+// Block b: the in-edges of keys [b*kKeyChunk, (b+1)*kKeyChunk) into
+// key_part[b].
+__global__ __launch_bounds__(kSplitBlock) void k_split_keysum(const SplitDesc *__restrict__ sd) {
+    __shared__ uint64_t s_w[kSplitBlock / kWave];
+    const SplitDesc &S = sd[0];
+    const uint64_t i0 = (uint64_t)blockIdx.x * kKeyChunk + 4ull * threadIdx.x;
+    uint64_t x = 0;
+#pragma unroll
+    for (uint32_t z = 0; z < 4; z++)
+        if (i0 + z < S.keys) x += S.key_tot[i0 + z];
+    x = wave_sum(x);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < kSplitBlock / kWave; w++) t += s_w[w];
+        S.key_part[blockIdx.x] = t;
+    }
+}
+
+// Block b (key_part then holding the exclusive prefix of the key blocks):
+// rp_in of its keys; the block of key `rows` puts the top rows' in-edges
+// into meta, the last block rp_in[keys] and the rest of meta (tile counters
+// and wedge tallies zeroed, the counted wedges and all in-edges) and
+// counts[0] = 0.
 template <typename OffT>
-__global__ __launch_bounds__(kSplitBlock) void k_split_rowscan(const SplitDesc *__restrict__ sd,
+__global__ __launch_bounds__(kSplitBlock) void k_split_keyscan(const SplitDesc *__restrict__ sd,
                                                                uint64_t *__restrict__ meta,
                                                                unsigned long long *__restrict__ counts) {
-    constexpr int kWaves = kSplitBlock / kWave;
-    __shared__ uint64_t s_w[kWaves];
+    __shared__ uint64_t s_w[kSplitBlock / kWave];
     const SplitDesc &S = sd[0];
     const uint32_t keys = S.keys, rows = S.rows;
-    const uint32_t *__restrict__ tot = S.key_tot;
     OffT *__restrict__ rp = static_cast<OffT *>(S.rp_in);
-    const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
-    uint64_t carry = 0;
-    for (uint32_t base = 0; base < keys; base += kSplitBlock) {
-        const uint32_t i = base + threadIdx.x;
-        const uint64_t x = i < keys ? tot[i] : 0;
-        const uint64_t inc = wave_incl_scan(x);
-        if (lane == kWave - 1) s_w[wv] = inc;
-        __syncthreads();
-        uint64_t off = 0, all = 0;
+    const uint64_t i0 = (uint64_t)blockIdx.x * kKeyChunk + 4ull * threadIdx.x;
+    uint32_t v[4];
+    uint64_t x = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; w++) {
-            const uint64_t t = s_w[w];
-            off += w < (int)wv ? t : 0;
-            all += t;
-        }
-        __syncthreads();
-        const uint64_t at = carry + off + inc - x;
-        if (i < keys) rp[i] = (OffT)at;
-        if (i == rows) meta[kMetaInEdges] = at;
-        carry += all;
+    for (uint32_t z = 0; z < 4; z++) {
+        v[z] = i0 + z < keys ? S.key_tot[i0 + z] : 0u;
+        x += v[z];
     }
-    if (threadIdx.x == 0) {
-        rp[keys] = (OffT)carry;
-        if (rows >= keys) meta[kMetaInEdges] = carry;
+    uint64_t tot;
+    uint64_t at = S.key_part[blockIdx.x] + split_block_scan(x, s_w, tot);
+#pragma unroll
+    for (uint32_t z = 0; z < 4; z++) {
+        if (i0 + z < keys) rp[i0 + z] = (OffT)at;
+        if (i0 + z == rows) meta[kMetaInEdges] = at;
+        at += v[z];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const uint64_t all = S.key_part[blockIdx.x] + tot;
+        rp[keys] = (OffT)all;
+        if (rows >= keys) meta[kMetaInEdges] = all;
         meta[kMetaRows] = 0;
         meta[kMetaSlabs] = 0;
         meta[kMetaTopWedges] = 0;
         meta[kMetaListWedges] = 0;
         meta[kMetaWedges] = S.w_hi - S.w_lo;
-        meta[kMetaAllEdges] = carry;
+        meta[kMetaAllEdges] = all;
         counts[0] = 0;
     }
 }
@@ -1069,9 +1101,11 @@ hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipS
 }
 
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
-                                  uint32_t c2cap, uint32_t c3cap, uint32_t subs, uint32_t keys, uint64_t *meta,
-                                  unsigned long long *counts, hipStream_t st) {
-    if (keys == 0 || chunks == 0 || c2cap == 0 || c3cap == 0 || subs == 0) return hipErrorInvalidValue;
+                                  uint32_t c2cap, uint32_t c3cap, uint32_t parts, uint32_t keys, uint64_t *key_part,
+                                  uint64_t *meta, unsigned long long *counts, hipStream_t st) {
+    const uint32_t subs = parts * kSplitFan2;
+    if (keys == 0 || chunks == 0 || c2cap == 0 || c3cap == 0 || parts == 0 || parts > kSplitPartsMax)
+        return hipErrorInvalidValue;
     const dim3 blk(kSplitBlock), g1(chunks);
     const dim3 g2(c2cap < kLevelGridMax ? c2cap : kLevelGridMax), g3(c3cap < kLevelGridMax ? c3cap : kLevelGridMax);
     const dim3 gs((subs + kBlock - 1) / kBlock), gk((keys + kBlock - 1) / kBlock);
@@ -1085,7 +1119,9 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
     if (ob == 8) k_split_l1<uint64_t, false><<<g1, blk, 0, st>>>(desc, sd);
     else k_split_l1<uint32_t, false><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    k_split_l1scan<<<dim3(1), dim3(kSplitPartsMax), 0, st>>>(sd);
+    k_split_l1cols<<<dim3(parts), blk, 0, st>>>(sd);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    k_split_l1bases<<<dim3(1), dim3(kBlock), 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ob == 8) k_split_l1<uint64_t, true><<<g1, blk, 0, st>>>(desc, sd);
     else k_split_l1<uint32_t, true><<<g1, blk, 0, st>>>(desc, sd);
@@ -1099,8 +1135,12 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
     WTC_SPLIT_LAUNCH(k_split_level_count, g3, blk, 3);
     k_split_levelcols<3><<<gk, dim3(kBlock), 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_rowscan<uint64_t><<<dim3(1), blk, 0, st>>>(sd, meta, counts);
-    else k_split_rowscan<uint32_t><<<dim3(1), blk, 0, st>>>(sd, meta, counts);
+    const dim3 gkb((keys + kKeyChunk - 1) / kKeyChunk);
+    k_split_keysum<<<gkb, blk, 0, st>>>(sd);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = launch_scan_partials(key_part, gkb.x, st)) != hipSuccess) return e;
+    if (ob == 8) k_split_keyscan<uint64_t><<<gkb, blk, 0, st>>>(sd, meta, counts);
+    else k_split_keyscan<uint32_t><<<gkb, blk, 0, st>>>(sd, meta, counts);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     WTC_SPLIT_LAUNCH(k_split_level_scatter, g3, blk, 3);
 #undef WTC_SPLIT_LAUNCH
