@@ -299,24 +299,12 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     // the wave, so a part's records of one step are consecutive.  Called by
     // the whole wave.
     auto take = [&](bool act, uint32_t key, OffT first, OffT hi, uint32_t aux) {
-        const uint32_t q = act ? key / kSplitPart : kSplitPartsMax;
-        uint64_t pend = __ballot(act);
-        while (pend) {
-            const int src = __builtin_ctzll(pend);
-            const uint32_t q0 = (uint32_t)__builtin_amdgcn_readlane((int)q, src);
-            const uint64_t same = __ballot(q == q0);
-            pend &= ~same;
-            const uint32_t cnt = (uint32_t)__popcll(same);
-            uint32_t at = 0;
-            if ((int)lane == src) at = atomicAdd(&s_cnt[q0], cnt);
-            at = (uint32_t)__builtin_amdgcn_readlane((int)at, src);
-            if (q == q0) {
-                const uint64_t p = s_base[q0] + at + (uint32_t)__popcll(same & below);
-                rec_id[p] = key;
-                rec[p] = TailPair<OffT>{first, hi};
-                if (key >= rows) rec_aux[p] = aux;
-            }
-        }
+        if (!act) return;
+        const uint32_t q = key / kSplitPart;
+        const uint64_t p = s_base[q] + atomicAdd(&s_cnt[q], 1u);
+        rec_id[p] = key;
+        rec[p] = TailPair<OffT>{first, hi};
+        if (key >= rows) rec_aux[p] = aux;
     };
     // key of middle vertex id, given rowPtr[id] for a list row
     auto key_of = [&](uint32_t id, OffT rp) -> uint32_t {
