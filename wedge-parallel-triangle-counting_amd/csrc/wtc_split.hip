@@ -250,9 +250,9 @@ constexpr uint32_t kSplitBigVertex = kWave;
 // kScatter = false: the block counts its in-edges per part into
 // hist1[b][part] (a list vertex's part from list_bid, without reading its
 // row); kScatter = true (hist1 then holding each part's records in earlier
-// chunks, and part_base its first record): the in-edges of a wave step take the next record slots of
-// their parts (one LDS atomic per part present), and their keys and records
-// are stored there.  Both passes see the same in-edges in the same chunks.
+// chunks, and part_base its first record): each in-edge takes the next
+// record slot of its part (an LDS cursor), and its key and record are
+// stored there.  Both passes see the same in-edges in the same chunks.
 template <typename OffT, bool kScatter>
 __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__restrict__ desc,
                                                           const SplitDesc *__restrict__ sd) {
@@ -278,7 +278,6 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
-    const uint64_t below = lane ? ~0ull >> (kWave - lane) : 0ull;  // lanes under this one
 
     // The part of a list vertex's key, from list_bid (no rowPtr read).
     auto list_part = [&](uint32_t id) -> uint32_t {
@@ -294,10 +293,10 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     auto count = [&](bool act, uint32_t id) {
         if (act) atomicAdd(&s_cnt[id >= thr ? (id - thr) / kSplitPart : list_part(id)], 1u);
     };
-    // The scatter pass: the wave's in-edges of one step (act lanes) stored at
-    // their parts' next record slots.  One LDS atomic per part present in
-    // the wave, so a part's records of one step are consecutive.  Called by
-    // the whole wave.
+    // The scatter pass: the in-edge of an act lane stored at its part's next
+    // record slot.  The lanes of a wave that share a part take consecutive
+    // slots, and a chunk's few part cursors stay in the L2 cache, so the
+    // stores leave in whole lines.
     auto take = [&](bool act, uint32_t key, OffT first, OffT hi, uint32_t aux) {
         if (!act) return;
         const uint32_t q = key / kSplitPart;
