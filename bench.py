@@ -447,9 +447,9 @@ def tune_sweep(args):
             for a in (split_alens if kernel == wtc.KERNEL_SPLIT else alens):
                 if kernel == wtc.KERNEL_SPLIT and (a == 0 or (big and a < 131072)):
                     continue
-                # (the split count's spread shapes only its list part's walk)
+                # (the split count has no spread: one value stands in its row of the table)
                 log(f"tune {workload}: kernel {kernel} a {a}")
-                for sp in ((2, 4, 8, 16, 32, 64, 128) if kernel == wtc.KERNEL_WALK else (12,) if big else (8, 12, 16)):
+                for sp in ((2, 4, 8, 16, 32, 64, 128) if kernel == wtc.KERNEL_WALK else (12,)):
                     if sp > 8 and W < (1 << 20):
                         continue
                     for sw0 in ((8, 16) if kernel == wtc.KERNEL_SPLIT and a > 262144 else (base.split_waves,)):
@@ -460,9 +460,10 @@ def tune_sweep(args):
                             best = (ms, t)
         ms0, t0 = best
         log(f"tune {workload}: stage 1 best {ms0:.3f} ms {t0.as_dict()}")
-        for waves in ((4,) if big else (2, 4, 8)):
-            for ways in ((4,) if big else (2, 4, 8)):
-                for sw in ((8, 16) if big else (4, 8, 16) if t0.kernel == wtc.KERNEL_SPLIT else (8,)):
+        split = t0.kernel == wtc.KERNEL_SPLIT  # its kernels take only the row kernel's block shape
+        for waves in ((4,) if big or split else (2, 4, 8)):
+            for ways in ((4,) if big or split else (2, 4, 8)):
+                for sw in ((8, 16) if big else (4, 8, 16) if split else (8,)):
                     t = wtc.Tuned(t0.spread, t0.adjacency_matrix_len, t0.kernel, waves, ways, sw)
                     ctx.use_tuned(t)
                     ms, _ = _time_count(dg, t.spread, t.adjacency_matrix_len)

@@ -230,16 +230,17 @@ int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int wav
  *   WTC_KERNEL_STAGED each lane decodes its spread of s wedges and stages
  *                     them through LDS, as the reference does
  *                     (tc.cu:174-244); lds_chunk applies.
- *   WTC_KERNEL_SPLIT  a single graph with a > 0: the wedges whose middle
- *                     vertex w is among the top-a ranks (the reference's
- *                     bitmap probes, tc.cu:262-272) are counted grouped by
- *                     w against adj+(w) held as a bitmap row in LDS (the
- *                     in-edges of the top rows are grouped by a counting
- *                     sort per count; no a*a/8-byte bitmap in HBM, an extra
- *                     8 or 16 bytes per edge of workspace instead); the rest
- *                     (sorted-list probes, tc.cu:273-286) by the row walk.
- *                     Batches, a = 0 and colInd not 16-byte aligned: the
- *                     row walk.  Same count for every wedge range.
+ *   WTC_KERNEL_SPLIT  a single graph with a > 0: every in-edge v -> w is
+ *                     grouped by its middle vertex w with a counting sort
+ *                     per count; the wedges whose w is among the top-a
+ *                     ranks (the reference's bitmap probes, tc.cu:262-272)
+ *                     are counted against adj+(w) held as a bitmap row in
+ *                     LDS, the rest (sorted-list probes, tc.cu:273-286)
+ *                     against hash sets of adj+(w) in LDS.  No a*a/8-byte
+ *                     bitmap in HBM; 32 or 48 bytes per edge of workspace
+ *                     instead.  Batches, a = 0, a row of a bits that does
+ *                     not fit a block's LDS and colInd not 16-byte aligned:
+ *                     the row walk.  Same count for every wedge range.
  * Default WTC_KERNEL_WALK or $WTC_COUNT_KERNEL. */
 #define WTC_KERNEL_WALK 0
 #define WTC_KERNEL_STAGED 1
