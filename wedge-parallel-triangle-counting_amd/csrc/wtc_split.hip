@@ -1032,14 +1032,14 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc 
                         if (t < lo || t >= hi) continue;
                         const uint32_t u = ua[z];
                         if (len != kSlabAuxLong) {
+                            // one 16-byte LDS read per bucket; the four slots compared without
+                            // branches; slots fill in order, so a free last slot ends the chain
                             uint32_t bk = st + slab_bucket(u, len);
                             for (;;) {
-                                const uint4 v = reinterpret_cast<const uint4 *>(s_tab)[bk];
-                                if (v.x == u || v.y == u || v.z == u || v.w == u) {
-                                    h++;
-                                    break;
-                                }
-                                if (v.w == kSentinel) break;  // slots fill in order: a free one ends the chain
+                                const u32x4 v = reinterpret_cast<const u32x4 *>(s_tab)[bk];
+                                const bool f = (v[0] == u) | (v[1] == u) | (v[2] == u) | (v[3] == u);
+                                h += f ? 1u : 0u;
+                                if (f | (v[3] == kSentinel)) break;
                                 bk = bk + 1 == st + len ? st : bk + 1;
                             }
                         } else {
