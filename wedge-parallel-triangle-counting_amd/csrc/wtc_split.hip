@@ -922,8 +922,9 @@ __device__ __forceinline__ uint32_t slab_bucket(uint32_t u, uint32_t len) {
 // quarter of its slots are filled and a lookup nearly always ends in its
 // first bucket (one 16-byte LDS read).  Then the segment's in-edges are
 // dealt to the waves, kLanes lanes each; each tail is streamed by 16-byte
-// loads and each id u looked up in adj+(w) by linear probing from its hash
-// bucket, or by a binary search in HBM for a row longer than kSlabRowMax.
+// loads, two in flight per lane, and each id u looked up in adj+(w) by
+// linear probing from its hash bucket, or by a binary search in HBM for a
+// row longer than kSlabRowMax.
 // Same alignment and slack needs as the row kernel.
 template <typename OffT, int kWaves, int kLanes>
 __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc *__restrict__ desc,
@@ -1023,12 +1024,15 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc 
                     gb = rowPtr[w];
                     ge = rowPtr[w + 1];
                 }
-                for (OffT q = (lo & ~(OffT)3) + 4 * sl; q < hi; q += 4 * kLanes) {
+                for (OffT q = (lo & ~(OffT)3) + 4 * sl; q < hi; q += 8 * kLanes) {  // two loads in flight
+                    const OffT q1 = q + 4 * kLanes;
                     const u32x4 a = *reinterpret_cast<gptr<u32x4>>(colInd + q);
-                    const uint32_t ua[4] = {a[0], a[1], a[2], a[3]};
+                    u32x4 b = {0u, 0u, 0u, 0u};
+                    if (q1 < hi) b = *reinterpret_cast<gptr<u32x4>>(colInd + q1);
+                    const uint32_t ua[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
 #pragma unroll
-                    for (int z = 0; z < 4; z++) {
-                        const OffT t = q + (OffT)z;
+                    for (int z = 0; z < 8; z++) {
+                        const OffT t = (z < 4 ? q : q1) + (OffT)(z & 3);
                         if (t < lo || t >= hi) continue;
                         const uint32_t u = ua[z];
                         if (len != kSlabAuxLong) {
