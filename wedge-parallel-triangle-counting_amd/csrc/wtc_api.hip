@@ -194,10 +194,10 @@ SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
     L.part_tot = take((size_t)L.parts * sizeof(uint64_t));
     L.key_part = take(((size_t)L.keys / kKeyChunk + 2) * sizeof(uint64_t));
     L.key_tot = take((size_t)L.keys * sizeof(uint32_t));
-    L.recid = take((size_t)L.cap * sizeof(uint32_t));
+    L.recid = take((size_t)L.cap * sizeof(uint16_t));
     L.rec = take((size_t)L.cap * pair);
     L.recaux = take((size_t)L.cap * sizeof(uint32_t));
-    L.recid2 = take((size_t)L.cap * sizeof(uint32_t));
+    L.recid2 = take((size_t)L.cap * sizeof(uint8_t));
     L.rec2 = take((size_t)L.cap * pair);
     L.recaux2 = take((size_t)L.cap * sizeof(uint32_t));
     L.rpin = take(((size_t)L.keys + 1) * (size_t)ob);
@@ -235,10 +235,10 @@ void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t ro
     sd->part_tot = reinterpret_cast<uint64_t *>(base + L.part_tot);
     sd->key_part = reinterpret_cast<uint64_t *>(base + L.key_part);
     sd->key_tot = reinterpret_cast<uint32_t *>(base + L.key_tot);
-    sd->rec_id = reinterpret_cast<uint32_t *>(base + L.recid);
+    sd->rec_id = reinterpret_cast<uint16_t *>(base + L.recid);
     sd->rec = base + L.rec;
     sd->rec_aux = reinterpret_cast<uint32_t *>(base + L.recaux);
-    sd->rec_id2 = reinterpret_cast<uint32_t *>(base + L.recid2);
+    sd->rec_id2 = reinterpret_cast<uint8_t *>(base + L.recid2);
     sd->rec2 = base + L.rec2;
     sd->rec_aux2 = reinterpret_cast<uint32_t *>(base + L.recaux2);
     sd->rp_in = base + L.rpin;

@@ -108,6 +108,7 @@ constexpr uint32_t kSplitSub = 256;
 constexpr uint32_t kSplitSubShift = 8;
 constexpr uint32_t kSplitFan2 = kSplitPart / kSplitSub;
 static_assert(kSplitSub == 1u << kSplitSubShift && kSplitFan2 * kSplitSub == kSplitPart, "key levels");
+static_assert(kSplitPart <= 65536 && kSplitSub <= 256, "A keys fit 16 bits, B keys 8");
 constexpr uint32_t kSplitChunk2 = 65536;
 constexpr uint32_t kSplitChunk3 = 65536;
 constexpr int kSplitBlock = 1024;
@@ -132,7 +133,8 @@ constexpr uint32_t kSlabTile = 1024;
 // rows + the slab of w.  Records (key, pair, slab_aux) go through the
 // levels A -> B -> ie: A bucketed by part, B by sub-part, ie by key (the
 // pairs only, in A's pair array; a list record's slab_aux in A's aux
-// array).  A top record's pair is the tail [lo, hi) of colInd; a list
+// array).  A record's position names its part (in A) or sub-part (in B), so
+// A keeps the key's low 15 bits and B its low 8.  A top record's pair is the tail [lo, hi) of colInd; a list
 // record's is (j, hi), its tail [j + 1, hi) (the one in-edge whose tail the
 // range cuts at the front: [clip_lo, hi)).
 struct SplitDesc {
@@ -151,10 +153,10 @@ struct SplitDesc {
     uint64_t *part_tot;   // parts: records of each part
     uint64_t *key_part;   // key blocks: in-edges of each block of kKeyChunk keys, then their prefix
     uint32_t *key_tot;    // keys: in-edges of each key
-    uint32_t *rec_id;     // cap: A keys
+    uint16_t *rec_id;     // cap: A keys within their part (key mod kSplitPart)
     void *rec;            // cap pairs of OffT: A pairs, then ie
     uint32_t *rec_aux;    // cap: A slab_aux words (list records), then ie_aux
-    uint32_t *rec_id2;    // cap: B keys
+    uint8_t *rec_id2;     // cap: B keys within their sub-part (key mod kSplitSub)
     void *rec2;           // cap pairs of OffT: B pairs
     uint32_t *rec_aux2;   // cap: B slab_aux words
     void *rp_in;          // keys+1 OffT: first in-edge of each key

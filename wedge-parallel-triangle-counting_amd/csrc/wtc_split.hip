@@ -34,6 +34,8 @@
 // k_scan_partials, k_split_keyscan, k_split_level_scatter<3> (each
 // sub-part's records grouped by key);
 // k_split_rows (top rows); k_split_slabs (list rows).
+#include <type_traits>
+
 #include "wtc_dev.h"
 #include "wtc_kernels.h"
 #include "wtc_launch.h"
@@ -265,7 +267,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     const gptr<uint32_t> colInd = to_global(D.colInd);
     const uint64_t *__restrict__ wf = S.ws_full;
     uint64_t *__restrict__ hist1 = S.hist1;
-    uint32_t *__restrict__ rec_id = S.rec_id;
+    uint16_t *__restrict__ rec_id = S.rec_id;
     TailPair<OffT> *__restrict__ rec = static_cast<TailPair<OffT> *>(S.rec);
     uint32_t *__restrict__ rec_aux = S.rec_aux;
     const uint32_t n = D.n, thr = D.thr, rows = S.rows, parts = S.parts;
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
         if (!act) return;
         const uint32_t q = key / kSplitPart;
         const uint64_t p = s_base[q] + atomicAdd(&s_cnt[q], 1u);
-        rec_id[p] = key;
+        rec_id[p] = (uint16_t)(key & (kSplitPart - 1));
         rec[p] = TailPair<OffT>{first, hi};
         if (key >= rows) rec_aux[p] = aux;
     };
@@ -499,11 +501,15 @@ constexpr int kLevelRecords = 4;  // records per thread per step, loads issued t
 // Most blocks a level launches (they stride over its blocks of records).
 constexpr uint32_t kLevelGridMax = 2048;
 
+// A level's input keys are the keys within their input bucket (15 bits in
+// A, 8 in B): key = b * kUnit + local.
 template <int kLevel>
 struct SplitLevel {
     static constexpr uint32_t kF = kLevel == 2 ? kSplitFan2 : kSplitSub;
     static constexpr uint32_t kShift = kLevel == 2 ? kSplitSubShift : 0u;
     static constexpr uint32_t kChunk = kLevel == 2 ? kSplitChunk2 : kSplitChunk3;
+    static constexpr uint32_t kUnit = kLevel == 2 ? kSplitPart : kSplitSub;
+    using KeyIn = typename std::conditional<kLevel == 2, uint16_t, uint8_t>::type;
 };
 
 // The last b in [0, nb) with cbase[b] <= c (cbase[0] = 0 <= c).
@@ -527,7 +533,8 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_level_count(const SplitDe
     const uint64_t *__restrict__ base_in = kLevel == 2 ? S.part_base : S.sub_base;
     const uint32_t *__restrict__ cbase = kLevel == 2 ? S.c2base : S.c3base;
     uint32_t *__restrict__ hist = kLevel == 2 ? S.hist2 : S.hist3;
-    const uint32_t *__restrict__ key_in = kLevel == 2 ? S.rec_id : S.rec_id2;
+    const typename L::KeyIn *__restrict__ key_in =
+        reinterpret_cast<const typename L::KeyIn *>(kLevel == 2 ? (const void *)S.rec_id : (const void *)S.rec_id2);
     const uint32_t nblk = cbase[nb];
     for (uint32_t c = blockIdx.x; c < nblk; c += gridDim.x) {
         const uint32_t b = bucket_of_block(cbase, nb, c);
@@ -585,10 +592,11 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_level_scatter(const Split
     const uint64_t *__restrict__ base_in = kLevel == 2 ? S.part_base : S.sub_base;
     const uint32_t *__restrict__ cbase = kLevel == 2 ? S.c2base : S.c3base;
     const uint32_t *__restrict__ hist = kLevel == 2 ? S.hist2 : S.hist3;
-    const uint32_t *__restrict__ key_in = kLevel == 2 ? S.rec_id : S.rec_id2;
+    const typename L::KeyIn *__restrict__ key_in =
+        reinterpret_cast<const typename L::KeyIn *>(kLevel == 2 ? (const void *)S.rec_id : (const void *)S.rec_id2);
     const TailPair<OffT> *__restrict__ rec_in = static_cast<const TailPair<OffT> *>(kLevel == 2 ? S.rec : S.rec2);
     const uint32_t *__restrict__ aux_in = kLevel == 2 ? S.rec_aux : S.rec_aux2;
-    uint32_t *__restrict__ key_out = S.rec_id2;  // level 2 only
+    uint8_t *__restrict__ key_out = S.rec_id2;  // level 2 only
     TailPair<OffT> *__restrict__ rec_out = static_cast<TailPair<OffT> *>(kLevel == 2 ? S.rec2 : S.ie);
     uint32_t *__restrict__ aux_out = kLevel == 2 ? S.rec_aux2 : S.ie_aux;
     const OffT *__restrict__ rp_in = static_cast<const OffT *>(S.rp_in);
@@ -619,7 +627,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_level_scatter(const Split
                 aux[u] = 0u;
                 if (key[u] != kSentinel) {
                     t[u] = rec_in[k];
-                    if (key[u] >= rows) aux[u] = aux_in[k];
+                    if (b * L::kUnit + key[u] >= rows) aux[u] = aux_in[k];  // a list record
                 }
             }
 #pragma unroll
@@ -662,11 +670,11 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_level_scatter(const Split
             for (uint32_t u = 0; u < kPerThread; u++) {
                 const uint32_t i = threadIdx.x + u * kSplitBlock;
                 if (i >= nt) continue;
-                const uint32_t k = st_key[i], q = bucket(k);
+                const uint32_t k = st_key[i], q = bucket(k);  // the key within the input bucket
                 const uint64_t p = s_base[q] + (i - s_toff[q]);
-                if (kLevel == 2) key_out[p] = k;
+                if (kLevel == 2) key_out[p] = (uint8_t)(k & (kSplitSub - 1));
                 rec_out[p] = st_rec[i];
-                if (k >= rows) aux_out[p] = st_aux[i];
+                if (b * L::kUnit + k >= rows) aux_out[p] = st_aux[i];
             }
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < kF; i += kSplitBlock) s_base[i] += s_tcnt[i];
