@@ -441,7 +441,7 @@ def tune_sweep(args):
         alens = [0] + [x for x in (2048, 8192, 32768, 131072, 262144) if x <= n32 and W > 0]
         # bitmap rows past 2^18 only for the split count (one row in LDS, no a*a/8 bytes in HBM)
         split_alens = [x for x in alens if x] + [x for x in (524288, 1048576) if x <= n32 and W > 0]
-        best = None
+        best = best_walk = None
         base = wtc.Tuned(0, 0, wtc.KERNEL_WALK, 4, 4, 8)
         for kernel in ((wtc.KERNEL_SPLIT,) if big else (wtc.KERNEL_WALK, wtc.KERNEL_SPLIT)):
             for a in (split_alens if kernel == wtc.KERNEL_SPLIT else alens):
@@ -458,17 +458,37 @@ def tune_sweep(args):
                         ms, _ = _time_count(dg, sp, a, reps=2 if big else 3)
                         if best is None or ms < best[0]:
                             best = (ms, t)
+                        if kernel == wtc.KERNEL_WALK and (best_walk is None or ms < best_walk[0]):
+                            best_walk = (ms, t)
         ms0, t0 = best
         log(f"tune {workload}: stage 1 best {ms0:.3f} ms {t0.as_dict()}")
-        split = t0.kernel == wtc.KERNEL_SPLIT  # its kernels take only the row kernel's block shape
-        for waves in ((4,) if big or split else (2, 4, 8)):
-            for ways in ((4,) if big or split else (2, 4, 8)):
-                for sw in ((8, 16) if big else (4, 8, 16) if split else (8,)):
-                    t = wtc.Tuned(t0.spread, t0.adjacency_matrix_len, t0.kernel, waves, ways, sw)
+        split = t0.kernel == wtc.KERNEL_SPLIT
+        # stage 2: the block shape of the winner; a split row also keeps the
+        # walk's best spread and block shape, which the walk takes where the
+        # split count does not apply (a = 0, a set of graphs)
+        walk0 = (best_walk[1] if best_walk else None) if split else t0
+        walk_best = None
+        if walk0 is not None and not big:  # (graphs built on the GPU: the split count only)
+            for waves in (2, 4, 8):
+                for ways in (2, 4, 8):
+                    t = wtc.Tuned(walk0.spread, walk0.adjacency_matrix_len, wtc.KERNEL_WALK, waves, ways, 8)
                     ctx.use_tuned(t)
                     ms, _ = _time_count(dg, t.spread, t.adjacency_matrix_len)
-                    if ms < best[0]:
-                        best = (ms, t)
+                    if walk_best is None or ms < walk_best[0]:
+                        walk_best = (ms, t)
+        if split:
+            for sw in ((8, 16) if big else (4, 8, 16)):
+                t = wtc.Tuned(t0.spread, t0.adjacency_matrix_len, t0.kernel, 4, 4, sw)
+                ctx.use_tuned(t)
+                ms, _ = _time_count(dg, t.spread, t.adjacency_matrix_len)
+                if ms < best[0]:
+                    best = (ms, t)
+            if walk_best:  # the walk's spread and block shape for the fallback
+                w = walk_best[1]
+                best = (best[0], wtc.Tuned(w.spread, best[1].adjacency_matrix_len, wtc.KERNEL_SPLIT, w.waves_per_block,
+                                           w.probe_ways, best[1].split_waves))
+        elif walk_best and walk_best[0] < best[0]:
+            best = walk_best
         dg.free()
         ms, t = best
         a = t.adjacency_matrix_len

@@ -377,7 +377,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->ways = t.ways;
     plan->wpb = t.wpb;
     plan->kernel = kernel;
-    plan->work = count_work(t.grab, t.deal);
+    plan->work = count_work(t.grab, t.deal, max_spread);
     plan->chunk = t.chunk;
     plan->desc_bytes = desc_bytes;
     plan->small_count = (uint32_t)small_parts;
