@@ -137,10 +137,12 @@ def batch_bench(ctx, k, base_seed):
     """Batched vs plain graphs/s on the same host-resident oriented CSRs.
     plain: one reference-style tc_GPU call per graph (alloc, copy, count, copy
     back, free); batched: one upload of all graphs and one persistent counting
-    launch.  Both use the autotuner's spread and bitmap length per graph."""
+    launch.  Both use the autotuner's spread and bitmap length per graph;
+    the batched counts take the table's batch row's kernel and block shape."""
     t0 = time.perf_counter()
     graphs = batch_graphs(k, base_seed)
     tgen = time.perf_counter() - t0
+    ctx.use_tuned(wtc.autotune_shape(graphs[0].n, graphs[0].m, graphs[0].wedges(), batch=True))
     tune = [wtc.autotune(g.n, g.m, g.wedges(), batch=True) for g in graphs]
     spread = [s for s, _ in tune]
     alen = [a for _, a in tune]
@@ -443,13 +445,16 @@ def tune_sweep(args):
         split_alens = [x for x in alens if x] + [x for x in (524288, 1048576) if x <= n32 and W > 0]
         best = best_walk = None
         base = wtc.Tuned(0, 0, wtc.KERNEL_WALK, 4, 4, 8)
-        for kernel in ((wtc.KERNEL_SPLIT,) if big else (wtc.KERNEL_WALK, wtc.KERNEL_SPLIT)):
+        kernels = (wtc.KERNEL_SPLIT,) if big else (wtc.KERNEL_WALK, wtc.KERNEL_STAGED, wtc.KERNEL_SPLIT)
+        spreads = {wtc.KERNEL_WALK: (2, 4, 8, 16, 32, 64, 128), wtc.KERNEL_STAGED: (2, 4, 8, 16, 32),
+                   wtc.KERNEL_SPLIT: (12,)}
+        for kernel in kernels:
             for a in (split_alens if kernel == wtc.KERNEL_SPLIT else alens):
                 if kernel == wtc.KERNEL_SPLIT and (a == 0 or (big and a < 131072)):
                     continue
                 # (the split count has no spread: one value stands in its row of the table)
                 log(f"tune {workload}: kernel {kernel} a {a}")
-                for sp in ((2, 4, 8, 16, 32, 64, 128) if kernel == wtc.KERNEL_WALK else (12,)):
+                for sp in spreads[kernel]:
                     if sp > 8 and W < (1 << 20):
                         continue
                     for sw0 in ((8, 16) if kernel == wtc.KERNEL_SPLIT and a > 262144 else (base.split_waves,)):
@@ -471,7 +476,7 @@ def tune_sweep(args):
         if walk0 is not None and not big:  # (graphs built on the GPU: the split count only)
             for waves in (2, 4, 8):
                 for ways in (2, 4, 8):
-                    t = wtc.Tuned(walk0.spread, walk0.adjacency_matrix_len, wtc.KERNEL_WALK, waves, ways, 8)
+                    t = wtc.Tuned(walk0.spread, walk0.adjacency_matrix_len, walk0.kernel, waves, ways, 8)
                     ctx.use_tuned(t)
                     ms, _ = _time_count(dg, t.spread, t.adjacency_matrix_len)
                     if walk_best is None or ms < walk_best[0]:
@@ -510,23 +515,25 @@ def tune_sweep(args):
         graphs = WL.batch_graphs(args.tune_batch)
         b = wtc.Batch(ctx, graphs)
         best = None
-        for sp in (4, 8, 12, 16):
-            for waves in (2, 4, 8):
-                for ways in (2, 4, 8):
-                    ctx.use_tuned(wtc.Tuned(sp, 0, wtc.KERNEL_WALK, waves, ways, 8))
-                    spreads, alens = [sp] * len(graphs), [0] * len(graphs)
-                    b.count(spreads, alens)
-                    ts = []
-                    for _ in range(3):
-                        t0 = time.perf_counter()
+        for kernel in (wtc.KERNEL_WALK, wtc.KERNEL_STAGED):
+            for sp in (4, 8, 12, 16):
+                for waves in (2, 4, 8):
+                    for ways in (2, 4, 8):
+                        ctx.use_tuned(wtc.Tuned(sp, 0, kernel, waves, ways, 8))
+                        spreads, alens = [sp] * len(graphs), [0] * len(graphs)
                         b.count(spreads, alens)
-                        ts.append((time.perf_counter() - t0) * 1e3)
-                    ms = statistics.median(ts)
-                    if best is None or ms < best[0]:
-                        best = (ms, sp, waves, ways)
+                        ts = []
+                        for _ in range(3):
+                            t0 = time.perf_counter()
+                            b.count(spreads, alens)
+                            ts.append((time.perf_counter() - t0) * 1e3)
+                        ms = statistics.median(ts)
+                        if best is None or ms < best[0]:
+                            best = (ms, sp, waves, ways, kernel)
         b.free()
-        ms, sp, waves, ways = best
-        rows.append(("batch", "any", 0, sp, 0, 0, "walk", waves, ways, 8, ms, f"batch{args.tune_batch}"))
+        ms, sp, waves, ways, kernel = best
+        kname = "staged" if kernel == wtc.KERNEL_STAGED else "walk"
+        rows.append(("batch", "any", 0, sp, 0, 0, kname, waves, ways, 8, ms, f"batch{args.tune_batch}"))
         log(f"tune batch: {rows[-1]}")
     ctx.close()
     # buckets below the smallest measured graph of a family take its row

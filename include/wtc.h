@@ -248,10 +248,13 @@ int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int wav
 int wtc_ctx_set_count_kernel(wtc_ctx *ctx, int kernel);
 /* How the counting kernels hand out work: the wedge order is cut into tiles
  * of 64*s wedges, and a wave takes up to tiles_per_grab consecutive tiles
- * per atomic grab (1..65535; 0 = 32).  deal_to_xcds != 0 deals super-chunks
- * of grabs to the 8 XCDs in turn, each XCD taking from its own counter;
- * 0 gives one stream of grabs for the whole grid.  Default 4 tiles per grab,
- * one stream.  Counts do not depend on these. */
+ * per atomic grab (1..65535; 0 = the kernel's default: 4 for the walk, 32
+ * for the staged kernel).  deal_to_xcds != 0 deals super-chunks of grabs to
+ * the 8 XCDs in turn, each XCD taking from its own counter; 0 gives one
+ * stream of grabs for the whole grid.  Default 0 (the kernel's), one stream.
+ * A grab covers at least 2048 wedges whatever tiles_per_grab says, and a
+ * launch with few tiles takes smaller grabs (about two per wave).  Counts do
+ * not depend on these. */
 int wtc_ctx_set_work_split(wtc_ctx *ctx, uint32_t tiles_per_grab, int deal_to_xcds);
 /* Kernel choice for sets of small graphs (the role of the vertex- and
  * edge-parallel kernels, experiments/tc_vertex.cu:113-274 and

@@ -377,7 +377,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->ways = t.ways;
     plan->wpb = t.wpb;
     plan->kernel = kernel;
-    plan->work = count_work(t.grab, t.deal, max_spread);
+    plan->work = count_work(t.grab_for(kernel), t.deal, max_spread);
     plan->chunk = t.chunk;
     plan->desc_bytes = desc_bytes;
     plan->small_count = (uint32_t)small_parts;
@@ -1124,7 +1124,7 @@ int batch_capture(wtc_ctx *c, wtc_batch *b, const std::vector<uint32_t> &s, cons
     b->key_ways = c->tune.ways;
     b->key_wpb = c->tune.wpb;
     b->key_kernel = c->tune.kernel;
-    b->key_work = count_work(c->tune.grab, c->tune.deal);
+    b->key_work = count_work(c->tune.grab_for(c->tune.kernel), c->tune.deal);
     b->key_chunk = c->tune.chunk;
     b->key_small = c->tune.small_words;
     return WTC_OK;
@@ -1257,7 +1257,7 @@ int wtc_batch_count(wtc_ctx *c, wtc_batch *b, const uint32_t *spread, const uint
     int rc = batch_params(b, spread, a, s, aa);
     if (rc) return rc;
     if (!b->exec || s != b->key_s || aa != b->key_a || c->tune.ways != b->key_ways || c->tune.wpb != b->key_wpb || c->tune.kernel != b->key_kernel ||
-        count_work(c->tune.grab, c->tune.deal) != b->key_work ||
+        count_work(c->tune.grab_for(c->tune.kernel), c->tune.deal) != b->key_work ||
         c->tune.chunk != b->key_chunk || c->tune.small_words != b->key_small) {
         if ((rc = batch_capture(c, b, s, aa))) return rc;
     }

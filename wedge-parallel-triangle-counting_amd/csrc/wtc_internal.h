@@ -111,10 +111,9 @@ struct Tuning {
     // the split count only with a workspace the caller provides (the
     // server carves it from its HBM pool), never from the slot
     bool split_needs_base = false;
-    // work split of the counting kernels: at most 4 tiles per grab, one
-    // stream of grabs (measured best on R-MAT-24, where the cost per wedge
-    // varies along the order and bigger grabs leave the slowest waves last)
-    uint32_t grab = 4;    // most tiles per grab (0: the kernel's default, 32)
+    // work split of the counting kernels: most tiles per grab (0: the
+    // kernel's own default, grab_for), one stream of grabs
+    uint32_t grab = 0;
     bool deal = false;    // deal super-chunks of grabs to the XCDs in turn
     uint32_t chunk = 16;  // wedges per lane staged in LDS per phase
     // In sets of two or more graphs, a graph with 2(n+1) + m <= small_words
@@ -126,6 +125,12 @@ struct Tuning {
     // LDS per block in bytes: the split count's row bitmap (a bits) must fit
     size_t lds_bytes = 0;
     uint32_t grid_cache[2][2][4][4][kChunkMax + 2] = {};  // by kernel, offset width, ways, waves, staged
+    // Most tiles per grab for a counting kernel.  The walk's default is 4:
+    // on R-MAT-24 the cost per wedge varies along the order and bigger grabs
+    // left the slowest waves last.  The staged kernel's is 32: it searches
+    // the first tile of each grab from the tree, and with grabs of 4 that
+    // search doubled its time on R-MAT-20 (7.5 vs 3.8 ms at s = 8).
+    uint32_t grab_for(int k) const { return grab ? grab : k == kKernelStaged ? 32u : 4u; }
 };
 
 // Default small-graph limit of new contexts and servers, in LDS words

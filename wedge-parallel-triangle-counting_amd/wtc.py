@@ -492,9 +492,10 @@ class Context:
         """Applies an autotuned shape (kernel, block waves, probe ways, row-kernel waves)."""
         _check(_lib.wtc_ctx_use_tuned(self._p, C.byref(t)), "use_tuned")
 
-    def set_work_split(self, tiles_per_grab: int = 4, deal_to_xcds: bool = False):
-        """Most tiles per grab (0: 32) and XCD dealing of the counting kernels
-        (default: 4 tiles, one stream of grabs)."""
+    def set_work_split(self, tiles_per_grab: int = 0, deal_to_xcds: bool = False):
+        """Most tiles per grab (0: the kernel's default, 4 for the walk and 32
+        for the staged kernel) and XCD dealing of the counting kernels
+        (default: the kernel's grab, one stream of grabs)."""
         _check(_lib.wtc_ctx_set_work_split(self._p, tiles_per_grab, int(deal_to_xcds)), "set_work_split")
 
     def set_small_graphs(self, max_words: int):
