@@ -157,7 +157,7 @@ struct SplitDesc {
     void *rec;            // cap pairs of OffT: A pairs, then ie
     uint32_t *rec_aux;    // cap: A slab_aux words (list records), then ie_aux
     uint8_t *rec_id2;     // cap: B keys within their sub-part (key mod kSplitSub)
-    void *rec2;           // cap pairs of OffT: B pairs
+    void *rec2;           // cap pairs of OffT: B pairs, then the packed top ids (three 21-bit ids per word)
     uint32_t *rec_aux2;   // cap: B slab_aux words
     void *rp_in;          // keys+1 OffT: first in-edge of each key
     void *ie;             // = rec: the records' pairs grouped by key

@@ -9,15 +9,15 @@
 //  * w among the top-a ranks (w >= thr): the wedges whose probe the
 //    reference answers from its top-a bitmap (tc.cu:262-272).  One key per
 //    row.  A block of the row kernel holds adj+(w) as a bitmap of a bits in
-//    LDS, and its waves stream the tails of w's in-edges through it, four
-//    ids per lane per 16-byte load.  Each probe is one LDS read, and the
-//    top-a bitmap never exists in HBM: one row of it at a time in LDS instead
-//    of a*a/8 bytes.
+//    LDS, and its waves stream the tails of w's in-edges through it from a
+//    packed copy of colInd (ids above thr as 21-bit offsets, six per 16-byte
+//    load).  Each probe is one LDS read, and the top-a bitmap never exists
+//    in HBM: one row of it at a time in LDS instead of a*a/8 bytes.
 //  * w below thr: the sorted-list probes of tc.cu:273-286.  One key per slab
 //    of colInd (kSlabSpan positions; a row belongs to the slab its adjacency
-//    starts in).  A block of the slab kernel copies the slab's window of
-//    colInd into LDS once, and its waves stream the tails of the slab's
-//    in-edges, searching each id in adj+(w) inside that window.
+//    starts in).  A block of the slab kernel hashes the rows of the slab its
+//    in-edges name into LDS, and its waves stream the tails of the slab's
+//    in-edges, looking each id up in adj+(w)'s hash set.
 //
 // A wedge range [w_begin, w_end) of the full order (the reference's wedge
 // numbering, tc.cu:203-212) holds the in-edges of the vertices between its
@@ -32,7 +32,8 @@
 // part's records bucketed by sub-part of kSplitSub keys),
 // k_split_level_count<3>, k_split_levelcols<3>, k_split_keysum,
 // k_scan_partials, k_split_keyscan, k_split_level_scatter<3> (each
-// sub-part's records grouped by key);
+// sub-part's records grouped by key), k_split_pack (colInd as 21-bit ids
+// above thr, over the second level's records);
 // k_split_rows (top rows); k_split_slabs (list rows).
 #include <type_traits>
 
@@ -46,6 +47,8 @@ namespace {
 
 // Four ids loaded by one 16-byte load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+// Two words of packed top ids (six ids) loaded by one 16-byte load.
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 template <typename OffT>
 struct TailPair {
@@ -816,21 +819,103 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_keyscan(const SplitDesc *
 
 // ------------------------------------------------------------- top-a part
 
-// Lanes per in-edge tail in the row kernel: a wave takes kWave / kSplitLanes
-// in-edges per step.  Tails average ~180 ids on R-MAT-24; with a whole wave
-// per tail most lanes idled (37.4 ms vs 25.1 ms at 16 lanes, 28.1 at 32,
-// 24.8 at 8).
-constexpr int kSplitLanes = 16;
+// The top rows' tails as packed ids.  A top in-edge v -> w (w >= thr) has a
+// tail of ids above w, so each of them lies in [thr, n) and u - thr < rows
+// <= 2^21 (the a-bit row must fit a block's LDS).  Word i of the packed
+// array holds (colInd[3i + f] - thr) mod 2^21 in bits [21f, 21f + 21), f =
+// 0, 1, 2: the row kernel streams 8/3 bytes per id instead of 4.
+constexpr uint32_t kPackBits = 21;
+constexpr uint64_t kPackMask = (1ull << kPackBits) - 1;
+constexpr uint32_t kPackRowsMax = 1u << kPackBits;
+constexpr uint32_t kPackGrid = 4096;  // blocks of the packing kernel (grid-stride)
+
+// An id u >= thr packs as u - thr, an id below thr (never in a top tail) as
+// 0, so that every field is a bit of the row bitmap.
+__device__ __forceinline__ uint64_t pack1(uint32_t u, uint32_t thr) { return u >= thr ? (uint64_t)(u - thr) : 0ull; }
+__device__ __forceinline__ uint64_t pack3(uint32_t a, uint32_t b, uint32_t c, uint32_t thr) {
+    return pack1(a, thr) | (pack1(b, thr) << kPackBits) | (pack1(c, thr) << (2 * kPackBits));
+}
+
+// Packs colInd into S.rec2 (free once the third level has grouped its
+// records): a thread takes 12 positions (three 16-byte loads, two 16-byte
+// stores); the last group reads only positions below m.  Writes
+// ceil(m / 12) * 32 bytes, within rec2's cap * 2o bytes (and the 256-byte
+// rounding of every workspace piece).
+template <typename OffT>
+__global__ __launch_bounds__(kBlock) void k_split_pack(const GraphDesc *__restrict__ desc,
+                                                       const SplitDesc *__restrict__ sd) {
+    const GraphDesc &D = desc[0];
+    const SplitDesc &S = sd[0];
+    const uint64_t m = (uint64_t) static_cast<const OffT *>(D.rowPtr)[D.n];
+    const gptr<uint32_t> colInd = to_global(D.colInd);
+    u64x2 *__restrict__ dst = static_cast<u64x2 *>(S.rec2);
+    const uint32_t thr = D.thr;
+    const uint64_t groups = (m + 11) / 12;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < groups;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t u[12];
+        if (12 * t + 12 <= m) {
+            const gptr<u32x4> src = reinterpret_cast<gptr<u32x4>>(colInd + 12 * t);
+            const u32x4 a = src[0], b = src[1], c = src[2];
+#pragma unroll
+            for (int z = 0; z < 4; z++) {
+                u[z] = a[z];
+                u[4 + z] = b[z];
+                u[8 + z] = c[z];
+            }
+        } else {
+#pragma unroll
+            for (int z = 0; z < 12; z++) u[z] = 12 * t + z < m ? colInd[12 * t + z] : thr;
+        }
+        dst[2 * t] = u64x2{pack3(u[0], u[1], u[2], thr), pack3(u[3], u[4], u[5], thr)};
+        dst[2 * t + 1] = u64x2{pack3(u[6], u[7], u[8], thr), pack3(u[9], u[10], u[11], thr)};
+    }
+}
+
+// Lanes per in-edge tail in the row kernel (a wave takes kWave / kSplitLanes
+// in-edges per step) and 16-byte chunks per lane per step.  Tails average
+// ~180 ids on R-MAT-24 (median 86), and a step lasts as long as its longest
+// tail: 8 lanes of one chunk keep 80% of the id slots busy, 16 lanes of two
+// chunks 57%.  Measured on R-MAT-24: 18.2 ms at 8 x 1, 19.0 at 8 x 2, 18.8
+// at 16 x 1, 19.1 at 4 x 1, 22.1 at 16 x 2.
+constexpr int kSplitLanes = 8;
+constexpr int kSplitChunks = 1;
+
+// Counts the hits of the six packed ids of chunk c of a tail.  base = 6c -
+// lo (mod 2^32): position 6c + z is in the tail iff base + z < len.  Every
+// packed field is below rows, so all six ids are looked up (word addresses
+// and bit offsets straight from the fields: a field's bit offset is its low
+// 5 bits), the hits gathered as a 6-bit mask and the positions outside the
+// tail masked off at the end.
+__device__ __forceinline__ uint32_t rows_chunk_hits(const uint32_t *s_bits, u64x2 ch, uint32_t base,
+                                                    uint32_t len) {
+    const uint32_t w0 = (uint32_t)ch[0], w1 = (uint32_t)(ch[0] >> 32);
+    const uint32_t w2 = (uint32_t)ch[1], w3 = (uint32_t)(ch[1] >> 32);
+    // the six fields shifted so that bits [0, 21) hold the id
+    const uint32_t f[6] = {w0, __builtin_amdgcn_alignbit(w1, w0, kPackBits), w1 >> (2 * kPackBits - 32),
+                           w2, __builtin_amdgcn_alignbit(w3, w2, kPackBits), w3 >> (2 * kPackBits - 32)};
+    uint32_t wrd[6];
+#pragma unroll
+    for (int z = 0; z < 6; z++) wrd[z] = s_bits[__builtin_amdgcn_ubfe(f[z], 5u, kPackBits - 5u)];
+    uint32_t hm = 0;
+#pragma unroll
+    for (int z = 0; z < 6; z++) hm |= __builtin_amdgcn_ubfe(wrd[z], f[z], 1u) << z;
+    // valid z: [lc, hc) with lc = -base when base is negative (the tail's
+    // first chunk), hc = min(len - base, 6)
+    const uint32_t lc = base > 0xfffffff0u ? 0u - base : 0u;
+    const uint32_t rem = len - base, hc = rem < 6u ? rem : 6u;
+    const uint32_t vm = ((1u << hc) - 1u) & (0xffffffffu << lc);
+    return __builtin_popcount(hm & vm);
+}
 
 // Persistent blocks take tiles of kSplitTile consecutive in-edges (in row
 // order) from one counter.  Per row of a tile: adj+(w) set as bits (u - thr)
 // in LDS, the tile's in-edges of the row dealt to the waves (kLanes lanes
-// per in-edge), and each tail streamed through the bitmap by 16-byte loads,
-// two in flight per lane; then the row's bits cleared.  colInd must be
-// 16-byte aligned and readable 12 bytes past its last id (every buffer of
-// this library carries 256 bytes of slack).
-
-template <typename OffT, int kWaves, int kLanes>
+// per in-edge), and each tail streamed through the bitmap from the packed
+// ids (k_split_pack) by 16-byte chunks of six ids, kCh chunks per lane per
+// step and the next step's chunks loaded before this step's are counted;
+// then the row's bits cleared.
+template <typename OffT, int kWaves, int kLanes, int kCh>
 __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *__restrict__ desc,
                                                                const SplitDesc *__restrict__ sd,
                                                                uint64_t *__restrict__ meta,
@@ -841,6 +926,7 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
     const SplitDesc &S = sd[0];
     const cptr<OffT> rowPtrU = to_const(static_cast<const OffT *>(D.rowPtr));
     const gptr<uint32_t> colInd = to_global(D.colInd);
+    const gptr<u64x2> packed = to_global(static_cast<const u64x2 *>(S.rec2));
     const cptr<OffT> rp_in = to_const(static_cast<const OffT *>(S.rp_in));
     const cptr<OffT> ieU = to_const(static_cast<const OffT *>(S.ie));
     const gptr<OffT> ieG = to_global(static_cast<const OffT *>(S.ie));
@@ -884,20 +970,31 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
                     hi = ieG[2 * (x0 + sub) + 1];
                 }
                 if (sl == 0) checked += (uint64_t)(hi - lo);
-                for (OffT q = (lo & ~(OffT)3) + 4 * sl; q < hi; q += 8 * kLanes) {
-                    const OffT q1 = q + 4 * kLanes;
-                    const u32x4 a = *reinterpret_cast<gptr<u32x4>>(colInd + q);
-                    u32x4 b = {0u, 0u, 0u, 0u};
-                    if (q1 < hi) b = *reinterpret_cast<gptr<u32x4>>(colInd + q1);
-                    const uint32_t ua[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+                // chunk c holds positions [6c, 6c + 6); a tail is shorter
+                // than 2^32 positions
+                const uint32_t len = (uint32_t)(hi - lo);
+                const OffT c_end = hi / 6 + (hi % 6 != 0 ? 1 : 0);
+                OffT c = lo / 6 + sl;
+                u64x2 cur[kCh];
 #pragma unroll
-                    for (int z = 0; z < 8; z++) {
-                        const OffT t = (z < 4 ? q : q1) + (OffT)(z & 3);
-                        if (t >= lo && t < hi) {
-                            const uint32_t c = ua[z] - thr;
-                            h += (s_bits[c >> 5] >> (c & 31u)) & 1u;
-                        }
+                for (int q = 0; q < kCh; q++) {
+                    cur[q] = u64x2{0ull, 0ull};
+                    if (c + (OffT)(q * kLanes) < c_end) cur[q] = packed[c + (OffT)(q * kLanes)];
+                }
+                while (c < c_end) {
+                    const OffT cn = c + (OffT)(kCh * kLanes);
+                    u64x2 nxt[kCh];
+#pragma unroll
+                    for (int q = 0; q < kCh; q++) {
+                        nxt[q] = u64x2{0ull, 0ull};
+                        if (cn + (OffT)(q * kLanes) < c_end) nxt[q] = packed[cn + (OffT)(q * kLanes)];
                     }
+                    const uint32_t base = (uint32_t)(6 * c - lo);
+#pragma unroll
+                    for (int q = 0; q < kCh; q++) h += rows_chunk_hits(s_bits, cur[q], base + 6u * q * kLanes, len);
+#pragma unroll
+                    for (int q = 0; q < kCh; q++) cur[q] = nxt[q];
+                    c = cn;
                 }
             }
             hits += h;
@@ -1143,22 +1240,23 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
     if ((e = hipGetLastError()) != hipSuccess) return e;
     WTC_SPLIT_LAUNCH(k_split_level_scatter, g3, blk, 3);
 #undef WTC_SPLIT_LAUNCH
-    return hipSuccess;
+    // the packed top ids, over B's pairs (the third level has read them)
+    return WTC_LAUNCH_OFF(ob, k_split_pack, dim3(kPackGrid), dim3(kBlock), 0, st, desc, sd);
 }
 
 size_t split_rows_lds(uint32_t rows) { return (size_t)((rows + 31) / 32) * sizeof(uint32_t); }
 
-template <int W>
+template <int W, int L, int C>
 static const void *split_rows_fn(int ob) {
-    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, kSplitLanes>)
-                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, kSplitLanes>);
+    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, L, C>)
+                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, L, C>);
 }
 
 static const void *split_rows_kernel(int ob, int wpb) {
     switch (wpb) {
-        case 4: return split_rows_fn<4>(ob);
-        case 16: return split_rows_fn<16>(ob);
-        default: return split_rows_fn<8>(ob);
+        case 4: return split_rows_fn<4, kSplitLanes, kSplitChunks>(ob);
+        case 16: return split_rows_fn<16, kSplitLanes, kSplitChunks>(ob);
+        default: return split_rows_fn<8, kSplitLanes, kSplitChunks>(ob);
     }
 }
 
@@ -1181,7 +1279,7 @@ hipError_t split_rows_grid(int ob, uint32_t rows, int wpb, int num_cus, uint32_t
 hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t rows, int wpb,
                              uint64_t *meta, unsigned long long *counts, uint32_t grid, hipStream_t st) {
     if (rows == 0) return hipSuccess;
-    if (!valid_split_waves(wpb) || grid == 0) return hipErrorInvalidValue;
+    if (!valid_split_waves(wpb) || grid == 0 || rows > kPackRowsMax) return hipErrorInvalidValue;
     const void *fn = split_rows_kernel(ob, wpb);
     void *args[] = {&desc, &sd, &meta, &counts};
     return hipLaunchKernel(fn, dim3(grid), dim3(wpb * kWave), args, split_rows_lds(rows), st);
