@@ -157,3 +157,21 @@ def test_split_long_list_rows_closed_form(split_ctx):
             lo_c, hi_c = min(lo, W), min(hi, W)
             assert dg.count(8, a, lo, hi) == max(0, hi_c - lo_c), (a, lo, hi)
     dg.free()
+
+
+def test_split_wide_packing_matches_oracle(split_ctx):
+    """Bitmap rows above 2^19 take the packed ids' wide layout (the id offset
+    itself instead of the LDS word address and bit): R-MAT-20 with a = 786,432
+    (thr > 0) and a = 2^20 (every vertex a top row), whole and ranges, against
+    the oracle; a = 2^19 (the narrow layout's limit) beside them."""
+    g = _rmat(20, seed=2)
+    assert g.n > (1 << 19)
+    expect = O.tc_merge(g)
+    W = g.wedges()
+    dg = split_ctx.upload(g)
+    for a in (1 << 19, 786432, 1 << 20):
+        cnt, st = dg.count(8, a, stats=True)
+        assert cnt == expect and st.split == 1, a
+        lo, hi = W // 3, W // 3 + W // 500
+        assert dg.count(8, a, lo, hi) == O.tc_wedge(g, 8, a, lo, hi), a
+    dg.free()

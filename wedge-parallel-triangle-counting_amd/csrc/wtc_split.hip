@@ -820,20 +820,27 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_keyscan(const SplitDesc *
 // ------------------------------------------------------------- top-a part
 
 // The top rows' tails as packed ids.  A top in-edge v -> w (w >= thr) has a
-// tail of ids above w, so each of them lies in [thr, n) and u - thr < rows
-// <= 2^21 (the a-bit row must fit a block's LDS).  Word i of the packed
-// array holds (colInd[3i + f] - thr) mod 2^21 in bits [21f, 21f + 21), f =
-// 0, 1, 2: the row kernel streams 8/3 bytes per id instead of 4.
+// tail of ids above w, so each of them lies in [thr, n) and its offset x =
+// u - thr is below rows <= 2^21 (the a-bit row must fit a block's LDS).
+// Word i of the packed array holds the fields of colInd[3i + f] in bits
+// [21f, 21f + 21), f = 0, 1, 2: the row kernel streams 8/3 bytes per id
+// instead of 4.  A field is x itself (the wide layout), or, when rows <=
+// 2^19, (x >> 5) << 7 | (x & 31) (the narrow layout: bits [5, 21) are the
+// LDS byte address of x's bitmap word and bits [0, 5) its bit, so that the
+// row kernel takes both with one bit-field extract).  An id below thr
+// (never in a top tail) packs as 0.
 constexpr uint32_t kPackBits = 21;
 constexpr uint64_t kPackMask = (1ull << kPackBits) - 1;
 constexpr uint32_t kPackRowsMax = 1u << kPackBits;
+constexpr uint32_t kPackNarrowRowsMax = 1u << 19;
 constexpr uint32_t kPackGrid = 4096;  // blocks of the packing kernel (grid-stride)
 
-// An id u >= thr packs as u - thr, an id below thr (never in a top tail) as
-// 0, so that every field is a bit of the row bitmap.
-__device__ __forceinline__ uint64_t pack1(uint32_t u, uint32_t thr) { return u >= thr ? (uint64_t)(u - thr) : 0ull; }
-__device__ __forceinline__ uint64_t pack3(uint32_t a, uint32_t b, uint32_t c, uint32_t thr) {
-    return pack1(a, thr) | (pack1(b, thr) << kPackBits) | (pack1(c, thr) << (2 * kPackBits));
+__device__ __forceinline__ uint64_t pack1(uint32_t u, uint32_t thr, bool narrow) {
+    const uint32_t x = u >= thr ? u - thr : 0u;
+    return narrow ? (uint64_t)(((x >> 5) << 7) | (x & 31u)) : (uint64_t)x;
+}
+__device__ __forceinline__ uint64_t pack3(uint32_t a, uint32_t b, uint32_t c, uint32_t thr, bool narrow) {
+    return pack1(a, thr, narrow) | (pack1(b, thr, narrow) << kPackBits) | (pack1(c, thr, narrow) << (2 * kPackBits));
 }
 
 // Packs colInd into S.rec2 (free once the third level has grouped its
@@ -850,6 +857,7 @@ __global__ __launch_bounds__(kBlock) void k_split_pack(const GraphDesc *__restri
     const gptr<uint32_t> colInd = to_global(D.colInd);
     u64x2 *__restrict__ dst = static_cast<u64x2 *>(S.rec2);
     const uint32_t thr = D.thr;
+    const bool narrow = S.rows <= kPackNarrowRowsMax;
     const uint64_t groups = (m + 11) / 12;
     for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < groups;
          t += (uint64_t)gridDim.x * blockDim.x) {
@@ -867,8 +875,8 @@ __global__ __launch_bounds__(kBlock) void k_split_pack(const GraphDesc *__restri
 #pragma unroll
             for (int z = 0; z < 12; z++) u[z] = 12 * t + z < m ? colInd[12 * t + z] : thr;
         }
-        dst[2 * t] = u64x2{pack3(u[0], u[1], u[2], thr), pack3(u[3], u[4], u[5], thr)};
-        dst[2 * t + 1] = u64x2{pack3(u[6], u[7], u[8], thr), pack3(u[9], u[10], u[11], thr)};
+        dst[2 * t] = u64x2{pack3(u[0], u[1], u[2], thr, narrow), pack3(u[3], u[4], u[5], thr, narrow)};
+        dst[2 * t + 1] = u64x2{pack3(u[6], u[7], u[8], thr, narrow), pack3(u[9], u[10], u[11], thr, narrow)};
     }
 }
 
@@ -883,20 +891,29 @@ constexpr int kSplitChunks = 1;
 
 // Counts the hits of the six packed ids of chunk c of a tail.  base = 6c -
 // lo (mod 2^32): position 6c + z is in the tail iff base + z < len.  Every
-// packed field is below rows, so all six ids are looked up (word addresses
-// and bit offsets straight from the fields: a field's bit offset is its low
-// 5 bits), the hits gathered as a 6-bit mask and the positions outside the
-// tail masked off at the end.
+// packed field names a bit of the row bitmap, so all six are looked up (word
+// and bit straight from the fields: a field's bit offset is its low 5 bits),
+// the hits gathered as a 6-bit mask and the positions outside the tail
+// masked off at the end.  s_bits sits at LDS address 0, so a narrow field's
+// bits [5, 21) are the word's address.
+template <bool kNarrow>
 __device__ __forceinline__ uint32_t rows_chunk_hits(const uint32_t *s_bits, u64x2 ch, uint32_t base,
                                                     uint32_t len) {
     const uint32_t w0 = (uint32_t)ch[0], w1 = (uint32_t)(ch[0] >> 32);
     const uint32_t w2 = (uint32_t)ch[1], w3 = (uint32_t)(ch[1] >> 32);
-    // the six fields shifted so that bits [0, 21) hold the id
+    // the six fields shifted so that bits [0, 21) hold the field
     const uint32_t f[6] = {w0, __builtin_amdgcn_alignbit(w1, w0, kPackBits), w1 >> (2 * kPackBits - 32),
                            w2, __builtin_amdgcn_alignbit(w3, w2, kPackBits), w3 >> (2 * kPackBits - 32)};
     uint32_t wrd[6];
 #pragma unroll
-    for (int z = 0; z < 6; z++) wrd[z] = s_bits[__builtin_amdgcn_ubfe(f[z], 5u, kPackBits - 5u)];
+    for (int z = 0; z < 6; z++) {
+        if (kNarrow) {
+            const uint32_t addr = __builtin_amdgcn_ubfe(f[z], 5u, kPackBits - 5u);
+            wrd[z] = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(s_bits) + addr);
+        } else {
+            wrd[z] = s_bits[__builtin_amdgcn_ubfe(f[z], 5u, kPackBits - 5u)];
+        }
+    }
     uint32_t hm = 0;
 #pragma unroll
     for (int z = 0; z < 6; z++) hm |= __builtin_amdgcn_ubfe(wrd[z], f[z], 1u) << z;
@@ -915,13 +932,13 @@ __device__ __forceinline__ uint32_t rows_chunk_hits(const uint32_t *s_bits, u64x
 // ids (k_split_pack) by 16-byte chunks of six ids, kCh chunks per lane per
 // step and the next step's chunks loaded before this step's are counted;
 // then the row's bits cleared.
-template <typename OffT, int kWaves, int kLanes, int kCh>
+template <typename OffT, int kWaves, int kLanes, int kCh, bool kNarrow>
 __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *__restrict__ desc,
                                                                const SplitDesc *__restrict__ sd,
                                                                uint64_t *__restrict__ meta,
                                                                unsigned long long *__restrict__ counts) {
+    // dynamic LDS only: the row bitmap at address 0, the tile slot after it
     extern __shared__ uint32_t s_bits[];
-    __shared__ uint64_t s_tile;
     const GraphDesc &D = desc[0];
     const SplitDesc &S = sd[0];
     const cptr<OffT> rowPtrU = to_const(static_cast<const OffT *>(D.rowPtr));
@@ -933,6 +950,7 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
     const uint32_t thr = D.thr, rows = S.rows;
     constexpr uint32_t kPer = kWave / kLanes;  // in-edges a wave takes per step
     const uint32_t words = (rows + 31) >> 5;
+    uint64_t &s_tile = *reinterpret_cast<uint64_t *>(s_bits + ((words + 1) & ~1u));
     for (uint32_t i = threadIdx.x; i < words; i += kWave * kWaves) s_bits[i] = 0;
     const uint64_t K = (uint64_t)rp_in[rows];
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = uni32(threadIdx.x >> 6);
@@ -983,15 +1001,19 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
                 }
                 while (c < c_end) {
                     const OffT cn = c + (OffT)(kCh * kLanes);
+                    // the next chunks, or chunk c again past the tail's end
+                    // (loaded unconditionally; a chunk past c_end is not counted)
                     u64x2 nxt[kCh];
 #pragma unroll
                     for (int q = 0; q < kCh; q++) {
-                        nxt[q] = u64x2{0ull, 0ull};
-                        if (cn + (OffT)(q * kLanes) < c_end) nxt[q] = packed[cn + (OffT)(q * kLanes)];
+                        const OffT cq = cn + (OffT)(q * kLanes);
+                        nxt[q] = packed[cq < c_end ? cq : c];
                     }
                     const uint32_t base = (uint32_t)(6 * c - lo);
 #pragma unroll
-                    for (int q = 0; q < kCh; q++) h += rows_chunk_hits(s_bits, cur[q], base + 6u * q * kLanes, len);
+                    for (int q = 0; q < kCh; q++)
+                        if (q == 0 || c + (OffT)(q * kLanes) < c_end)
+                            h += rows_chunk_hits<kNarrow>(s_bits, cur[q], base + 6u * q * kLanes, len);
 #pragma unroll
                     for (int q = 0; q < kCh; q++) cur[q] = nxt[q];
                     c = cn;
@@ -1244,27 +1266,33 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
     return WTC_LAUNCH_OFF(ob, k_split_pack, dim3(kPackGrid), dim3(kBlock), 0, st, desc, sd);
 }
 
-size_t split_rows_lds(uint32_t rows) { return (size_t)((rows + 31) / 32) * sizeof(uint32_t); }
+size_t split_rows_lds(uint32_t rows) { return (size_t)(((rows + 31) / 32 + 1) & ~1u) * sizeof(uint32_t) + sizeof(uint64_t); }
 
-template <int W, int L, int C>
+template <int W, bool N>
 static const void *split_rows_fn(int ob) {
-    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, L, C>)
-                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, L, C>);
+    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, kSplitLanes, kSplitChunks, N>)
+                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, kSplitLanes, kSplitChunks, N>);
 }
 
-static const void *split_rows_kernel(int ob, int wpb) {
+template <bool N>
+static const void *split_rows_kernel_n(int ob, int wpb) {
     switch (wpb) {
-        case 4: return split_rows_fn<4, kSplitLanes, kSplitChunks>(ob);
-        case 16: return split_rows_fn<16, kSplitLanes, kSplitChunks>(ob);
-        default: return split_rows_fn<8, kSplitLanes, kSplitChunks>(ob);
+        case 4: return split_rows_fn<4, N>(ob);
+        case 16: return split_rows_fn<16, N>(ob);
+        default: return split_rows_fn<8, N>(ob);
     }
+}
+
+// The row kernel for a graph of `rows` top rows (the packing's layout).
+static const void *split_rows_kernel(int ob, int wpb, uint32_t rows) {
+    return rows <= kPackNarrowRowsMax ? split_rows_kernel_n<true>(ob, wpb) : split_rows_kernel_n<false>(ob, wpb);
 }
 
 bool valid_split_waves(int wpb) { return wpb == 4 || wpb == 8 || wpb == 16; }
 
 hipError_t split_rows_grid(int ob, uint32_t rows, int wpb, int num_cus, uint32_t *grid) {
     if (!valid_split_waves(wpb)) return hipErrorInvalidValue;
-    const void *fn = split_rows_kernel(ob, wpb);
+    const void *fn = split_rows_kernel(ob, wpb, rows);
     const size_t shm = split_rows_lds(rows);
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
     if (e != hipSuccess) return e;
@@ -1280,7 +1308,7 @@ hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob,
                              uint64_t *meta, unsigned long long *counts, uint32_t grid, hipStream_t st) {
     if (rows == 0) return hipSuccess;
     if (!valid_split_waves(wpb) || grid == 0 || rows > kPackRowsMax) return hipErrorInvalidValue;
-    const void *fn = split_rows_kernel(ob, wpb);
+    const void *fn = split_rows_kernel(ob, wpb, rows);
     void *args[] = {&desc, &sd, &meta, &counts};
     return hipLaunchKernel(fn, dim3(grid), dim3(wpb * kWave), args, split_rows_lds(rows), st);
 }
