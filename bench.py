@@ -25,9 +25,11 @@ import wtc  # noqa: E402
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 
-# Bytes the split count's row kernel reads per tail id: three 21-bit ids per
-# 8-byte word of its packed copy of colInd.
-SPLIT_ID_BYTES = 8.0 / 3.0
+def split_id_bytes(rows):
+    """Bytes the split count's row kernel reads per tail id: its packed copy
+    of colInd holds P ids per 16-byte chunk, P = 8, 7 or 6 for graphs of at
+    most 2^16, 2^18 or 2^21 top rows (the library's packing layouts)."""
+    return 16.0 / (8 if rows <= 1 << 16 else 7 if rows <= 1 << 18 else 6)
 
 
 def log(msg):
@@ -734,7 +736,7 @@ def offsets_u64_leg(ctx, g, W, spread, a, steps, warmup, expect):
            "triangles": counts[0], "counts_equal_u32_offsets": all(c == expect for c in counts)}
     if stats[-1]["split"]:  # the row kernel's roofline, as for the headline (2*8 B per in-edge record)
         rows_ms = statistics.median(s["rows_ms"] for s in stats)
-        dom_bytes = SPLIT_ID_BYTES * stats[-1]["top_wedges"] + 16.0 * stats[-1]["in_edges"]
+        dom_bytes = split_id_bytes(min(a, g.n)) * stats[-1]["top_wedges"] + 16.0 * stats[-1]["in_edges"]
         out["kernel_ms"]["split_rows"] = rows_ms
         out["kernel_ms"]["split_slabs"] = statistics.median(s["list_ms"] for s in stats)
         out["roofline"] = {"kernel": "k_split_rows", "achieved": dom_bytes / (rows_ms / 1e3) / 1e9, "unit": "GB/s",
@@ -913,14 +915,15 @@ def main():
     split = bool(stats[-1]["split"])
     if split:
         # dominant kernel: the split count's row kernel, timed by its own HIP
-        # events on the count's stream.  Its algorithmic bytes: the packed
-        # 21-bit id of every wedge it checks (8/3 bytes: three ids per 8-byte
-        # word; each tail id read once; the probe itself is an LDS read) and
-        # every in-edge's (lo, hi) record of 2*ob bytes
+        # events on the count's stream.  Its algorithmic bytes: the packed id
+        # of every wedge it checks (16/P bytes; each tail id read once; the
+        # probe itself is an LDS read) and every in-edge's (lo, hi) record of
+        # 2*ob bytes
         dom, dom_ms = "k_split_rows", statistics.median(s["rows_ms"] for s in stats)
         top_w, in_e = stats[-1]["top_wedges"], stats[-1]["in_edges"]
-        dom_bytes = SPLIT_ID_BYTES * top_w + 2.0 * ob * in_e
-        dom_units = {"wedges_checked": top_w, "in_edges": in_e, "bytes_per_wedge": SPLIT_ID_BYTES,
+        id_bytes = split_id_bytes(min(args.a, dg.n))
+        dom_bytes = id_bytes * top_w + 2.0 * ob * in_e
+        dom_units = {"wedges_checked": top_w, "in_edges": in_e, "bytes_per_wedge": id_bytes,
                      "bytes_per_in_edge": 2 * ob}
     else:
         # dominant kernel: the counting kernel.  Algorithmic bytes: the CSR once

@@ -159,17 +159,19 @@ def test_split_long_list_rows_closed_form(split_ctx):
     dg.free()
 
 
-def test_split_wide_packing_matches_oracle(split_ctx):
-    """Bitmap rows above 2^19 take the packed ids' wide layout (the id offset
-    itself instead of the LDS word address and bit): R-MAT-20 with a = 786,432
-    (thr > 0) and a = 2^20 (every vertex a top row), whole and ranges, against
-    the oracle; a = 2^19 (the narrow layout's limit) beside them."""
+def test_split_packed_layouts_match_oracle(split_ctx):
+    """Every packing of the top tails' ids, at and around its row limit, on
+    R-MAT-20, whole and a range, against the oracle: eight 16-bit ids per
+    chunk (a = 2^16), seven 18-bit (a = 2^16 + 32, 2^18), six 21-bit fields
+    holding the LDS word address and bit (a = 2^18 + 32, 2^19) and six
+    21-bit ids (a = 786,432 with thr > 0, and a = 2^20 where every vertex is
+    a top row)."""
     g = _rmat(20, seed=2)
     assert g.n > (1 << 19)
     expect = O.tc_merge(g)
     W = g.wedges()
     dg = split_ctx.upload(g)
-    for a in (1 << 19, 786432, 1 << 20):
+    for a in (1 << 16, (1 << 16) + 32, 1 << 18, (1 << 18) + 32, 1 << 19, 786432, 1 << 20):
         cnt, st = dg.count(8, a, stats=True)
         assert cnt == expect and st.split == 1, a
         lo, hi = W // 3, W // 3 + W // 500

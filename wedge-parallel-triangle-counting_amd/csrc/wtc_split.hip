@@ -36,6 +36,7 @@
 // above thr, over the second level's records);
 // k_split_rows (top rows); k_split_slabs (list rows).
 #include <type_traits>
+#include <utility>
 
 #include "wtc_dev.h"
 #include "wtc_kernels.h"
@@ -47,8 +48,6 @@ namespace {
 
 // Four ids loaded by one 16-byte load.
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-// Two words of packed top ids (six ids) loaded by one 16-byte load.
-typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 template <typename OffT>
 struct TailPair {
@@ -821,62 +820,105 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_keyscan(const SplitDesc *
 
 // The top rows' tails as packed ids.  A top in-edge v -> w (w >= thr) has a
 // tail of ids above w, so each of them lies in [thr, n) and its offset x =
-// u - thr is below rows <= 2^21 (the a-bit row must fit a block's LDS).
-// Word i of the packed array holds the fields of colInd[3i + f] in bits
-// [21f, 21f + 21), f = 0, 1, 2: the row kernel streams 8/3 bytes per id
-// instead of 4.  A field is x itself (the wide layout), or, when rows <=
-// 2^19, (x >> 5) << 7 | (x & 31) (the narrow layout: bits [5, 21) are the
-// LDS byte address of x's bitmap word and bits [0, 5) its bit, so that the
-// row kernel takes both with one bit-field extract).  An id below thr
-// (never in a top tail) packs as 0.
-constexpr uint32_t kPackBits = 21;
-constexpr uint64_t kPackMask = (1ull << kPackBits) - 1;
-constexpr uint32_t kPackRowsMax = 1u << kPackBits;
-constexpr uint32_t kPackNarrowRowsMax = 1u << 19;
+// u - thr is below rows (the a-bit row must fit a block's LDS, so rows <=
+// 2^21).  The packed array is a sequence of 16-byte chunks; chunk c holds
+// the fields of colInd positions [Pc, Pc + P) in bits [Bi, Bi + B), i < P,
+// with the fewest bits B the graph's rows allow: the row kernel streams 16/P
+// bytes per tail id instead of 4.  Layouts (PackLayout<L>):
+//   0: P = 6, B = 21, rows <= 2^19, field (x >> 5) << 7 | (x & 31): bits
+//      [5, 21) are the LDS byte address of x's bitmap word and bits [0, 5)
+//      its bit, so that the row kernel takes both with one extract;
+//   1: P = 6, B = 21, rows <= 2^21, field x;
+//   2: P = 7, B = 18, rows <= 2^18, field x;
+//   3: P = 8, B = 16, rows <= 2^16, field x.
+// An id below thr (never in a top tail) packs as 0.
+template <int L>
+struct PackLayout;
+template <>
+struct PackLayout<0> {
+    static constexpr int P = 6, B = 21;
+    static constexpr bool kAddr = true;
+};
+template <>
+struct PackLayout<1> {
+    static constexpr int P = 6, B = 21;
+    static constexpr bool kAddr = false;
+};
+template <>
+struct PackLayout<2> {
+    static constexpr int P = 7, B = 18;
+    static constexpr bool kAddr = false;
+};
+template <>
+struct PackLayout<3> {
+    static constexpr int P = 8, B = 16;
+    static constexpr bool kAddr = false;
+};
+constexpr uint32_t kPackRowsMax = 1u << 21;
 constexpr uint32_t kPackGrid = 4096;  // blocks of the packing kernel (grid-stride)
 
-__device__ __forceinline__ uint64_t pack1(uint32_t u, uint32_t thr, bool narrow) {
-    const uint32_t x = u >= thr ? u - thr : 0u;
-    return narrow ? (uint64_t)(((x >> 5) << 7) | (x & 31u)) : (uint64_t)x;
+// The layout of a graph with `rows` top rows (rows <= kPackRowsMax).
+__host__ __device__ __forceinline__ int pack_layout(uint32_t rows) {
+    return rows <= (1u << 16) ? 3 : rows <= (1u << 18) ? 2 : rows <= (1u << 19) ? 0 : 1;
 }
-__device__ __forceinline__ uint64_t pack3(uint32_t a, uint32_t b, uint32_t c, uint32_t thr, bool narrow) {
-    return pack1(a, thr, narrow) | (pack1(b, thr, narrow) << kPackBits) | (pack1(c, thr, narrow) << (2 * kPackBits));
+
+// Field i of a chunk, shifted down so that its bits start at bit 0 (bits
+// above B hold the next field's; the readers extract what they need).
+template <int B, int i>
+__device__ __forceinline__ uint32_t chunk_field(u32x4 d) {
+    constexpr int bit = B * i, k = bit / 32, sh = bit % 32;
+    if constexpr (sh + B <= 32) {
+        return d[k] >> sh;
+    } else {
+        return __builtin_amdgcn_alignbit(d[k + 1], d[k], sh);
+    }
+}
+
+template <int B, int... I>
+__device__ __forceinline__ void chunk_fields(u32x4 d, uint32_t *f, std::integer_sequence<int, I...>) {
+    ((f[I] = chunk_field<B, I>(d)), ...);
 }
 
 // Packs colInd into S.rec2 (free once the third level has grouped its
-// records): a thread takes 12 positions (three 16-byte loads, two 16-byte
-// stores); the last group reads only positions below m.  Writes
-// ceil(m / 12) * 32 bytes, within rec2's cap * 2o bytes (and the 256-byte
-// rounding of every workspace piece).
+// records): a thread per chunk, P 4-byte loads and one 16-byte store;
+// positions past m pack as 0.  Writes ceil(m / P) * 16 bytes, within rec2's
+// cap * 2o bytes (and the 256-byte rounding of every workspace piece).
+template <int L>
+__device__ __forceinline__ void pack_chunks(const GraphDesc &D, uint64_t m, u32x4 *__restrict__ dst) {
+    using PL = PackLayout<L>;
+    constexpr int P = PL::P, B = PL::B;
+    const gptr<uint32_t> colInd = to_global(D.colInd);
+    const uint32_t thr = D.thr;
+    const uint64_t chunks = (m + P - 1) / P;
+    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < chunks;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t d[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int i = 0; i < P; i++) {
+            const uint64_t q = (uint64_t)P * t + i;
+            const uint32_t u = q < m ? colInd[q] : 0u;
+            const uint32_t x = u >= thr ? u - thr : 0u;
+            const uint32_t f = PL::kAddr ? ((x >> 5) << 7) | (x & 31u) : x;
+            const int bit = B * i, k = bit / 32, sh = bit % 32;
+            d[k] |= f << sh;
+            if (sh + B > 32) d[k + 1] |= f >> (32 - sh);
+        }
+        dst[t] = u32x4{d[0], d[1], d[2], d[3]};
+    }
+}
+
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void k_split_pack(const GraphDesc *__restrict__ desc,
                                                        const SplitDesc *__restrict__ sd) {
     const GraphDesc &D = desc[0];
     const SplitDesc &S = sd[0];
     const uint64_t m = (uint64_t) static_cast<const OffT *>(D.rowPtr)[D.n];
-    const gptr<uint32_t> colInd = to_global(D.colInd);
-    u64x2 *__restrict__ dst = static_cast<u64x2 *>(S.rec2);
-    const uint32_t thr = D.thr;
-    const bool narrow = S.rows <= kPackNarrowRowsMax;
-    const uint64_t groups = (m + 11) / 12;
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < groups;
-         t += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t u[12];
-        if (12 * t + 12 <= m) {
-            const gptr<u32x4> src = reinterpret_cast<gptr<u32x4>>(colInd + 12 * t);
-            const u32x4 a = src[0], b = src[1], c = src[2];
-#pragma unroll
-            for (int z = 0; z < 4; z++) {
-                u[z] = a[z];
-                u[4 + z] = b[z];
-                u[8 + z] = c[z];
-            }
-        } else {
-#pragma unroll
-            for (int z = 0; z < 12; z++) u[z] = 12 * t + z < m ? colInd[12 * t + z] : thr;
-        }
-        dst[2 * t] = u64x2{pack3(u[0], u[1], u[2], thr, narrow), pack3(u[3], u[4], u[5], thr, narrow)};
-        dst[2 * t + 1] = u64x2{pack3(u[6], u[7], u[8], thr, narrow), pack3(u[9], u[10], u[11], thr, narrow)};
+    u32x4 *__restrict__ dst = static_cast<u32x4 *>(S.rec2);
+    switch (pack_layout(S.rows)) {  // block-uniform
+        case 0: pack_chunks<0>(D, m, dst); break;
+        case 1: pack_chunks<1>(D, m, dst); break;
+        case 2: pack_chunks<2>(D, m, dst); break;
+        default: pack_chunks<3>(D, m, dst); break;
     }
 }
 
@@ -889,38 +931,37 @@ __global__ __launch_bounds__(kBlock) void k_split_pack(const GraphDesc *__restri
 constexpr int kSplitLanes = 8;
 constexpr int kSplitChunks = 1;
 
-// Counts the hits of the six packed ids of chunk c of a tail.  base = 6c -
-// lo (mod 2^32): position 6c + z is in the tail iff base + z < len.  Every
-// packed field names a bit of the row bitmap, so all six are looked up (word
-// and bit straight from the fields: a field's bit offset is its low 5 bits),
-// the hits gathered as a 6-bit mask and the positions outside the tail
-// masked off at the end.  s_bits sits at LDS address 0, so a narrow field's
-// bits [5, 21) are the word's address.
-template <bool kNarrow>
-__device__ __forceinline__ uint32_t rows_chunk_hits(const uint32_t *s_bits, u64x2 ch, uint32_t base,
+// Counts the hits of the P packed ids of chunk c of a tail.  base = Pc - lo
+// (mod 2^32): position Pc + z is in the tail iff base + z < len.  Every
+// packed field names a bit of the row bitmap, so all P are looked up (word
+// and bit straight from the fields: a field's bit offset is its low 5
+// bits), the hits gathered as a P-bit mask and the positions outside the
+// tail masked off at the end.  s_bits sits at LDS address 0, so an address
+// field's bits [5, 21) are the word's address.
+template <int L>
+__device__ __forceinline__ uint32_t rows_chunk_hits(const uint32_t *s_bits, u32x4 ch, uint32_t base,
                                                     uint32_t len) {
-    const uint32_t w0 = (uint32_t)ch[0], w1 = (uint32_t)(ch[0] >> 32);
-    const uint32_t w2 = (uint32_t)ch[1], w3 = (uint32_t)(ch[1] >> 32);
-    // the six fields shifted so that bits [0, 21) hold the field
-    const uint32_t f[6] = {w0, __builtin_amdgcn_alignbit(w1, w0, kPackBits), w1 >> (2 * kPackBits - 32),
-                           w2, __builtin_amdgcn_alignbit(w3, w2, kPackBits), w3 >> (2 * kPackBits - 32)};
-    uint32_t wrd[6];
+    using PL = PackLayout<L>;
+    constexpr int P = PL::P, B = PL::B;
+    uint32_t f[P];
+    chunk_fields<B>(ch, f, std::make_integer_sequence<int, P>{});
+    uint32_t wrd[P];
 #pragma unroll
-    for (int z = 0; z < 6; z++) {
-        if (kNarrow) {
-            const uint32_t addr = __builtin_amdgcn_ubfe(f[z], 5u, kPackBits - 5u);
+    for (int z = 0; z < P; z++) {
+        if (PL::kAddr) {
+            const uint32_t addr = __builtin_amdgcn_ubfe(f[z], 5u, 16u);
             wrd[z] = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(s_bits) + addr);
         } else {
-            wrd[z] = s_bits[__builtin_amdgcn_ubfe(f[z], 5u, kPackBits - 5u)];
+            wrd[z] = s_bits[__builtin_amdgcn_ubfe(f[z], 5u, (uint32_t)B - 5u)];
         }
     }
     uint32_t hm = 0;
 #pragma unroll
-    for (int z = 0; z < 6; z++) hm |= __builtin_amdgcn_ubfe(wrd[z], f[z], 1u) << z;
+    for (int z = 0; z < P; z++) hm |= __builtin_amdgcn_ubfe(wrd[z], f[z], 1u) << z;
     // valid z: [lc, hc) with lc = -base when base is negative (the tail's
-    // first chunk), hc = min(len - base, 6)
+    // first chunk), hc = min(len - base, P)
     const uint32_t lc = base > 0xfffffff0u ? 0u - base : 0u;
-    const uint32_t rem = len - base, hc = rem < 6u ? rem : 6u;
+    const uint32_t rem = len - base, hc = rem < (uint32_t)P ? rem : (uint32_t)P;
     const uint32_t vm = ((1u << hc) - 1u) & (0xffffffffu << lc);
     return __builtin_popcount(hm & vm);
 }
@@ -932,7 +973,7 @@ __device__ __forceinline__ uint32_t rows_chunk_hits(const uint32_t *s_bits, u64x
 // ids (k_split_pack) by 16-byte chunks of six ids, kCh chunks per lane per
 // step and the next step's chunks loaded before this step's are counted;
 // then the row's bits cleared.
-template <typename OffT, int kWaves, int kLanes, int kCh, bool kNarrow>
+template <typename OffT, int kWaves, int kLanes, int kCh, int kLayout>
 __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *__restrict__ desc,
                                                                const SplitDesc *__restrict__ sd,
                                                                uint64_t *__restrict__ meta,
@@ -943,7 +984,7 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
     const SplitDesc &S = sd[0];
     const cptr<OffT> rowPtrU = to_const(static_cast<const OffT *>(D.rowPtr));
     const gptr<uint32_t> colInd = to_global(D.colInd);
-    const gptr<u64x2> packed = to_global(static_cast<const u64x2 *>(S.rec2));
+    const gptr<u32x4> packed = to_global(static_cast<const u32x4 *>(S.rec2));
     const cptr<OffT> rp_in = to_const(static_cast<const OffT *>(S.rp_in));
     const cptr<OffT> ieU = to_const(static_cast<const OffT *>(S.ie));
     const gptr<OffT> ieG = to_global(static_cast<const OffT *>(S.ie));
@@ -988,32 +1029,33 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *
                     hi = ieG[2 * (x0 + sub) + 1];
                 }
                 if (sl == 0) checked += (uint64_t)(hi - lo);
-                // chunk c holds positions [6c, 6c + 6); a tail is shorter
+                // chunk c holds positions [Pc, Pc + P); a tail is shorter
                 // than 2^32 positions
+                constexpr uint32_t P = PackLayout<kLayout>::P;
                 const uint32_t len = (uint32_t)(hi - lo);
-                const OffT c_end = hi / 6 + (hi % 6 != 0 ? 1 : 0);
-                OffT c = lo / 6 + sl;
-                u64x2 cur[kCh];
+                const OffT c_end = hi / P + (hi % P != 0 ? 1 : 0);
+                OffT c = lo / P + sl;
+                u32x4 cur[kCh];
 #pragma unroll
                 for (int q = 0; q < kCh; q++) {
-                    cur[q] = u64x2{0ull, 0ull};
+                    cur[q] = u32x4{0u, 0u, 0u, 0u};
                     if (c + (OffT)(q * kLanes) < c_end) cur[q] = packed[c + (OffT)(q * kLanes)];
                 }
                 while (c < c_end) {
                     const OffT cn = c + (OffT)(kCh * kLanes);
                     // the next chunks, or chunk c again past the tail's end
                     // (loaded unconditionally; a chunk past c_end is not counted)
-                    u64x2 nxt[kCh];
+                    u32x4 nxt[kCh];
 #pragma unroll
                     for (int q = 0; q < kCh; q++) {
                         const OffT cq = cn + (OffT)(q * kLanes);
                         nxt[q] = packed[cq < c_end ? cq : c];
                     }
-                    const uint32_t base = (uint32_t)(6 * c - lo);
+                    const uint32_t base = (uint32_t)(P * c - lo);
 #pragma unroll
                     for (int q = 0; q < kCh; q++)
                         if (q == 0 || c + (OffT)(q * kLanes) < c_end)
-                            h += rows_chunk_hits<kNarrow>(s_bits, cur[q], base + 6u * q * kLanes, len);
+                            h += rows_chunk_hits<kLayout>(s_bits, cur[q], base + P * q * kLanes, len);
 #pragma unroll
                     for (int q = 0; q < kCh; q++) cur[q] = nxt[q];
                     c = cn;
@@ -1268,24 +1310,29 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
 
 size_t split_rows_lds(uint32_t rows) { return (size_t)(((rows + 31) / 32 + 1) & ~1u) * sizeof(uint32_t) + sizeof(uint64_t); }
 
-template <int W, bool N>
+template <int W, int L>
 static const void *split_rows_fn(int ob) {
-    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, kSplitLanes, kSplitChunks, N>)
-                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, kSplitLanes, kSplitChunks, N>);
+    return ob == 8 ? reinterpret_cast<const void *>(&k_split_rows<uint64_t, W, kSplitLanes, kSplitChunks, L>)
+                   : reinterpret_cast<const void *>(&k_split_rows<uint32_t, W, kSplitLanes, kSplitChunks, L>);
 }
 
-template <bool N>
-static const void *split_rows_kernel_n(int ob, int wpb) {
+template <int L>
+static const void *split_rows_kernel_l(int ob, int wpb) {
     switch (wpb) {
-        case 4: return split_rows_fn<4, N>(ob);
-        case 16: return split_rows_fn<16, N>(ob);
-        default: return split_rows_fn<8, N>(ob);
+        case 4: return split_rows_fn<4, L>(ob);
+        case 16: return split_rows_fn<16, L>(ob);
+        default: return split_rows_fn<8, L>(ob);
     }
 }
 
 // The row kernel for a graph of `rows` top rows (the packing's layout).
 static const void *split_rows_kernel(int ob, int wpb, uint32_t rows) {
-    return rows <= kPackNarrowRowsMax ? split_rows_kernel_n<true>(ob, wpb) : split_rows_kernel_n<false>(ob, wpb);
+    switch (pack_layout(rows)) {
+        case 0: return split_rows_kernel_l<0>(ob, wpb);
+        case 1: return split_rows_kernel_l<1>(ob, wpb);
+        case 2: return split_rows_kernel_l<2>(ob, wpb);
+        default: return split_rows_kernel_l<3>(ob, wpb);
+    }
 }
 
 bool valid_split_waves(int wpb) { return wpb == 4 || wpb == 8 || wpb == 16; }
