@@ -711,6 +711,40 @@ __global__ __launch_bounds__(kBlock) void k_split_levelcols(const SplitDesc *__r
     (kLevel == 2 ? S.sub_tot : S.key_tot)[o] = s;
 }
 
+// The second level's column prefixes, as k_split_levelcols<2>, with one
+// block per part: the part's blocks are cut into kSplitBlock / kSplitFan2
+// segments whose column sums are taken in parallel first (a part spans
+// hundreds of blocks, which one thread per column walked in turn).
+__global__ __launch_bounds__(kSplitBlock) void k_split_levelcols2(const SplitDesc *__restrict__ sd) {
+    constexpr uint32_t kF = kSplitFan2, kSeg = kSplitBlock / kF;
+    __shared__ uint32_t s_sum[kSeg][kF];
+    const SplitDesc &S = sd[0];
+    const uint32_t b = blockIdx.x, i = threadIdx.x % kF, sg = threadIdx.x / kF;
+    const uint32_t cb0 = S.c2base[b], nb = S.c2base[b + 1] - cb0;
+    const uint32_t per = (nb + kSeg - 1) / kSeg;
+    const uint32_t s0 = cb0 + (sg * per < nb ? sg * per : nb), s1 = cb0 + ((sg + 1) * per < nb ? (sg + 1) * per : nb);
+    uint32_t *__restrict__ hist = S.hist2;
+    uint32_t sum = 0;
+#pragma unroll 8
+    for (uint32_t c = s0; c < s1; c++) sum += hist[(uint64_t)c * kF + i];
+    s_sum[sg][i] = sum;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+#pragma unroll
+    for (uint32_t z = 0; z < kSeg; z++) {
+        const uint32_t x = s_sum[z][i];
+        pre += z < sg ? x : 0u;
+        tot += x;
+    }
+#pragma unroll 8
+    for (uint32_t c = s0; c < s1; c++) {
+        const uint32_t x = hist[(uint64_t)c * kF + i];
+        hist[(uint64_t)c * kF + i] = pre;
+        pre += x;
+    }
+    if (sg == 0) S.sub_tot[b * kF + i] = tot;
+}
+
 // One block: sub_base = exclusive prefix of the sub-part totals (the
 // sub-parts in key order, so each part's sub-parts start at its own base)
 // and c3base = exclusive prefix of their third-level blocks.
@@ -880,45 +914,59 @@ __device__ __forceinline__ void chunk_fields(u32x4 d, uint32_t *f, std::integer_
 }
 
 // Packs colInd into S.rec2 (free once the third level has grouped its
-// records): a thread per chunk, P 4-byte loads and one 16-byte store;
-// positions past m pack as 0.  Writes ceil(m / P) * 16 bytes, within rec2's
-// cap * 2o bytes (and the 256-byte rounding of every workspace piece).
+// records): a block takes kBlock chunks at a time, loads their P * kBlock
+// ids into LDS with coalesced 4-byte loads, and each thread packs one chunk
+// from LDS (stride P words: no bank conflicts for odd P, two-way for P = 6
+// and 8) into one 16-byte store.  Positions past m pack as 0.  Writes
+// ceil(m / P) * 16 bytes, within rec2's cap * 2o bytes (and the 256-byte
+// rounding of every workspace piece).
 template <int L>
-__device__ __forceinline__ void pack_chunks(const GraphDesc &D, uint64_t m, u32x4 *__restrict__ dst) {
+__device__ __forceinline__ void pack_chunks(const GraphDesc &D, uint64_t m, u32x4 *__restrict__ dst,
+                                            uint32_t *s_ids) {
     using PL = PackLayout<L>;
     constexpr int P = PL::P, B = PL::B;
     const gptr<uint32_t> colInd = to_global(D.colInd);
     const uint32_t thr = D.thr;
     const uint64_t chunks = (m + P - 1) / P;
-    for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < chunks;
-         t += (uint64_t)gridDim.x * blockDim.x) {
-        uint32_t d[4] = {0u, 0u, 0u, 0u};
+    for (uint64_t g0 = (uint64_t)blockIdx.x * kBlock; g0 < chunks; g0 += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t q0 = g0 * P;
 #pragma unroll
-        for (int i = 0; i < P; i++) {
-            const uint64_t q = (uint64_t)P * t + i;
-            const uint32_t u = q < m ? colInd[q] : 0u;
-            const uint32_t x = u >= thr ? u - thr : 0u;
-            const uint32_t f = PL::kAddr ? ((x >> 5) << 7) | (x & 31u) : x;
-            const int bit = B * i, k = bit / 32, sh = bit % 32;
-            d[k] |= f << sh;
-            if (sh + B > 32) d[k + 1] |= f >> (32 - sh);
+        for (int z = 0; z < P; z++) {
+            const uint64_t q = q0 + (uint64_t)z * kBlock + threadIdx.x;
+            s_ids[z * kBlock + threadIdx.x] = q < m ? colInd[q] : 0u;
         }
-        dst[t] = u32x4{d[0], d[1], d[2], d[3]};
+        __syncthreads();
+        const uint64_t t = g0 + threadIdx.x;
+        if (t < chunks) {
+            uint32_t d[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int i = 0; i < P; i++) {
+                const uint32_t u = s_ids[P * threadIdx.x + i];
+                const uint32_t x = u >= thr ? u - thr : 0u;
+                const uint32_t f = PL::kAddr ? ((x >> 5) << 7) | (x & 31u) : x;
+                const int bit = B * i, k = bit / 32, sh = bit % 32;
+                d[k] |= f << sh;
+                if (sh + B > 32) d[k + 1] |= f >> (32 - sh);
+            }
+            dst[t] = u32x4{d[0], d[1], d[2], d[3]};
+        }
+        __syncthreads();
     }
 }
 
 template <typename OffT>
 __global__ __launch_bounds__(kBlock) void k_split_pack(const GraphDesc *__restrict__ desc,
                                                        const SplitDesc *__restrict__ sd) {
+    __shared__ uint32_t s_ids[8 * kBlock];
     const GraphDesc &D = desc[0];
     const SplitDesc &S = sd[0];
     const uint64_t m = (uint64_t) static_cast<const OffT *>(D.rowPtr)[D.n];
     u32x4 *__restrict__ dst = static_cast<u32x4 *>(S.rec2);
     switch (pack_layout(S.rows)) {  // block-uniform
-        case 0: pack_chunks<0>(D, m, dst); break;
-        case 1: pack_chunks<1>(D, m, dst); break;
-        case 2: pack_chunks<2>(D, m, dst); break;
-        default: pack_chunks<3>(D, m, dst); break;
+        case 0: pack_chunks<0>(D, m, dst, s_ids); break;
+        case 1: pack_chunks<1>(D, m, dst, s_ids); break;
+        case 2: pack_chunks<2>(D, m, dst, s_ids); break;
+        default: pack_chunks<3>(D, m, dst, s_ids); break;
     }
 }
 
@@ -1263,12 +1311,11 @@ hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipS
 hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, int ob, uint32_t chunks,
                                   uint32_t c2cap, uint32_t c3cap, uint32_t parts, uint32_t keys, uint64_t *key_part,
                                   uint64_t *meta, unsigned long long *counts, hipStream_t st) {
-    const uint32_t subs = parts * kSplitFan2;
     if (keys == 0 || chunks == 0 || c2cap == 0 || c3cap == 0 || parts == 0 || parts > kSplitPartsMax)
         return hipErrorInvalidValue;
     const dim3 blk(kSplitBlock), g1(chunks);
     const dim3 g2(c2cap < kLevelGridMax ? c2cap : kLevelGridMax), g3(c3cap < kLevelGridMax ? c3cap : kLevelGridMax);
-    const dim3 gs((subs + kBlock - 1) / kBlock), gk((keys + kBlock - 1) / kBlock);
+    const dim3 gk((keys + kBlock - 1) / kBlock);
     hipError_t e;
 #define WTC_SPLIT_LAUNCH(KERNEL, GRID, BLOCK, ...)                                             \
     do {                                                                                       \
@@ -1287,7 +1334,7 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
     else k_split_l1<uint32_t, true><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     WTC_SPLIT_LAUNCH(k_split_level_count, g2, blk, 2);
-    k_split_levelcols<2><<<gs, dim3(kBlock), 0, st>>>(sd);
+    k_split_levelcols2<<<dim3(parts), blk, 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_split_subscan<<<dim3(1), blk, 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
