@@ -26,7 +26,7 @@
 // Pipeline on one stream: k_split_partials, k_scan_partials (twice) and
 // k_split_scan (the full wedge prefix and the in-edge prefix);
 // k_split_range; the transposition in three counting-sort levels,
-// k_split_l1<count>, k_split_l1cols, k_split_l1bases, k_split_l1<scatter>
+// k_split_l1count, k_split_l1cols, k_split_l1bases, k_split_l1
 // (records bucketed by part of kSplitPart keys), k_split_level_count<2>,
 // k_split_levelcols<2>, k_split_subscan, k_split_level_scatter<2> (each
 // part's records bucketed by sub-part of kSplitSub keys),
@@ -238,6 +238,99 @@ __device__ __forceinline__ void clip_tail(OffT j, OffT e, OffT rb, uint64_t ws, 
     }
 }
 
+// First-level count, by positions: block b counts the in-edges of vertex
+// chunk b (clipped to the counted vertices) per part into hist1[b][part],
+// the same in-edges the scatter pass of k_split_l1 takes.  An in-edge is a
+// position of adj+(v) other than the row's last, so the block counts every
+// position of its chunk's rows (colInd read once, coalesced), takes back
+// each row's last position, and, when the counted range clips its end
+// vertices, takes back their in-edges whose clipped tail is empty.  A list
+// vertex's part comes from list_bid (no rowPtr read).
+template <typename OffT>
+__global__ __launch_bounds__(kSplitBlock) void k_split_l1count(const GraphDesc *__restrict__ desc,
+                                                               const SplitDesc *__restrict__ sd) {
+    __shared__ uint32_t s_cnt[kSplitPartsMax];
+    __shared__ uint32_t s_bid[kSplitPartsMax];
+    const GraphDesc &D = desc[0];
+    const SplitDesc &S = sd[0];
+    const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
+    const gptr<uint32_t> colInd = to_global(D.colInd);
+    const uint32_t n = D.n, thr = D.thr, parts = S.parts;
+    const uint32_t b = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) {
+        s_cnt[i] = 0u;
+        s_bid[i] = S.list_bid[i];
+    }
+    __syncthreads();
+    auto part_of = [&](uint32_t id) -> uint32_t {
+        if (id >= thr) return (id - thr) / kSplitPart;
+        uint32_t lo = 0, hi = parts;  // s_bid[lo] <= id < s_bid[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_bid[mid] <= id) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const bool clip = S.clip != 0;
+    const uint32_t v_lo = S.v_lo, v_hi = S.v_hi;
+    const cptr<uint64_t> ie_pre = to_const(static_cast<const uint64_t *>(S.ie_pre));
+    uint64_t c0 = uni32(chunk_start(ie_pre, n, b, S.chunks)), c1 = uni32(chunk_start(ie_pre, n, b + 1, S.chunks));
+    if (clip) {
+        c0 = c0 > v_lo ? c0 : v_lo;
+        c1 = c1 < (uint64_t)v_hi + 1 ? c1 : (uint64_t)v_hi + 1;
+    }
+    if (c0 < c1) {  // block-uniform
+        const OffT p0 = rowPtr[c0], p1 = rowPtr[c1];
+        constexpr uint32_t kIn = 8;  // loads in flight per thread
+        for (OffT q0 = p0 + threadIdx.x; q0 < p1; q0 += kIn * kSplitBlock) {
+            uint32_t id[kIn];
+#pragma unroll
+            for (uint32_t z = 0; z < kIn; z++) {
+                const OffT q = q0 + (OffT)(z * kSplitBlock);
+                id[z] = q < p1 ? colInd[q] : 0u;
+            }
+#pragma unroll
+            for (uint32_t z = 0; z < kIn; z++)
+                if (q0 + (OffT)(z * kSplitBlock) < p1) atomicAdd(&s_cnt[part_of(id[z])], 1u);
+        }
+        for (uint64_t v0 = c0 + threadIdx.x; v0 < c1; v0 += 4 * kSplitBlock) {
+            uint32_t id[4];
+            bool has[4];
+#pragma unroll
+            for (uint32_t z = 0; z < 4; z++) {
+                const uint64_t v = v0 + z * kSplitBlock;
+                OffT rb = 0, e = 0;
+                if (v < c1) {
+                    rb = rowPtr[v];
+                    e = rowPtr[v + 1];
+                }
+                has[z] = e > rb;
+                id[z] = has[z] ? colInd[e - 1] : 0u;
+            }
+#pragma unroll
+            for (uint32_t z = 0; z < 4; z++)
+                if (has[z]) atomicSub(&s_cnt[part_of(id[z])], 1u);
+        }
+        if (clip) {
+            const uint64_t w_lo = S.w_lo, w_hi = S.w_hi;
+            for (int z = 0; z < 2; z++) {
+                const uint64_t ve = z == 0 ? v_lo : v_hi;
+                if (ve < c0 || ve >= c1 || (z == 1 && v_hi == v_lo)) continue;  // block-uniform
+                const OffT rb = rowPtr[ve], e = rowPtr[ve + 1];
+                const uint64_t ws = S.ws_full[ve];
+                for (OffT j = rb + threadIdx.x; j + 1 < e; j += kSplitBlock) {
+                    OffT lo, hi;
+                    clip_tail<OffT>(j, e, rb, ws, w_lo, w_hi, lo, hi);
+                    if (lo >= hi) atomicSub(&s_cnt[part_of(colInd[j])], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) S.hist1[b * parts + i] = s_cnt[i];
+}
+
 // Vertices with at least this many in-edges are taken by a whole wave.
 constexpr uint32_t kSplitBigVertex = kWave;
 
@@ -250,19 +343,16 @@ constexpr uint32_t kSplitBigVertex = kWave;
 //    (a lane finds its vertex by a binary lifting over the scanned counts).
 // Every in-edge (its tail clipped at the ends of the counted range; empty
 // ones dropped) gets its key -- w - thr for a top row w, else rows +
-// rowPtr[w] / kSlabSpan -- and belongs to the part key / kSplitPart.
-// kScatter = false: the block counts its in-edges per part into
-// hist1[b][part] (a list vertex's part from list_bid, without reading its
-// row); kScatter = true (hist1 then holding each part's records in earlier
-// chunks, and part_base its first record): each in-edge takes the next
-// record slot of its part (an LDS cursor), and its key and record are
-// stored there.  Both passes see the same in-edges in the same chunks.
-template <typename OffT, bool kScatter>
+// rowPtr[w] / kSlabSpan -- and belongs to the part key / kSplitPart.  With
+// hist1 holding each part's records in earlier chunks (k_split_l1count,
+// then the offset scans) and part_base its first record, each in-edge takes
+// the next record slot of its part (an LDS cursor), and its key and record
+// are stored there.  The count saw the same in-edges in the same chunks.
+template <typename OffT>
 __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__restrict__ desc,
                                                           const SplitDesc *__restrict__ sd) {
     __shared__ uint32_t s_cnt[kSplitPartsMax];
     __shared__ uint64_t s_base[kSplitPartsMax];
-    __shared__ uint32_t s_bid[kSplitPartsMax];
     const GraphDesc &D = desc[0];
     const SplitDesc &S = sd[0];
     const OffT *__restrict__ rowPtr = static_cast<const OffT *>(D.rowPtr);
@@ -276,29 +366,13 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
     const uint32_t b = blockIdx.x;
     for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) {
         s_cnt[i] = 0u;
-        s_base[i] = kScatter ? S.part_base[i] + hist1[b * parts + i] : 0u;
-        s_bid[i] = S.list_bid[i];
+        s_base[i] = S.part_base[i] + hist1[b * parts + i];
     }
     __syncthreads();
 
     const uint32_t lane = threadIdx.x & (kWave - 1), wv = threadIdx.x >> 6;
 
-    // The part of a list vertex's key, from list_bid (no rowPtr read).
-    auto list_part = [&](uint32_t id) -> uint32_t {
-        uint32_t lo = 0, hi = parts;  // s_bid[lo] <= id < s_bid[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_bid[mid] <= id) lo = mid;
-            else hi = mid;
-        }
-        return lo;
-    };
-    // The count pass: the in-edge of an act lane counted in its part.
-    auto count = [&](bool act, uint32_t id) {
-        if (act) atomicAdd(&s_cnt[id >= thr ? (id - thr) / kSplitPart : list_part(id)], 1u);
-    };
-    // The scatter pass: the in-edge of an act lane stored at its part's next
-    // record slot.  The lanes of a wave that share a part take consecutive
+    // The in-edge of an act lane stored at its part's next record slot.  The lanes of a wave that share a part take consecutive
     // slots, and a chunk's few part cursors stay in the L2 cache, so the
     // stores leave in whole lines.
     auto take = [&](bool act, uint32_t key, OffT first, OffT hi, uint32_t aux) {
@@ -364,7 +438,7 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
                 for (int u = 0; u < 4; u++) {
                     const OffT j = jb + (OffT)(u * kWave) + lane;
                     rp[u] = rp1[u] = 0;
-                    if (kScatter && j < vend && ids[u] < thr) {
+                    if (j < vend && ids[u] < thr) {
                         rp[u] = rowPtr[ids[u]];
                         rp1[u] = rowPtr[ids[u] + 1];
                     }
@@ -374,12 +448,8 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
                     const OffT j = jb + (OffT)(u * kWave) + lane;
                     OffT lo = j + 1, hi = ve;
                     if (vedge && j < vend) clip_tail<OffT>(j, ve, vrb, vws, w_lo, w_hi, lo, hi);
-                    if (!kScatter) {
-                        count(j < vend && lo < hi, ids[u]);
-                    } else {
-                        const bool top = ids[u] >= thr;
-                        take(j < vend && lo < hi, key_of(ids[u], rp[u]), top ? lo : j, hi, aux_of(rp[u], rp1[u]));
-                    }
+                    const bool top = ids[u] >= thr;
+                    take(j < vend && lo < hi, key_of(ids[u], rp[u]), top ? lo : j, hi, aux_of(rp[u], rp1[u]));
                 }
             }
         }
@@ -415,21 +485,13 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
             }
             const bool act = idx < tot && lo < hi;
             const uint32_t id = act ? colInd[j] : 0u;
-            if (!kScatter) {
-                count(act, id);
-            } else {
-                OffT rp = 0, rp1 = 0;
-                if (act && id < thr) {
-                    rp = rowPtr[id];
-                    rp1 = rowPtr[id + 1];
-                }
-                take(act, key_of(id, rp), id >= thr ? lo : j, hi, aux_of(rp, rp1));
+            OffT rp = 0, rp1 = 0;
+            if (act && id < thr) {
+                rp = rowPtr[id];
+                rp1 = rowPtr[id + 1];
             }
+            take(act, key_of(id, rp), id >= thr ? lo : j, hi, aux_of(rp, rp1));
         }
-    }
-    if (!kScatter) {
-        __syncthreads();
-        for (uint32_t i = threadIdx.x; i < parts; i += kSplitBlock) hist1[b * parts + i] = s_cnt[i];
     }
 }
 
@@ -1323,15 +1385,15 @@ hipError_t launch_split_transpose(const GraphDesc *desc, const SplitDesc *sd, in
         else KERNEL<uint32_t, __VA_ARGS__><<<GRID, BLOCK, 0, st>>>(sd);                        \
         if ((e = hipGetLastError()) != hipSuccess) return e;                                   \
     } while (0)
-    if (ob == 8) k_split_l1<uint64_t, false><<<g1, blk, 0, st>>>(desc, sd);
-    else k_split_l1<uint32_t, false><<<g1, blk, 0, st>>>(desc, sd);
+    if (ob == 8) k_split_l1count<uint64_t><<<g1, blk, 0, st>>>(desc, sd);
+    else k_split_l1count<uint32_t><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_split_l1cols<<<dim3(parts), blk, 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     k_split_l1bases<<<dim3(1), dim3(kBlock), 0, st>>>(sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (ob == 8) k_split_l1<uint64_t, true><<<g1, blk, 0, st>>>(desc, sd);
-    else k_split_l1<uint32_t, true><<<g1, blk, 0, st>>>(desc, sd);
+    if (ob == 8) k_split_l1<uint64_t><<<g1, blk, 0, st>>>(desc, sd);
+    else k_split_l1<uint32_t><<<g1, blk, 0, st>>>(desc, sd);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     WTC_SPLIT_LAUNCH(k_split_level_count, g2, blk, 2);
     k_split_levelcols2<<<dim3(parts), blk, 0, st>>>(sd);
@@ -1407,10 +1469,11 @@ hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob,
     return hipLaunchKernel(fn, dim3(grid), dim3(wpb * kWave), args, split_rows_lds(rows), st);
 }
 
-// The slab kernel's block: 8 waves, 8 lanes per in-edge tail (list tails
-// are short: ~50 ids on R-MAT-24).
+// The slab kernel's block: 8 waves, 4 lanes per in-edge tail (list tails
+// are short: ~50 ids on R-MAT-24; 2.69 ms there at 4 lanes, 2.82 at 8,
+// 3.40 at 16).
 constexpr int kSlabWaves = 8;
-constexpr int kSlabLanes = 8;
+constexpr int kSlabLanes = 4;
 
 static const void *split_slabs_kernel(int ob) {
     return ob == 8 ? reinterpret_cast<const void *>(&k_split_slabs<uint64_t, kSlabWaves, kSlabLanes>)
