@@ -448,8 +448,9 @@ def tune_sweep(args):
             W = dg.count(12, min(262144, ((n + 31) // 32) * 32), stats=True)[1].wedges
         n32 = ((n + 31) // 32) * 32
         alens = [0] + [x for x in (2048, 8192, 32768, 131072, 262144) if x <= n32 and W > 0]
-        # bitmap rows past 2^18 only for the split count (one row in LDS, no a*a/8 bytes in HBM)
-        split_alens = [x for x in alens if x] + [x for x in (524288, 1048576) if x <= n32 and W > 0]
+        # bitmap rows past 2^18 only for the split count (one row in LDS, no a*a/8 bytes in HBM);
+        # 65,536 is the largest a whose packed ids take 16 bits
+        split_alens = sorted(set([x for x in alens if x] + [x for x in (65536, 524288, 1048576) if x <= n32 and W > 0]))
         best = best_walk = None
         base = wtc.Tuned(0, 0, wtc.KERNEL_WALK, 4, 4, 8)
         kernels = (wtc.KERNEL_SPLIT,) if big else (wtc.KERNEL_WALK, wtc.KERNEL_STAGED, wtc.KERNEL_SPLIT)

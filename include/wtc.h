@@ -235,9 +235,11 @@ int wtc_ctx_set_tuning(wtc_ctx *ctx, int probe_ways, uint32_t lds_chunk, int wav
  *                     per count; the wedges whose w is among the top-a
  *                     ranks (the reference's bitmap probes, tc.cu:262-272)
  *                     are counted against adj+(w) held as a bitmap row in
- *                     LDS, the rest (sorted-list probes, tc.cu:273-286)
- *                     against hash sets of adj+(w) in LDS.  No a*a/8-byte
- *                     bitmap in HBM; 32 or 48 bytes per edge of workspace
+ *                     LDS (their tails streamed from a copy of colInd
+ *                     packed to 16-21 bits per id), the rest (sorted-list
+ *                     probes, tc.cu:273-286) against hash sets of adj+(w)
+ *                     in LDS.  No a*a/8-byte bitmap in HBM; about 27 or 43
+ *                     bytes per edge of workspace (4- or 8-byte offsets)
  *                     instead.  Batches, a = 0, a row of a bits that does
  *                     not fit a block's LDS and colInd not 16-byte aligned:
  *                     the row walk.  Same count for every wedge range.
