@@ -37,6 +37,9 @@ def log(msg):
 
 
 def traffic_name(workload, spread, a, kernel="k_walk"):
+    # the split count's kernels do not depend on the spread
+    if kernel.startswith("k_split"):
+        return f"{kernel}_{workload}_a{a}_traffic.json"
     return f"{kernel}_{workload}_s{spread}_a{a}_traffic.json"
 
 
@@ -286,11 +289,13 @@ def reference_configs(iters=10):
     (R-MAT-20, s in {1,3,5,8} x a in {0,4096,8192,16384}).  Per (s, a): the
     median over `iters` calls of GPU_time's copy and exec (ms, tc.cu:76-79)
     and of the call's wall time, and the device time of the same count on a
-    resident copy of the graph."""
+    resident copy of the graph (with the autotuner's kernel and block shape
+    for the graph, as the drop-in takes them)."""
     ctx = wtc.Context(0)
 
     def run(g, params):
         rows = []
+        ctx.use_tuned(wtc.autotune_shape(g.n, g.m, g.wedges()))
         dg = ctx.upload(g)
         for s_, a_ in params:
             ref = wtc.tc_gpu(g, s_, a_)  # warm-up
