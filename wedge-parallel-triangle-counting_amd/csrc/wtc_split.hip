@@ -340,7 +340,8 @@ constexpr uint32_t kSplitBigVertex = kWave;
 //    wave in turn: the lanes load its ids 64 consecutive positions at a
 //    time, four loads in flight;
 //  * the others' in-edges are scanned and dealt to the lanes 64 at a time
-//    (a lane finds its vertex by a binary lifting over the scanned counts).
+//    (a lane finds its vertex by a binary lifting over the scanned counts),
+//    four such rounds together so that their loads are in flight at once.
 // Every in-edge (its tail clipped at the ends of the counted range; empty
 // ones dropped) gets its key -- w - thr for a top row w, else rows +
 // rowPtr[w] / kSlabSpan -- and belongs to the part key / kSplitPart.  With
@@ -464,33 +465,50 @@ __global__ __launch_bounds__(kSplitBlock) void k_split_l1(const GraphDesc *__res
         const uint32_t tot = uni32((uint32_t)__shfl((int)incl, kWave - 1, kWave));
         const uint32_t excl = incl - cnt;
         const bool any_edge = __ballot(edge) != 0;  // wave-uniform
-        for (uint32_t base = 0; base < tot; base += kWave) {
-            const uint32_t idx = base + lane;
-            // the lane's vertex: the rightmost whose first in-edge is <= idx
-            uint32_t pos = 0;
+        // kDeal rounds of 64 in-edges at a time, so that their colInd and
+        // rowPtr loads are in flight together
+        constexpr uint32_t kDeal = 4;
+        for (uint32_t base = 0; base < tot; base += kDeal * kWave) {
+            OffT j[kDeal], lo[kDeal], hi[kDeal];
+            bool act[kDeal];
 #pragma unroll
-            for (uint32_t st = kWave / 2; st > 0; st >>= 1) {
-                const uint32_t c = pos + st;
-                const uint32_t x = (uint32_t)__shfl((int)excl, (int)(c & (kWave - 1)), kWave);
-                if (c < (uint32_t)kWave && x <= idx) pos = c;
+            for (uint32_t u = 0; u < kDeal; u++) {
+                const uint32_t idx = base + u * kWave + lane;
+                // the lane's vertex: the rightmost whose first in-edge is <= idx
+                uint32_t pos = 0;
+#pragma unroll
+                for (uint32_t st = kWave / 2; st > 0; st >>= 1) {
+                    const uint32_t c = pos + st;
+                    const uint32_t x = (uint32_t)__shfl((int)excl, (int)(c & (kWave - 1)), kWave);
+                    if (c < (uint32_t)kWave && x <= idx) pos = c;
+                }
+                const OffT rbv = shfl_off(rb, pos);
+                j[u] = rbv + (OffT)(idx - (uint32_t)__shfl((int)excl, (int)pos, kWave));
+                const OffT ev = shfl_off(e, pos);
+                lo[u] = j[u] + 1;
+                hi[u] = ev;
+                if (any_edge) {
+                    const bool ed = __shfl((int)edge, (int)pos, kWave) != 0;
+                    const uint64_t wv0 = shfl_u64(wsv, pos);
+                    if (ed) clip_tail<OffT>(j[u], ev, rbv, wv0, w_lo, w_hi, lo[u], hi[u]);
+                }
+                act[u] = idx < tot && lo[u] < hi[u];
             }
-            const OffT rbv = shfl_off(rb, pos);
-            const OffT j = rbv + (OffT)(idx - (uint32_t)__shfl((int)excl, (int)pos, kWave));
-            const OffT ev = shfl_off(e, pos);
-            OffT lo = j + 1, hi = ev;
-            if (any_edge) {
-                const bool ed = __shfl((int)edge, (int)pos, kWave) != 0;
-                const uint64_t wv0 = shfl_u64(wsv, pos);
-                if (ed) clip_tail<OffT>(j, ev, rbv, wv0, w_lo, w_hi, lo, hi);
+            uint32_t id[kDeal];
+#pragma unroll
+            for (uint32_t u = 0; u < kDeal; u++) id[u] = act[u] ? colInd[j[u]] : 0u;
+            OffT rp[kDeal], rp1[kDeal];
+#pragma unroll
+            for (uint32_t u = 0; u < kDeal; u++) {
+                rp[u] = rp1[u] = 0;
+                if (act[u] && id[u] < thr) {
+                    rp[u] = rowPtr[id[u]];
+                    rp1[u] = rowPtr[id[u] + 1];
+                }
             }
-            const bool act = idx < tot && lo < hi;
-            const uint32_t id = act ? colInd[j] : 0u;
-            OffT rp = 0, rp1 = 0;
-            if (act && id < thr) {
-                rp = rowPtr[id];
-                rp1 = rowPtr[id + 1];
-            }
-            take(act, key_of(id, rp), id >= thr ? lo : j, hi, aux_of(rp, rp1));
+#pragma unroll
+            for (uint32_t u = 0; u < kDeal; u++)
+                take(act[u], key_of(id[u], rp[u]), id[u] >= thr ? lo[u] : j[u], hi[u], aux_of(rp[u], rp1[u]));
         }
     }
 }
