@@ -286,7 +286,7 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
         small_words = std::max(small_words, (uint32_t)words);
     }
     // The split count takes a single graph with a bitmap length whose colInd
-    // is 16-byte aligned (its row kernel loads four ids at a time); its top
+    // is 16-byte aligned (its slab kernel loads four ids at a time); its top
     // rows then need no bitmap in HBM.
     const bool split = G == 1 && m != nullptr && split_eligible(t, hd[0], m[0]) && (!t.split_needs_base || split_base);
     const uint32_t split_rows = split ? hd[0].bm_rows : 0;
