@@ -1327,36 +1327,59 @@ __global__ __launch_bounds__(kWave * kWaves) void k_split_slabs(const GraphDesc 
                     u32x4 b = {0u, 0u, 0u, 0u};
                     if (q1 < hi) b = *reinterpret_cast<gptr<u32x4>>(colInd + q1);
                     const uint32_t ua[8] = {a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+                    if (len != kSlabAuxLong) {
+                        // each id's first bucket read for all eight at once (one 16-byte LDS
+                        // read each; a position outside the tail reads the row's first
+                        // bucket and is not counted), the four slots compared without
+                        // branches; slots fill in order, so a free last slot ends the chain,
+                        // and only a full bucket without the id goes on to the next
+                        const u32x4 *tab4 = reinterpret_cast<const u32x4 *>(s_tab);
+                        uint32_t bk[8];
+                        bool in[8];
+                        u32x4 v[8];
+#pragma unroll
+                        for (int z = 0; z < 8; z++) {
+                            const OffT t = (z < 4 ? q : q1) + (OffT)(z & 3);
+                            in[z] = t >= lo && t < hi;
+                            bk[z] = st + slab_bucket(ua[z], len);
+                            v[z] = tab4[in[z] ? bk[z] : st];
+                        }
+#pragma unroll
+                        for (int z = 0; z < 8; z++) {
+                            const uint32_t u = ua[z];
+                            const bool f = (v[z][0] == u) | (v[z][1] == u) | (v[z][2] == u) | (v[z][3] == u);
+                            h += (in[z] & f) ? 1u : 0u;
+                            if (in[z] & !f & (v[z][3] != kSentinel)) {
+                                uint32_t b2 = bk[z] + 1 == st + len ? st : bk[z] + 1;
+                                for (;;) {
+                                    const u32x4 w2 = tab4[b2];
+                                    const bool f2 = (w2[0] == u) | (w2[1] == u) | (w2[2] == u) | (w2[3] == u);
+                                    h += f2 ? 1u : 0u;
+                                    if (f2 | (w2[3] == kSentinel)) break;
+                                    b2 = b2 + 1 == st + len ? st : b2 + 1;
+                                }
+                            }
+                        }
+                        continue;
+                    }
+                    // a long row: adj+(w) searched in HBM
 #pragma unroll
                     for (int z = 0; z < 8; z++) {
                         const OffT t = (z < 4 ? q : q1) + (OffT)(z & 3);
                         if (t < lo || t >= hi) continue;
                         const uint32_t u = ua[z];
-                        if (len != kSlabAuxLong) {
-                            // one 16-byte LDS read per bucket; the four slots compared without
-                            // branches; slots fill in order, so a free last slot ends the chain
-                            uint32_t bk = st + slab_bucket(u, len);
-                            for (;;) {
-                                const u32x4 v = reinterpret_cast<const u32x4 *>(s_tab)[bk];
-                                const bool f = (v[0] == u) | (v[1] == u) | (v[2] == u) | (v[3] == u);
-                                h += f ? 1u : 0u;
-                                if (f | (v[3] == kSentinel)) break;
-                                bk = bk + 1 == st + len ? st : bk + 1;
+                        OffT lb = gb, n_ = ge - gb;
+                        while (n_ > 0) {
+                            const OffT half = n_ >> 1;
+                            if (colInd[lb + half] < u) {
+                                lb += half + 1;
+                                n_ -= half + 1;
+                            } else {
+                                n_ = half;
                             }
-                        } else {
-                            OffT lb = gb, n_ = ge - gb;
-                            while (n_ > 0) {
-                                const OffT half = n_ >> 1;
-                                if (colInd[lb + half] < u) {
-                                    lb += half + 1;
-                                    n_ -= half + 1;
-                                } else {
-                                    n_ = half;
-                                }
-                            }
-                            gb = lb;  // the lane's next id is larger
-                            h += lb < ge && colInd[lb] == u ? 1u : 0u;
                         }
+                        gb = lb;  // the lane's next id is larger
+                        h += lb < ge && colInd[lb] == u ? 1u : 0u;
                     }
                 }
             }
