@@ -1511,8 +1511,8 @@ hipError_t launch_split_rows(const GraphDesc *desc, const SplitDesc *sd, int ob,
 }
 
 // The slab kernel's block: 8 waves, 4 lanes per in-edge tail (list tails
-// are short: ~50 ids on R-MAT-24; 2.69 ms there at 4 lanes, 2.82 at 8,
-// 3.40 at 16).
+// are short: ~50 ids on R-MAT-24; 2.53 ms there at 4 lanes, 2.71 at 2, 2.73
+// at 8).
 constexpr int kSlabWaves = 8;
 constexpr int kSlabLanes = 4;
 
