@@ -197,7 +197,8 @@ typedef struct {
     double prefix_ms;  /* wedge prefix scan + setup (split count: its two
                           prefixes and range map)                             */
     double bitmap_ms;  /* bitmap clear + build (split count: the counting
-                          sort of the in-edges by middle vertex)              */
+                          sort of the in-edges by middle vertex and the
+                          packing of the top tails' ids)                      */
     double count_ms;   /* counting kernel(s) (split count: row + slab kernels) */
     double total_ms;   /* first launch to count back on the host             */
     uint64_t wedges;   /* W counted                                          */
