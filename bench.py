@@ -740,9 +740,12 @@ def offsets_u64_leg(ctx, g, W, spread, a, steps, warmup, expect):
                          "bitmap_or_transposition": statistics.median(s["bitmap_ms"] for s in stats)},
            "count_frac_b_min": b_min8 / (count_ms / 1e3) / HBM_PEAK,
            "triangles": counts[0], "counts_equal_u32_offsets": all(c == expect for c in counts)}
-    if stats[-1]["split"]:  # the row kernel's roofline, as for the headline (2*8 B per in-edge record)
+    if stats[-1]["split"]:  # the row kernel's roofline, as for the headline
         rows_ms = statistics.median(s["rows_ms"] for s in stats)
-        dom_bytes = split_id_bytes(min(a, g.n)) * stats[-1]["top_wedges"] + 16.0 * stats[-1]["in_edges"]
+        # in-edge records of 2*4 B when every position fits 32 bits (the split
+        # count then runs on a 4-byte copy of the offsets), else 2*8 B
+        rec_bytes = 8.0 if g.m < 0xFFFFFFFF else 16.0
+        dom_bytes = split_id_bytes(min(a, g.n)) * stats[-1]["top_wedges"] + rec_bytes * stats[-1]["in_edges"]
         out["kernel_ms"]["split_rows"] = rows_ms
         out["kernel_ms"]["split_slabs"] = statistics.median(s["list_ms"] for s in stats)
         out["roofline"] = {"kernel": "k_split_rows", "achieved": dom_bytes / (rows_ms / 1e3) / 1e9, "unit": "GB/s",

@@ -42,12 +42,16 @@ def _ranges(W, rng, k=6):
     return out
 
 
-@pytest.mark.parametrize("ob", [4, 8])
-def test_split_whole_and_ranges_match_oracle(split_ctx, ob):
+@pytest.mark.parametrize("ob,narrow", [(4, True), (8, True), (8, False)])
+def test_split_whole_and_ranges_match_oracle(split_ctx, monkeypatch, ob, narrow):
     """R-MAT, Barabasi-Albert and Erdos-Renyi graphs at several bitmap
     lengths (a below n, a at n, a above n so that every vertex is a top
     row), whole graph and wedge ranges straddling vertices, against the
-    oracle's range count."""
+    oracle's range count.  With 8-byte offsets the split count runs on a
+    4-byte copy of them (every position fits 32 bits), or, with
+    $WTC_SPLIT_NARROW=0, on 8-byte offsets and records throughout."""
+    if not narrow:
+        monkeypatch.setenv("WTC_SPLIT_NARROW", "0")
     rng = np.random.default_rng(5)
     cases = [(_rmat(12, seed=3), (32, 1024, 4096)),
              (_rmat(10, ef=32, seed=4), (64, 1024)),
@@ -82,19 +86,21 @@ def test_split_complete_graph_closed_form(split_ctx):
     dg.free()
 
 
-def test_split_sum_of_parts(split_ctx):
+def test_split_sum_of_parts(split_ctx, monkeypatch):
     """A five-way range split of an R-MAT-14 graph sums to the whole count
-    at u32 and u64 offsets, and both equal the walk's count."""
+    at u32 offsets and at u64 offsets (narrowed to a 4-byte copy, and kept
+    8-byte with $WTC_SPLIT_NARROW=0), all equal to the oracle's count."""
     g = _rmat(14, seed=8)
     expect = O.tc_merge(g)
     W = g.wedges()
     cuts = [0, W // 9, W // 3, W // 2, (5 * W) // 6, W]
-    for ob in (4, 8):
+    for ob, narrow in ((4, "1"), (8, "1"), (8, "0")):
+        monkeypatch.setenv("WTC_SPLIT_NARROW", narrow)
         dg = split_ctx.upload(g, offset_bytes=ob)
         for a in (512, 2048):
             assert dg.count(12, a) == expect
             parts = [dg.count(12, a, cuts[i], cuts[i + 1]) for i in range(5)]
-            assert sum(parts) == expect, (ob, a, parts)
+            assert sum(parts) == expect, (ob, narrow, a, parts)
         dg.free()
 
 

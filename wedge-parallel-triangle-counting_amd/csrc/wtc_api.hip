@@ -5,6 +5,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -155,16 +156,30 @@ namespace {
 struct SplitLayout {
     size_t wsfull, hist1, part_base, c2base, hist2, sub_base, c3base, hist3, sub_tot, list_bid, slab_row, key_tot;
     size_t part_tot, key_part;
-    size_t recid, rec, recaux, recid2, rec2, recaux2, rpin, total;
+    size_t recid, rec, recaux, recid2, rec2, recaux2, rpin, rp32, total;
     uint32_t chunks, slabs, keys, parts, subs, c2cap, c3cap;
     uint64_t cap;
+    int ob;  // offset width of the split kernels (4 for a narrowed 8-byte CSR)
 };
+
+// Whether a split count of a graph with 8-byte offsets and m edges runs on a
+// 4-byte copy of its row offsets (every position fits 32 bits): records and
+// offsets of 8 bytes instead of 16 through the transposition.  On unless
+// $WTC_SPLIT_NARROW is 0.
+bool split_narrow(int ob, uint64_t m) {
+    if (ob != 8 || m >= 0xffffffffull) return false;
+    const char *e = std::getenv("WTC_SPLIT_NARROW");
+    return !(e && e[0] == '0');
+}
 
 // Slabs of the list rows of a graph of m edges.
 uint64_t split_slabs(uint64_t m) { return (m + kSlabSpan - 1) / kSlabSpan; }
 
-SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
+SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob_in) {
     SplitLayout L{};
+    const bool narrow = split_narrow(ob_in, m);
+    const int ob = narrow ? 4 : ob_in;
+    L.ob = ob;
     L.chunks = std::min<uint32_t>(kSplitChunksMax, std::max<uint32_t>(1, n / kSplitChunkVerts));
     L.slabs = (uint32_t)split_slabs(m);
     L.keys = rows + L.slabs;
@@ -201,6 +216,7 @@ SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
     L.rec2 = take((size_t)L.cap * pair);
     L.recaux2 = take((size_t)L.cap * sizeof(uint32_t));
     L.rpin = take(((size_t)L.keys + 1) * (size_t)ob);
+    L.rp32 = narrow ? take(((size_t)n + 1) * sizeof(uint32_t)) : 0;
     L.total = off;
     return L;
 }
@@ -209,8 +225,9 @@ SplitLayout split_plan(uint32_t n, uint64_t m, uint32_t rows, int ob) {
 
 size_t split_workspace_bytes(uint32_t n, uint64_t m, uint32_t rows, int ob) { return split_plan(n, m, rows, ob).total; }
 
-void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t rows, int ob) {
+int split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t rows, int ob, uint32_t **rp32) {
     const SplitLayout L = split_plan(n, m, rows, ob);
+    if (rp32) *rp32 = L.rp32 ? reinterpret_cast<uint32_t *>(base + L.rp32) : nullptr;
     sd->rows = rows;
     sd->keys = L.keys;
     sd->slabs = L.slabs;
@@ -244,6 +261,7 @@ void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t ro
     sd->rp_in = base + L.rpin;
     sd->ie = sd->rec;  // the third level writes the pairs over A's
     sd->ie_aux = sd->rec_aux;
+    return L.ob;
 }
 
 // A bitmap length whose row does not fit one block's LDS (or a graph whose
@@ -338,15 +356,24 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
         blk_off += prefix_blocks(d.n);
     }
     SplitDesc sd{};
+    int split_ob = ob;
+    const uint64_t *split_rp_src = nullptr;
+    uint32_t *split_rp_dst = nullptr;
     if (split) {
         char *base = split_base;
         if (!base) {
             WTC_HIPC(s.sp.ensure(split_workspace_bytes(hd[0].n, m[0], split_rows, ob)));
             base = s.sp.as<char>();
         }
-        split_layout(&sd, base, hd[0].n, m[0], split_rows, ob);
-        WTC_HIPC(split_rows_grid(ob, split_rows, t.split_wpb, t.num_cus, &s.rows_grid));
-        WTC_HIPC(split_slabs_grid(ob, t.num_cus, &s.slabs_grid));
+        uint32_t *rp32 = nullptr;
+        split_ob = split_layout(&sd, base, hd[0].n, m[0], split_rows, ob, &rp32);
+        if (rp32) {  // the split kernels read the 4-byte copy the count starts by making
+            split_rp_src = static_cast<const uint64_t *>(hd[0].rowPtr);
+            split_rp_dst = rp32;
+            hd[0].rowPtr = rp32;
+        }
+        WTC_HIPC(split_rows_grid(split_ob, split_rows, t.split_wpb, t.num_cus, &s.rows_grid));
+        WTC_HIPC(split_slabs_grid(split_ob, t.num_cus, &s.slabs_grid));
     }
     std::memcpy(s.h_desc.p, hd.data(), (size_t)G * sizeof(GraphDesc));
     if (split) std::memcpy(s.h_desc.as<char>() + split_off, &sd, sizeof(SplitDesc));
@@ -393,6 +420,10 @@ int stage_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, SetPlan *p
     plan->split_parts = sd.parts;
     plan->split_key_part = sd.key_part;
     plan->split_wpb = t.split_wpb;
+    plan->split_ob = split_ob;
+    plan->split_rp_src = split_rp_src;
+    plan->split_rp_dst = split_rp_dst;
+    plan->split_rp_n = (uint64_t)hd[0].n + 1;
     return WTC_OK;
 }
 
@@ -403,17 +434,19 @@ int launch_set(Slot &s, const SetPlan &p, bool timed) {
     uint64_t *tree = p.single ? s.tree.as<uint64_t>() : nullptr;
     if (p.split) {
         SplitDesc *sd = reinterpret_cast<SplitDesc *>(s.desc.as<char>() + p.split_off);
+        const int ob = p.split_ob;
         if (timed) WTC_HIPC(hipEventRecord(s.ev[0], s.stream));
-        WTC_HIPC(launch_split_prefix(ddesc, sd, p.ob, p.blocks, s.partials.as<uint64_t>(), s.stream));
-        WTC_HIPC(launch_split_range(ddesc, sd, p.ob, s.stream));
+        if (p.split_rp_dst) WTC_HIPC(launch_split_narrow_offsets(p.split_rp_src, p.split_rp_dst, p.split_rp_n, s.stream));
+        WTC_HIPC(launch_split_prefix(ddesc, sd, ob, p.blocks, s.partials.as<uint64_t>(), s.stream));
+        WTC_HIPC(launch_split_range(ddesc, sd, ob, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[1], s.stream));
-        WTC_HIPC(launch_split_transpose(ddesc, sd, p.ob, p.split_chunks, p.split_c2cap, p.split_c3cap, p.split_parts,
+        WTC_HIPC(launch_split_transpose(ddesc, sd, ob, p.split_chunks, p.split_c2cap, p.split_c3cap, p.split_parts,
                                         p.split_keys, p.split_key_part, meta, dcounts, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[2], s.stream));
-        WTC_HIPC(launch_split_rows(ddesc, sd, p.ob, p.split_rows, p.split_wpb, meta, dcounts, s.rows_grid,
+        WTC_HIPC(launch_split_rows(ddesc, sd, ob, p.split_rows, p.split_wpb, meta, dcounts, s.rows_grid,
                                    s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[4], s.stream));
-        WTC_HIPC(launch_split_slabs(ddesc, sd, p.ob, meta, dcounts, s.slabs_grid, s.stream));
+        WTC_HIPC(launch_split_slabs(ddesc, sd, ob, meta, dcounts, s.slabs_grid, s.stream));
         if (timed) WTC_HIPC(hipEventRecord(s.ev[3], s.stream));
         WTC_HIPC(hipMemcpyAsync(s.h_out.p, meta, (size_t)(kMetaSlots + p.G) * sizeof(uint64_t),
                                 hipMemcpyDeviceToHost, s.stream));

@@ -176,6 +176,12 @@ struct SetPlan {
     uint32_t split_rows = 0, split_keys = 0, split_chunks = 0, split_c2cap = 0, split_c3cap = 0, split_parts = 0;
     uint32_t split_wpb = 8;
     uint64_t *split_key_part = nullptr;  // device: the key-block prefix of the transposition
+    // offset width of the split kernels, and for a narrowed 8-byte CSR the
+    // row offsets copied to 4 bytes at the start of the count
+    int split_ob = 4;
+    const uint64_t *split_rp_src = nullptr;
+    uint32_t *split_rp_dst = nullptr;
+    uint64_t split_rp_n = 0;
 };
 // Sizes the slot's workspace for the set, assigns each graph its prefix and
 // bitmap (unless given), fills the pinned host descriptors and the plan.
@@ -200,7 +206,11 @@ int enqueue_set(Slot &s, Tuning &t, int ob, std::vector<GraphDesc> &hd, const ui
 // takes the workspace from split_base when given, else from the slot.
 bool split_eligible(const Tuning &t, const GraphDesc &d, uint64_t m);
 size_t split_workspace_bytes(uint32_t n, uint64_t m, uint32_t rows, int ob);
-void split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t rows, int ob);
+// split_layout returns the split kernels' offset width (4 for an 8-byte CSR
+// whose positions fit 32 bits, split_narrow) and, through rp32, where the
+// count copies its row offsets to 4 bytes (null when not narrowed).
+bool split_narrow(int ob, uint64_t m);
+int split_layout(SplitDesc *sd, char *base, uint32_t n, uint64_t m, uint32_t rows, int ob, uint32_t **rp32 = nullptr);
 // Waits for the slot (or, if it is done, does not wait) and reads results.
 int finish_set(Slot &s, uint64_t *counts, uint64_t *wedges, RunTimes *rt);
 // True when the slot's last enqueue has completed.

@@ -59,6 +59,8 @@ hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int o
                                uint64_t *partials, hipStream_t st);
 // The counted range's ends and the in-edge it cuts at the front.
 hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipStream_t st);
+// dst[i] = src[i] for i < count (8-byte row offsets below 2^32 to 4 bytes).
+hipError_t launch_split_narrow_offsets(const uint64_t *src, uint32_t *dst, uint64_t count, hipStream_t st);
 // The three-level counting sort of the in-edges by key (chunks first-level
 // blocks; c2cap and c3cap bound the second- and third-level blocks; parts
 // parts, keys keys; key_part = the SplitDesc's key_part, on the device);

@@ -1407,6 +1407,20 @@ hipError_t launch_split_prefix(const GraphDesc *desc, const SplitDesc *sd, int o
                           (const uint64_t *)pi);
 }
 
+// 8-byte row offsets that all fit 32 bits, copied to 4 bytes.
+__global__ __launch_bounds__(kBlock) void k_split_narrow_offsets(const uint64_t *__restrict__ src,
+                                                                uint32_t *__restrict__ dst, uint64_t count) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = (uint32_t)src[i];
+}
+
+hipError_t launch_split_narrow_offsets(const uint64_t *src, uint32_t *dst, uint64_t count, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    const uint64_t blocks = (count + kBlock - 1) / kBlock;
+    k_split_narrow_offsets<<<dim3((uint32_t)(blocks < 4096 ? blocks : 4096)), dim3(kBlock), 0, st>>>(src, dst, count);
+    return hipGetLastError();
+}
+
 hipError_t launch_split_range(const GraphDesc *desc, SplitDesc *sd, int ob, hipStream_t st) {
     return WTC_LAUNCH_OFF(ob, k_split_range, dim3(1), dim3(kSplitPartsMax), 0, st, desc, sd);
 }
