@@ -1064,8 +1064,8 @@ constexpr int kSplitChunks = 1;
 // packed field names a bit of the row bitmap, so all P are looked up (word
 // and bit straight from the fields: a field's bit offset is its low 5
 // bits), the hits gathered as a P-bit mask and the positions outside the
-// tail masked off at the end.  s_bits sits at LDS address 0, so an address
-// field's bits [5, 21) are the word's address.
+// tail masked off at the end.  An address field's bits [5, 21) are the byte
+// offset of its word in the bitmap.
 template <int L>
 __device__ __forceinline__ uint32_t rows_chunk_hits(const uint32_t *s_bits, u32x4 ch, uint32_t base,
                                                     uint32_t len) {
@@ -1098,15 +1098,15 @@ __device__ __forceinline__ uint32_t rows_chunk_hits(const uint32_t *s_bits, u32x
 // order) from one counter.  Per row of a tile: adj+(w) set as bits (u - thr)
 // in LDS, the tile's in-edges of the row dealt to the waves (kLanes lanes
 // per in-edge), and each tail streamed through the bitmap from the packed
-// ids (k_split_pack) by 16-byte chunks of six ids, kCh chunks per lane per
-// step and the next step's chunks loaded before this step's are counted;
-// then the row's bits cleared.
+// ids (k_split_pack) by 16-byte chunks of P ids (6 to 8, PackLayout), kCh
+// chunks per lane per step and the next step's chunks loaded before this
+// step's are counted; then the row's bits cleared.
 template <typename OffT, int kWaves, int kLanes, int kCh, int kLayout>
 __global__ __launch_bounds__(kWave * kWaves) void k_split_rows(const GraphDesc *__restrict__ desc,
                                                                const SplitDesc *__restrict__ sd,
                                                                uint64_t *__restrict__ meta,
                                                                unsigned long long *__restrict__ counts) {
-    // dynamic LDS only: the row bitmap at address 0, the tile slot after it
+    // dynamic LDS only: the row bitmap, then the tile slot
     extern __shared__ uint32_t s_bits[];
     const GraphDesc &D = desc[0];
     const SplitDesc &S = sd[0];
